@@ -1,0 +1,183 @@
+/* sem.h -- C-ABI of libsem_hip.so, the MI355X (gfx950) implementation of
+ * SchrodingerSEM's matrix-free spectral-element hot path.
+ *
+ * Conventions
+ *   - Every function returns an int status: SEM_OK (0) or a negative code;
+ *     sem_last_error() gives a thread-local message. No exception crosses the
+ *     ABI. Solver non-convergence is NOT an error: the call returns SEM_OK with
+ *     sem_solver_results.success = 0, as the reference returns
+ *     solver_results{success=false} (minres.hpp:102-106, pcg.hpp:91-95).
+ *   - Host arrays use the reference's element-local layout: element e's N x N
+ *     block at offset e*N*N, entry (i, j) at i + j*N (i along xi, j along eta;
+ *     column-major arma::Mat, types.hpp:9-10), N = p + 1.
+ *   - A mesh handle owns its device mirrors, its HIP stream and workspaces.
+ *     Calls on one handle are not re-entrant; separate handles may run
+ *     concurrently on separate host threads.
+ *   - FP64 everywhere at the ABI. Internally the solver may run its inner
+ *     operator applies in FP32 (SEM_PREC_MIXED), always judged by an FP64
+ *     residual.
+ */
+#ifndef SEM_H
+#define SEM_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum sem_status {
+    SEM_OK = 0,
+    SEM_EINVAL = -1, /* invalid argument */
+    SEM_EIO = -2,    /* mesh file missing or unreadable (load_mesh.hpp:93,102,118) */
+    SEM_EHIP = -3,   /* HIP runtime error (incl. no GPU) */
+    SEM_ENOMEM = -4,
+    SEM_EUNSUP = -5  /* unsupported configuration (e.g. degree above the compiled range) */
+};
+
+/* Element operators of the galerkin_op action (openmp_impl/galerkin.hpp:24-35). */
+enum sem_op_kind {
+    SEM_OP_LAPLACE = 0,   /* glaplace (glaplace.hpp:16-28), poisson.hpp:37-42 */
+    SEM_OP_HELMHOLTZ = 1, /* (W(u%c)W)%J + glaplace(u), helmholtz.hpp:38-46; coef = c */
+    SEM_OP_MASS = 2,      /* (W u W)%J, the gproj element op, galerkin.hpp:49-51 */
+    SEM_OP_SCHRO_RES = 3, /* Res lambda, schrodinger.hpp:185-207; 2 comps; coef = V, alpha */
+    SEM_OP_SCHRO_F = 4,   /* f lambda, schrodinger.hpp:163-183; 2 comps; coef = V */
+    SEM_OP_SCHRO_L = 5    /* L lambda glaplace(u) + (W(u%V)W)%J, schrodinger.hpp:37-43; coef = V */
+};
+
+/* Precision of the inner operator applies. */
+enum sem_precision {
+    SEM_PREC_FP64 = 0,  /* reference arithmetic */
+    SEM_PREC_MIXED = 1  /* FP32 inner MINRES inside FP64 iterative refinement */
+};
+
+/* solver_base.hpp:6-12 */
+typedef struct sem_solver_results {
+    int success;
+    int n_iter;
+    double residual;
+} sem_solver_results;
+
+/* ode_base.hpp:6-13 (max_iter, tol, dt, verbose) plus build-only knobs. */
+typedef struct sem_solver_opts {
+    int max_iter;
+    double tol;
+    double dt;
+    int verbose;
+    int precision; /* enum sem_precision; 0 = reference arithmetic */
+    int guess;     /* 0 = reference guesses (zero / previous psi); 1 = extrapolated; 2 = learned */
+} sem_solver_opts;
+
+/* Per-step observer (schrodinger.hpp:116,273,411): receives the unmasked
+ * psi (host copies, element-local layout); return nonzero to stop. */
+typedef int (*sem_step_cb)(const double* psi_re, const double* psi_im, void* user);
+
+/* Run statistics of the last stepper call on a mesh handle. */
+typedef struct sem_run_stats {
+    int steps;
+    int stage_solves;
+    long total_iters;        /* inner Krylov iterations, all stages */
+    long refinements;        /* FP64 refinement sweeps (mixed precision) */
+    double last_error_estimate; /* DIRK4S5 embedded estimate (schrodinger.hpp:262-263) */
+    double max_true_rel_residual; /* max over stage solves of ||b - A x|| / ||b||, FP64 */
+} sem_run_stats;
+
+typedef struct sem_mesh sem_mesh;
+
+const char* sem_last_error(void);
+int sem_version(void);
+int sem_device_info(char* name, int name_len, int* n_cu, long* hbm_bytes);
+
+/* ----- mesh setup: load_mesh<Mesh>(p, folder) + Mesh::compute_metrics()
+ * (load_mesh.hpp:76-209, openmp_impl/Mesh.hpp:70-75, Quad.hpp:27-55) ----- */
+int sem_mesh_create(int p, const double* xy, int n_nodes, const int* quads, int n_elements, sem_mesh** out);
+int sem_mesh_load(int p, const char* folder, sem_mesh** out);
+void sem_mesh_destroy(sem_mesh* m);
+/* out[0..5] = N, n_elements, n_nodes, n_edges, dof (openmp_impl/Mesh.hpp:84-87), elements per chunk */
+int sem_mesh_info(const sem_mesh* m, long* out);
+/* quad_rule x, w (gauss_lobatto.hpp:27-32) and D (derivative_matrix.hpp:12-38, column-major) */
+int sem_mesh_quadrature(const sem_mesh* m, double* x, double* w, double* D);
+/* physical (x, y) of every element-local GLL node, Quad::from_local_coo (Quad.hpp:57-62) */
+int sem_mesh_coords(const sem_mesh* m, double* X, double* Y);
+/* per-element metrics J, A, B, C in element-local layout (Quad.hpp:47-54) */
+int sem_mesh_metrics(const sem_mesh* m, double* J, double* A, double* B, double* C);
+/* Seeded structured n x n quad mesh on [x0,x1]x[y0,y1]: interior vertices
+ * moved by U(-jitter*h, jitter*h) per coordinate (SplitMix64 stream `seed`).
+ * Writes (n+1)^2 nodes (xy pairs) and n^2 counter-clockwise quads. No GPU. */
+int sem_mesh_structured(int n, double x0, double x1, double y0, double y1, double jitter, unsigned long long seed,
+                        double* xy, int* quads);
+/* Writes info.txt / coordinates.txt / elements.txt (the format load_mesh reads). */
+int sem_mesh_write(const char* folder, const double* xy, int n_nodes, const int* quads, int n_elements);
+
+/* ----- element operator and the Op action layer ----- */
+/* glaplace(u, Quad, D, quad_rule) on every element, no assembly (glaplace.hpp:16-28). */
+int sem_glaplace_apply(sem_mesh* m, const double* u, double* y);
+/* Element operator only (no unmask / assembly), any kind; coef/alpha as in sem_op_kind. */
+int sem_element_apply(sem_mesh* m, int kind, const double* coef, double alpha, const double* in, double* out);
+/* galerkin_op: unmask -> element op -> global_sum -> mask (galerkin.hpp:24-35);
+ * 2-component kinds take and return [re | im]. */
+int sem_galerkin_apply(sem_mesh* m, int kind, const double* coef, double alpha, const double* in, double* out);
+/* gproj (galerkin.hpp:42-57) */
+int sem_gproj(sem_mesh* m, const double* u, double* out);
+/* DSS pieces, in place: openmp_impl/unmask.hpp:43-55, global_sum.hpp:65-77, mask.hpp:16-28 */
+int sem_unmask(sem_mesh* m, double* a);
+int sem_global_sum(sem_mesh* m, double* a);
+int sem_mask(sem_mesh* m, double* a);
+/* unique-DOF dot product (openmp_impl/dot.hpp:11-61) */
+int sem_dot(sem_mesh* m, const double* a, const double* b, double* out);
+
+/* ----- PDE drivers ----- */
+/* poisson (openmp_impl/poisson.hpp:17-59): PCG on K u = M f; u in/out. */
+int sem_poisson(sem_mesh* m, double* u, const double* f, int max_iter, double tol, sem_solver_results* out);
+/* helmholtz (openmp_impl/helmholtz.hpp:18-64): MINRES on (K + M_c) u = M f. */
+int sem_helmholtz(sem_mesh* m, double* u, const double* c, const double* f, int max_iter, double tol,
+                  sem_solver_results* out);
+/* Schrodinger steppers (openmp_impl/schrodinger.hpp:22-128, 138-285, 294-423):
+ * psi in/out; *success mirrors the reference's bool return. opts->precision
+ * and opts->guess select the build-only modes; zero keeps reference behaviour. */
+int sem_schrodinger_bdf2(sem_mesh* m, double* re, double* im, const double* V, double T, const sem_solver_opts* opts,
+                         sem_step_cb cb, void* user, int* success);
+int sem_schrodinger_dirk4s3(sem_mesh* m, double* re, double* im, const double* V, double T, const sem_solver_opts* opts,
+                            sem_step_cb cb, void* user, int* success);
+int sem_schrodinger_dirk4s5(sem_mesh* m, double* re, double* im, const double* V, double T, const sem_solver_opts* opts,
+                            sem_step_cb cb, void* user, int* success);
+int sem_last_run_stats(const sem_mesh* m, sem_run_stats* out);
+
+/* Stepper session: the same three schemes with psi, V and the stage vectors
+ * resident on the GPU between calls (no counterpart in the reference, whose
+ * steppers run to T in one call). advance() runs the reference's while (t < T)
+ * loop, also stopping after max_steps steps when max_steps > 0. */
+typedef struct sem_stepper sem_stepper;
+int sem_stepper_create(sem_mesh* m, int scheme /* 0 bdf2, 1 dirk4s3, 2 dirk4s5 */, const double* re, const double* im,
+                       const double* V, const sem_solver_opts* opts, sem_stepper** out);
+int sem_stepper_advance(sem_stepper* s, double T, long max_steps, sem_step_cb cb, void* user, int* success);
+int sem_stepper_read(sem_stepper* s, double* re, double* im, double* t);
+int sem_stepper_stats(const sem_stepper* s, sem_run_stats* out);
+void sem_stepper_destroy(sem_stepper* s);
+/* One Schrodinger stage solve Res(alpha, V) z = b (schrodinger.hpp:223 etc.), z in/out. */
+int sem_stage_solve(sem_mesh* m, const double* V, double alpha, const double* b, double* z, int max_iter, double tol,
+                    int precision, sem_solver_results* out);
+
+/* ----- generic Krylov solvers over a tridiagonal operator y_i = lo x_{i-1} + di x_i + up x_{i+1},
+ * the arma::conv(u, {lo,di,up}, "same") operators of tests/pcg.cpp:13-16 and tests/minres.cpp:14-17,
+ * Euclidean dot; minres.hpp:19-107 and pcg.hpp:59-97 with IdentityPreconditioner. Host arrays. ----- */
+int sem_minres_tridiag(long n, double lo, double di, double up, const double* b, double* x, int maxit, double tol,
+                       sem_solver_results* out);
+int sem_pcg_tridiag(long n, double lo, double di, double up, const double* b, double* x, int maxit, double tol,
+                    sem_solver_results* out);
+
+/* ----- measurement hooks (device-resident data, HIP events on the handle's stream) ----- */
+/* Times `reps` launches of the element-operator kernel over device-resident
+ * random data (kind, precision 0 = FP64, 1 = FP32); returns mean microseconds
+ * per launch and the algorithmic bytes per launch. */
+int sem_time_element_op(sem_mesh* m, int kind, int precision, int reps, double* us_per_launch, double* bytes_per_launch);
+/* Live timing of the MINRES Lanczos-step kernel: when enabled, every launch of
+ * it is bracketed by HIP events on the handle's stream. which = 0 (FP64 kernel)
+ * or 1 (FP32 kernel); returns launches, summed microseconds and the
+ * algorithmic bytes of one launch (Res apply plus the fused w/x update). */
+int sem_set_kernel_timing(sem_mesh* m, int on);
+int sem_kernel_timing(const sem_mesh* m, int which, long* launches, double* total_us, double* bytes_per_launch);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SEM_H */

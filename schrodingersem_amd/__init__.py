@@ -1,0 +1,469 @@
+"""schrodingersem_amd -- Python mirror of SchrodingerSEM's hot-path API over
+libsem_hip.so (HIP, gfx950).
+
+Names and argument meaning follow the reference's namespace ``schro_omp``:
+``load_mesh``, ``Mesh.dof``, ``glaplace``, ``galerkin_op``, ``gproj``,
+``unmask``/``global_sum``/``mask``/``dot``, ``poisson``, ``helmholtz``,
+``schrodinger_bdf2``/``schrodinger_dirk4s3``/``schrodinger_dirk4s5``,
+``minres``/``pcg`` (on tridiagonal operators) and the ``SolverResults`` /
+``SolverOpts`` records (solver_base.hpp:6-12, ode_base.hpp:6-13).
+
+Element-local arrays are flat float64 arrays of length E*N*N in the
+reference layout: element e's block at e*N*N, entry (i, j) at i + j*N.
+Two-component operators take [re | im] (length 2*E*N*N).
+
+There is no CPU fallback: every call runs on the GPU through the shared
+library and raises if the library or the GPU is missing.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libsem_hip.so")
+
+OP_LAPLACE, OP_HELMHOLTZ, OP_MASS, OP_SCHRO_RES, OP_SCHRO_F, OP_SCHRO_L = range(6)
+PREC_FP64, PREC_MIXED = 0, 1
+GUESS_REFERENCE, GUESS_EXTRAPOLATED = 0, 1
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+_lp = C.POINTER(C.c_long)
+
+
+class SemError(RuntimeError):
+    """A libsem_hip call returned a negative status."""
+
+
+class _Results(C.Structure):
+    _fields_ = [("success", C.c_int), ("n_iter", C.c_int), ("residual", C.c_double)]
+
+
+class _Opts(C.Structure):
+    _fields_ = [("max_iter", C.c_int), ("tol", C.c_double), ("dt", C.c_double), ("verbose", C.c_int),
+                ("precision", C.c_int), ("guess", C.c_int)]
+
+
+class _Stats(C.Structure):
+    _fields_ = [("steps", C.c_int), ("stage_solves", C.c_int), ("total_iters", C.c_long),
+                ("refinements", C.c_long), ("last_error_estimate", C.c_double),
+                ("max_true_rel_residual", C.c_double)]
+
+
+STEP_CB = C.CFUNCTYPE(C.c_int, _dp, _dp, C.c_void_p)
+
+_lib = None
+
+
+def lib():
+    """Load libsem_hip.so (built in-tree by schrodingersem_amd.build)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise SemError("libsem_hip.so is not built; run python -m schrodingersem_amd.build")
+        L = C.CDLL(LIB_PATH)
+        sig = {
+            "sem_last_error": (C.c_char_p, []),
+            "sem_version": (C.c_int, []),
+            "sem_device_info": (C.c_int, [C.c_char_p, C.c_int, _ip, _lp]),
+            "sem_mesh_create": (C.c_int, [C.c_int, _dp, C.c_int, _ip, C.c_int, C.POINTER(C.c_void_p)]),
+            "sem_mesh_load": (C.c_int, [C.c_int, C.c_char_p, C.POINTER(C.c_void_p)]),
+            "sem_mesh_destroy": (None, [C.c_void_p]),
+            "sem_mesh_info": (C.c_int, [C.c_void_p, _lp]),
+            "sem_mesh_quadrature": (C.c_int, [C.c_void_p, _dp, _dp, _dp]),
+            "sem_mesh_coords": (C.c_int, [C.c_void_p, _dp, _dp]),
+            "sem_mesh_metrics": (C.c_int, [C.c_void_p, _dp, _dp, _dp, _dp]),
+            "sem_mesh_structured": (C.c_int, [C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
+                                              C.c_ulonglong, _dp, _ip]),
+            "sem_mesh_write": (C.c_int, [C.c_char_p, _dp, C.c_int, _ip, C.c_int]),
+            "sem_glaplace_apply": (C.c_int, [C.c_void_p, _dp, _dp]),
+            "sem_element_apply": (C.c_int, [C.c_void_p, C.c_int, _dp, C.c_double, _dp, _dp]),
+            "sem_galerkin_apply": (C.c_int, [C.c_void_p, C.c_int, _dp, C.c_double, _dp, _dp]),
+            "sem_gproj": (C.c_int, [C.c_void_p, _dp, _dp]),
+            "sem_unmask": (C.c_int, [C.c_void_p, _dp]),
+            "sem_global_sum": (C.c_int, [C.c_void_p, _dp]),
+            "sem_mask": (C.c_int, [C.c_void_p, _dp]),
+            "sem_dot": (C.c_int, [C.c_void_p, _dp, _dp, _dp]),
+            "sem_poisson": (C.c_int, [C.c_void_p, _dp, _dp, C.c_int, C.c_double, C.POINTER(_Results)]),
+            "sem_helmholtz": (C.c_int, [C.c_void_p, _dp, _dp, _dp, C.c_int, C.c_double, C.POINTER(_Results)]),
+            "sem_schrodinger_bdf2": (C.c_int, [C.c_void_p, _dp, _dp, _dp, C.c_double, C.POINTER(_Opts), STEP_CB,
+                                               C.c_void_p, _ip]),
+            "sem_schrodinger_dirk4s3": (C.c_int, [C.c_void_p, _dp, _dp, _dp, C.c_double, C.POINTER(_Opts),
+                                                  STEP_CB, C.c_void_p, _ip]),
+            "sem_schrodinger_dirk4s5": (C.c_int, [C.c_void_p, _dp, _dp, _dp, C.c_double, C.POINTER(_Opts),
+                                                  STEP_CB, C.c_void_p, _ip]),
+            "sem_last_run_stats": (C.c_int, [C.c_void_p, C.POINTER(_Stats)]),
+            "sem_stage_solve": (C.c_int, [C.c_void_p, _dp, C.c_double, _dp, _dp, C.c_int, C.c_double, C.c_int,
+                                          C.POINTER(_Results)]),
+            "sem_minres_tridiag": (C.c_int, [C.c_long, C.c_double, C.c_double, C.c_double, _dp, _dp, C.c_int,
+                                             C.c_double, C.POINTER(_Results)]),
+            "sem_pcg_tridiag": (C.c_int, [C.c_long, C.c_double, C.c_double, C.c_double, _dp, _dp, C.c_int,
+                                          C.c_double, C.POINTER(_Results)]),
+            "sem_time_element_op": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, _dp, _dp]),
+            "sem_set_kernel_timing": (C.c_int, [C.c_void_p, C.c_int]),
+            "sem_kernel_timing": (C.c_int, [C.c_void_p, C.c_int, _lp, _dp, _dp]),
+            "sem_stepper_create": (C.c_int, [C.c_void_p, C.c_int, _dp, _dp, _dp, C.POINTER(_Opts),
+                                             C.POINTER(C.c_void_p)]),
+            "sem_stepper_advance": (C.c_int, [C.c_void_p, C.c_double, C.c_long, STEP_CB, C.c_void_p, _ip]),
+            "sem_stepper_read": (C.c_int, [C.c_void_p, _dp, _dp, _dp]),
+            "sem_stepper_stats": (C.c_int, [C.c_void_p, C.POINTER(_Stats)]),
+            "sem_stepper_destroy": (None, [C.c_void_p]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise SemError("libsem_hip error %d: %s" % (rc, lib().sem_last_error().decode(errors="replace")))
+
+
+def _f64(a, n=None):
+    a = np.ascontiguousarray(a, dtype=np.float64).ravel()
+    if n is not None and a.size != n:
+        raise ValueError("expected %d values, got %d" % (n, a.size))
+    return a
+
+
+def _ptr(a):
+    return a.ctypes.data_as(_dp)
+
+
+class SolverResults:
+    """solver_results (solver_base.hpp:6-12)."""
+
+    def __init__(self, r):
+        self.success = bool(r.success)
+        self.n_iter = int(r.n_iter)
+        self.residual = float(r.residual)
+
+    def __repr__(self):
+        return "SolverResults(success=%s, n_iter=%d, residual=%.3e)" % (self.success, self.n_iter, self.residual)
+
+
+class SolverOpts:
+    """solver_opts (ode_base.hpp:6-13) plus the build-only precision / guess knobs."""
+
+    def __init__(self, max_iter=1000, tol=1e-8, dt=1e-3, verbose=False, precision=PREC_FP64,
+                 guess=GUESS_REFERENCE):
+        self.max_iter, self.tol, self.dt, self.verbose = max_iter, tol, dt, verbose
+        self.precision, self.guess = precision, guess
+
+    def _c(self):
+        return _Opts(int(self.max_iter), float(self.tol), float(self.dt), int(bool(self.verbose)),
+                     int(self.precision), int(self.guess))
+
+
+def structured_mesh(n, box=(0.0, 1.0, 0.0, 1.0), jitter=0.0, seed=0):
+    """Seeded n x n quad mesh; returns (xy[(n+1)^2, 2], quads[n^2, 4]). Host only."""
+    xy = np.zeros(((n + 1) ** 2, 2))
+    q = np.zeros((n * n, 4), dtype=np.int32)
+    _check(lib().sem_mesh_structured(int(n), box[0], box[1], box[2], box[3], float(jitter), int(seed), _ptr(xy),
+                                     q.ctypes.data_as(_ip)))
+    return xy, q
+
+
+def write_mesh(folder, xy, quads):
+    """Write the info.txt / coordinates.txt / elements.txt files load_mesh reads."""
+    xy = _f64(xy)
+    q = np.ascontiguousarray(quads, dtype=np.int32).ravel()
+    _check(lib().sem_mesh_write(folder.encode(), _ptr(xy), xy.size // 2, q.ctypes.data_as(_ip), q.size // 4))
+
+
+class Mesh:
+    """schro_omp::Mesh (openmp_impl/Mesh.hpp:16-88) resident on the GPU."""
+
+    def __init__(self, handle):
+        self._h = C.c_void_p(handle)
+        info = (C.c_long * 6)()
+        _check(lib().sem_mesh_info(self._h, info))
+        self.N, self.n_elements, self.n_nodes, self.n_edges, self._dof, self.chunk = (int(v) for v in info)
+        self.p = self.N - 1
+        x = np.zeros(self.N)
+        w = np.zeros(self.N)
+        D = np.zeros(self.N * self.N)
+        _check(lib().sem_mesh_quadrature(self._h, _ptr(x), _ptr(w), _ptr(D)))
+        self.x, self.w = x, w
+        self.D = D.reshape(self.N, self.N, order="F")
+        self.size = self.n_elements * self.N * self.N
+
+    @classmethod
+    def from_arrays(cls, p, xy, quads):
+        xy = _f64(xy)
+        q = np.ascontiguousarray(quads, dtype=np.int32).ravel()
+        h = C.c_void_p()
+        _check(lib().sem_mesh_create(int(p), _ptr(xy), xy.size // 2, q.ctypes.data_as(_ip), q.size // 4,
+                                     C.byref(h)))
+        return cls(h.value)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value and _lib is not None:
+            _lib.sem_mesh_destroy(h)
+            self._h = None
+
+    def dof(self):
+        return self._dof
+
+    def coords(self):
+        """(X, Y) physical coordinates of every element-local GLL node (Quad::from_local_coo)."""
+        X = np.zeros(self.size)
+        Y = np.zeros(self.size)
+        _check(lib().sem_mesh_coords(self._h, _ptr(X), _ptr(Y)))
+        return X, Y
+
+    def field(self, fn):
+        """Sample fn(x, y) at every element-local node (the loops of tests/poisson.cpp:20-26)."""
+        X, Y = self.coords()
+        return np.asarray(fn(X, Y), dtype=np.float64) * np.ones_like(X)
+
+    def metrics(self):
+        arrs = [np.zeros(self.size) for _ in range(4)]
+        _check(lib().sem_mesh_metrics(self._h, *[_ptr(a) for a in arrs]))
+        return tuple(arrs)
+
+    def last_run_stats(self):
+        s = _Stats()
+        _check(lib().sem_last_run_stats(self._h, C.byref(s)))
+        return {k: getattr(s, k) for k, _ in _Stats._fields_}
+
+
+def load_mesh(p, folder):
+    """load_mesh<Mesh>(p, folder) + compute_metrics() (load_mesh.hpp:76-209)."""
+    h = C.c_void_p()
+    _check(lib().sem_mesh_load(int(p), folder.encode(), C.byref(h)))
+    return Mesh(h.value)
+
+
+def _ncomp(kind):
+    return 2 if kind in (OP_SCHRO_RES, OP_SCHRO_F) else 1
+
+
+def glaplace(mesh, u):
+    """glaplace on every element, no assembly (glaplace.hpp:16-28)."""
+    u = _f64(u, mesh.size)
+    y = np.zeros_like(u)
+    _check(lib().sem_glaplace_apply(mesh._h, _ptr(u), _ptr(y)))
+    return y
+
+
+def element_op(kind, u, mesh, coef=None, alpha=0.0):
+    """The element-local operator of `kind` on every element, no assembly."""
+    nc = _ncomp(kind)
+    u = _f64(u, nc * mesh.size)
+    y = np.zeros_like(u)
+    keep = coef if coef is None else _f64(coef, mesh.size)
+    cp = _ptr(keep) if keep is not None else None
+    _check(lib().sem_element_apply(mesh._h, int(kind), cp, float(alpha), _ptr(u), _ptr(y)))
+    return y
+
+
+def galerkin_op(kind, u, mesh, coef=None, alpha=0.0):
+    """galerkin_op(op, u, mesh): unmask -> element op -> global_sum -> mask (galerkin.hpp:24-35)."""
+    nc = _ncomp(kind)
+    u = _f64(u, nc * mesh.size)
+    y = np.zeros_like(u)
+    keep = coef if coef is None else _f64(coef, mesh.size)
+    cp = _ptr(keep) if keep is not None else None
+    _check(lib().sem_galerkin_apply(mesh._h, int(kind), cp, float(alpha), _ptr(u), _ptr(y)))
+    return y
+
+
+def gproj(u, mesh):
+    """gproj (galerkin.hpp:42-57)."""
+    u = _f64(u, mesh.size)
+    y = np.zeros_like(u)
+    _check(lib().sem_gproj(mesh._h, _ptr(u), _ptr(y)))
+    return y
+
+
+def _inplace(fn, a, mesh):
+    b = _f64(a, mesh.size).copy()
+    _check(fn(mesh._h, _ptr(b)))
+    return b
+
+
+def unmask(a, mesh):
+    """unmask (openmp_impl/unmask.hpp:43-55); returns a new array."""
+    return _inplace(lib().sem_unmask, a, mesh)
+
+
+def global_sum(a, mesh):
+    """global_sum (openmp_impl/global_sum.hpp:65-77); returns a new array."""
+    return _inplace(lib().sem_global_sum, a, mesh)
+
+
+def mask(a, mesh):
+    """mask (openmp_impl/mask.hpp:16-28); returns a new array."""
+    return _inplace(lib().sem_mask, a, mesh)
+
+
+def dot(mesh, a, b):
+    """dot (openmp_impl/dot.hpp:11-61)."""
+    out = C.c_double()
+    _check(lib().sem_dot(mesh._h, _ptr(_f64(a, mesh.size)), _ptr(_f64(b, mesh.size)), C.byref(out)))
+    return out.value
+
+
+def poisson(u, f, mesh, max_iter, tol):
+    """poisson (openmp_impl/poisson.hpp:17-59); u (float64, length E*N*N) is updated in place."""
+    if not (isinstance(u, np.ndarray) and u.dtype == np.float64 and u.flags.c_contiguous and u.size == mesh.size):
+        raise ValueError("u must be a contiguous float64 array of mesh.size values")
+    r = _Results()
+    _check(lib().sem_poisson(mesh._h, _ptr(u), _ptr(_f64(f, mesh.size)), int(max_iter), float(tol), C.byref(r)))
+    return SolverResults(r)
+
+
+def helmholtz(u, c, f, mesh, max_iter, tol):
+    """helmholtz (openmp_impl/helmholtz.hpp:18-64); u updated in place."""
+    if not (isinstance(u, np.ndarray) and u.dtype == np.float64 and u.flags.c_contiguous and u.size == mesh.size):
+        raise ValueError("u must be a contiguous float64 array of mesh.size values")
+    r = _Results()
+    _check(lib().sem_helmholtz(mesh._h, _ptr(u), _ptr(_f64(c, mesh.size)), _ptr(_f64(f, mesh.size)),
+                               int(max_iter), float(tol), C.byref(r)))
+    return SolverResults(r)
+
+
+def _stepper(name, psi_real, psi_imag, potential, mesh, T, opts, callback):
+    for a in (psi_real, psi_imag):
+        if not (isinstance(a, np.ndarray) and a.dtype == np.float64 and a.flags.c_contiguous and a.size == mesh.size):
+            raise ValueError("psi arrays must be contiguous float64 arrays of mesh.size values")
+    n = mesh.size
+    if callback is None:
+        cb = STEP_CB()
+    else:
+        def _cb(re, im, _user):
+            return 1 if callback(np.ctypeslib.as_array(re, (n,)).copy(), np.ctypeslib.as_array(im, (n,)).copy()) else 0
+        cb = STEP_CB(_cb)
+    ok = C.c_int(0)
+    o = opts._c()
+    V = _f64(potential, n)
+    _check(getattr(lib(), name)(mesh._h, _ptr(psi_real), _ptr(psi_imag), _ptr(V), float(T), C.byref(o), cb, None,
+                                C.byref(ok)))
+    return bool(ok.value)
+
+
+def schrodinger_bdf2(psi_real, psi_imag, potential, mesh, T, opts, callback=None):
+    """schrodinger_bdf2 (openmp_impl/schrodinger.hpp:22-128); psi updated in place; returns success."""
+    return _stepper("sem_schrodinger_bdf2", psi_real, psi_imag, potential, mesh, T, opts, callback)
+
+
+def schrodinger_dirk4s3(psi_real, psi_imag, potential, mesh, T, opts, callback=None):
+    """schrodinger_dirk4s3 (openmp_impl/schrodinger.hpp:294-423)."""
+    return _stepper("sem_schrodinger_dirk4s3", psi_real, psi_imag, potential, mesh, T, opts, callback)
+
+
+def schrodinger_dirk4s5(psi_real, psi_imag, potential, mesh, T, opts, callback=None):
+    """schrodinger_dirk4s5 (openmp_impl/schrodinger.hpp:138-285)."""
+    return _stepper("sem_schrodinger_dirk4s5", psi_real, psi_imag, potential, mesh, T, opts, callback)
+
+
+class Stepper:
+    """A Schrodinger stepper whose state stays on the GPU between advance() calls.
+
+    scheme: 0 = bdf2, 1 = dirk4s3, 2 = dirk4s5 (openmp_impl/schrodinger.hpp).
+    """
+
+    def __init__(self, mesh, scheme, psi_real, psi_imag, potential, opts):
+        self.mesh = mesh
+        n = mesh.size
+        h = C.c_void_p()
+        self._opts = opts._c()
+        _check(lib().sem_stepper_create(mesh._h, int(scheme), _ptr(_f64(psi_real, n)), _ptr(_f64(psi_imag, n)),
+                                        _ptr(_f64(potential, n)), C.byref(self._opts), C.byref(h)))
+        self._h = h
+
+    def advance(self, T, max_steps=0, callback=None):
+        n = self.mesh.size
+        if callback is None:
+            cb = STEP_CB()
+        else:
+            def _cb(re, im, _user):
+                return 1 if callback(np.ctypeslib.as_array(re, (n,)).copy(),
+                                     np.ctypeslib.as_array(im, (n,)).copy()) else 0
+            cb = STEP_CB(_cb)
+        ok = C.c_int()
+        _check(lib().sem_stepper_advance(self._h, float(T), int(max_steps), cb, None, C.byref(ok)))
+        return bool(ok.value)
+
+    def read(self):
+        n = self.mesh.size
+        re = np.zeros(n)
+        im = np.zeros(n)
+        t = C.c_double()
+        _check(lib().sem_stepper_read(self._h, _ptr(re), _ptr(im), C.byref(t)))
+        return re, im, t.value
+
+    def stats(self):
+        s = _Stats()
+        _check(lib().sem_stepper_stats(self._h, C.byref(s)))
+        return {k: getattr(s, k) for k, _ in _Stats._fields_}
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value and _lib is not None:
+            _lib.sem_stepper_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+
+def set_kernel_timing(mesh, on):
+    _check(lib().sem_set_kernel_timing(mesh._h, int(bool(on))))
+
+
+def kernel_timing(mesh, which=0):
+    """(launches, total microseconds, algorithmic bytes per launch) of the MINRES step kernel."""
+    n = C.c_long()
+    us = C.c_double()
+    by = C.c_double()
+    _check(lib().sem_kernel_timing(mesh._h, int(which), C.byref(n), C.byref(us), C.byref(by)))
+    return n.value, us.value, by.value
+
+
+def stage_solve(mesh, V, alpha, b, z=None, max_iter=10000, tol=1e-10, precision=PREC_FP64):
+    """One Schrodinger stage solve Res(alpha, V) z = b; returns (z, SolverResults)."""
+    n = 2 * mesh.size
+    b = _f64(b, n)
+    z = np.zeros(n) if z is None else _f64(z, n).copy()
+    r = _Results()
+    _check(lib().sem_stage_solve(mesh._h, _ptr(_f64(V, mesh.size)), float(alpha), _ptr(b), _ptr(z), int(max_iter),
+                                 float(tol), int(precision), C.byref(r)))
+    return z, SolverResults(r)
+
+
+def minres_tridiag(lo, di, up, b, x, maxit, tol):
+    """minres (minres.hpp:19-107) on y_i = lo x_{i-1} + di x_i + up x_{i+1}; x updated in place."""
+    b = _f64(b)
+    r = _Results()
+    _check(lib().sem_minres_tridiag(b.size, lo, di, up, _ptr(b), _ptr(x), int(maxit), float(tol), C.byref(r)))
+    return SolverResults(r)
+
+
+def pcg_tridiag(lo, di, up, b, x, maxit, tol):
+    """pcg (pcg.hpp:59-97) on a tridiagonal operator; x updated in place."""
+    b = _f64(b)
+    r = _Results()
+    _check(lib().sem_pcg_tridiag(b.size, lo, di, up, _ptr(b), _ptr(x), int(maxit), float(tol), C.byref(r)))
+    return SolverResults(r)
+
+
+def time_element_op(mesh, kind, precision=PREC_FP64, reps=50):
+    """Mean microseconds per launch of the element-operator kernel and its algorithmic bytes."""
+    us = C.c_double()
+    by = C.c_double()
+    _check(lib().sem_time_element_op(mesh._h, int(kind), int(precision), int(reps), C.byref(us), C.byref(by)))
+    return us.value, by.value
+
+
+def device_info():
+    name = C.create_string_buffer(256)
+    ncu = C.c_int()
+    hbm = C.c_long()
+    _check(lib().sem_device_info(name, 256, C.byref(ncu), C.byref(hbm)))
+    return name.value.decode(), ncu.value, hbm.value

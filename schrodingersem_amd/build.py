@@ -1,0 +1,80 @@
+"""Build libsem_hip.so (gfx950) in-tree with hipcc, and the CPU oracle.
+
+Usage: python -m schrodingersem_amd.build [--jobs N]
+The shared library lands next to this file; nothing is installed.
+"""
+import argparse
+import os
+import subprocess
+import sys
+from concurrent.futures import ThreadPoolExecutor
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+OBJ = os.path.join(HERE, "_build")
+LIB = os.path.join(HERE, "libsem_hip.so")
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = "gfx950"
+# packed FP32 instructions are disabled for all device code
+COMMON = [
+    "--offload-arch=" + ARCH, "-O3", "-std=c++20", "-fPIC",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-I" + os.path.join(ROOT, "include"), "-I" + CSRC,
+    "-Wno-unused-result",
+]
+
+# (source, object, extra flags)
+UNITS = [
+    ("kernels_op.hip", "kernels_op_f64.o", []),
+    ("kernels_op.hip", "kernels_op_f32.o", ["-DSEM_OP_FP32"]),
+    ("kernels_vec.hip", "kernels_vec.o", []),
+    ("engine.hip", "engine.o", []),
+    ("mesh_host.cpp", "mesh_host.o", ["-x", "hip"]),
+]
+
+
+def _compile(unit):
+    src, obj, extra = unit
+    out = os.path.join(OBJ, obj)
+    cmd = [HIPCC] + COMMON + extra + ["-c", os.path.join(CSRC, src), "-o", out]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("hipcc failed for %s:\n%s\n%s" % (src, " ".join(cmd), r.stderr[-6000:]))
+    return out
+
+
+def build_lib(jobs=None):
+    os.makedirs(OBJ, exist_ok=True)
+    jobs = jobs or min(len(UNITS), os.cpu_count() or 4)
+    with ThreadPoolExecutor(max_workers=jobs) as ex:
+        objs = list(ex.map(_compile, UNITS))
+    tmp = LIB + ".tmp"
+    cmd = [HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("link failed:\n%s\n%s" % (" ".join(cmd), r.stderr[-4000:]))
+    os.replace(tmp, LIB)
+    return LIB
+
+
+def build_oracle():
+    r = subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle")], capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("oracle build failed:\n%s" % r.stderr[-4000:])
+    return os.path.join(ROOT, "oracle", "liboracle_sem.so")
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--jobs", type=int, default=None)
+    ap.add_argument("--skip-oracle", action="store_true")
+    a = ap.parse_args(argv)
+    print(build_lib(a.jobs))
+    if not a.skip_oracle:
+        print(build_oracle())
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,1532 @@
+// engine.hip -- device mesh, Krylov drivers, PDE drivers and the C-ABI of
+// libsem_hip.so. Host code: it sequences the kernels of kernels_op.hip and
+// kernels_vec.hip on the handle's stream; all vector work stays on the GPU.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <unordered_map>
+
+#include "sem_internal.hpp"
+#include "sem_kernels.hpp"
+
+namespace sem {
+
+static thread_local std::string g_error;
+void set_error(const std::string& s) { g_error = s; }
+
+// --------------------------------------------------------------- buffers --
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+};
+
+struct Results {
+    int success = 0;
+    int n_iter = 0;
+    double residual = 0;
+};
+
+struct KrylovOpts {
+    int maxit = 0;
+    double tol = 0;
+    double atol = 0;      // absolute term of the stop test (tol in the reference)
+    bool rel_to_b = false; // stop relative to ||b|| instead of ||b - A x0||
+};
+
+}  // namespace sem
+
+using namespace sem;
+
+struct sem_mesh {
+    HostMesh hm;
+    Layout L;
+    hipStream_t stream = nullptr;
+    double* geo = nullptr;
+    float* geo32 = nullptr;
+    uint8_t* obits = nullptr;
+    std::vector<uint8_t> h_obits;
+    // DSS lists (device layout positions)
+    int n_pairs = 0, n_corners = 0, n_bnd = 0;
+    int2* pairs = nullptr;
+    int* c_off = nullptr;
+    int* c_pos = nullptr;
+    int* bnd = nullptr;
+    int dss_nb = 1;
+    // D matrix
+    std::vector<double> D64;
+    std::vector<float> D32;
+    // scalars and partials
+    KrylovScalars* sc = nullptr;
+    KrylovScalars* h_sc = nullptr;  // pinned
+    double* partial = nullptr;
+    int n_partial = 0;
+    std::unordered_map<std::string, DevBuf> pool;
+    sem_run_stats stats{};
+    int batch = 8;
+    // live timing of the MINRES Lanczos-step kernel (index 0 FP64, 1 FP32)
+    bool timing = false;
+    std::vector<hipEvent_t> ev;
+    long kt_launches[2] = {0, 0};
+    double kt_us[2] = {0, 0};
+    double kt_bytes[2] = {0, 0};
+
+    ~sem_mesh()
+    {
+        if (stream) hipStreamSynchronize(stream);
+        for (hipEvent_t e : ev) hipEventDestroy(e);
+        for (auto& kv : pool) hipFree(kv.second.p);
+        hipFree(geo);
+        hipFree(geo32);
+        hipFree(obits);
+        hipFree(pairs);
+        hipFree(c_off);
+        hipFree(c_pos);
+        hipFree(bnd);
+        hipFree(sc);
+        hipFree(partial);
+        if (h_sc) hipHostFree(h_sc);
+        if (stream) hipStreamDestroy(stream);
+    }
+
+    template <typename T>
+    T* buf(const std::string& name, size_t n)
+    {
+        const size_t bytes = std::max<size_t>(n, 1) * sizeof(T);
+        auto it = pool.find(name);
+        if (it != pool.end() && it->second.bytes >= bytes) return (T*)it->second.p;
+        if (it != pool.end()) {
+            SEM_HIP(hipStreamSynchronize(stream));
+            hipFree(it->second.p);
+            pool.erase(it);
+        }
+        void* p = nullptr;
+        SEM_HIP(hipMalloc(&p, bytes));
+        pool[name] = DevBuf{p, bytes};
+        return (T*)p;
+    }
+
+    size_t len() const { return L.len; }
+    template <typename T>
+    const T* geo_t() const;
+    template <typename T>
+    const T* dmat() const;
+    void sync() { SEM_HIP(hipStreamSynchronize(stream)); }
+
+    // reference layout host -> device layout buffer (ncomp components)
+    void upload(double* dst, const double* src, int ncomp)
+    {
+        std::vector<double> tmp(L.len);
+        const size_t rl = (size_t)L.E * L.N * L.N;
+        for (int c = 0; c < ncomp; ++c) {
+            to_device_layout(L, src + c * rl, tmp.data());
+            SEM_HIP(hipMemcpyAsync(dst + c * L.len, tmp.data(), L.len * sizeof(double), hipMemcpyHostToDevice, stream));
+            sync();
+        }
+    }
+    void download(double* dst, const double* src, int ncomp)
+    {
+        std::vector<double> tmp(L.len);
+        const size_t rl = (size_t)L.E * L.N * L.N;
+        for (int c = 0; c < ncomp; ++c) {
+            SEM_HIP(hipMemcpyAsync(tmp.data(), src + c * L.len, L.len * sizeof(double), hipMemcpyDeviceToHost, stream));
+            sync();
+            from_device_layout(L, tmp.data(), dst + c * rl);
+        }
+    }
+    void read_scalars()
+    {
+        SEM_HIP(hipMemcpyAsync(h_sc, sc, sizeof(KrylovScalars), hipMemcpyDeviceToHost, stream));
+        sync();
+    }
+};
+
+template <>
+const double* sem_mesh::geo_t<double>() const { return geo; }
+template <>
+const float* sem_mesh::geo_t<float>() const { return geo32; }
+template <>
+const double* sem_mesh::dmat<double>() const { return D64.data(); }
+template <>
+const float* sem_mesh::dmat<float>() const { return D32.data(); }
+
+namespace sem {
+
+// -------------------------------------------------------- mesh creation --
+static void setup_device(sem_mesh* m)
+{
+    HostMesh& hm = m->hm;
+    const int N = hm.N;
+    m->L.init(N, hm.n_el);
+    const Layout& L = m->L;
+    SEM_HIP(hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking));
+    m->D64 = hm.D;
+    m->D32.assign(hm.D.begin(), hm.D.end());
+
+    // premultiplied geometry
+    const size_t rl = (size_t)hm.n_el * N * N;
+    std::vector<double> g(4 * L.len, 0.0), tmp(rl);
+    const std::vector<double>* src[4] = {&hm.A, &hm.B, &hm.C, &hm.J};
+    for (int f = 0; f < 4; ++f) {
+        for (int e = 0; e < hm.n_el; ++e)
+            for (int j = 0; j < N; ++j)
+                for (int i = 0; i < N; ++i) {
+                    size_t k = (size_t)e * N * N + i + (size_t)j * N;
+                    tmp[k] = (hm.w[i] * hm.w[j]) * (*src[f])[k];
+                }
+        to_device_layout(L, tmp.data(), g.data() + f * L.len);
+    }
+    std::vector<float> g32(g.begin(), g.end());
+    SEM_HIP(hipMalloc(&m->geo, g.size() * sizeof(double)));
+    SEM_HIP(hipMalloc(&m->geo32, g32.size() * sizeof(float)));
+    SEM_HIP(hipMemcpy(m->geo, g.data(), g.size() * sizeof(double), hipMemcpyHostToDevice));
+    SEM_HIP(hipMemcpy(m->geo32, g32.data(), g32.size() * sizeof(float), hipMemcpyHostToDevice));
+
+    // DSS lists
+    auto side_ij = [&](int s, int k, int& i, int& j) {
+        const int sm = (s == 1 || s == 4) ? 0 : N - 1;
+        if (s == 2 || s == 4) { i = sm; j = k; }
+        else { i = k; j = sm; }
+    };
+    static const int ci[4] = {0, 1, 1, 0}, cj[4] = {0, 0, 1, 1};
+    auto corner_pos = [&](int code) {
+        int e = code / 4, k = code % 4;
+        return (int)L.pos(e, ci[k] * (N - 1), cj[k] * (N - 1));
+    };
+    if (4 * L.len > (size_t)INT32_MAX) throw std::runtime_error("mesh too large for 32-bit DSS indices");
+    std::vector<int2> pairs;
+    std::vector<int> bnd, c_off{0}, c_pos;
+    m->h_obits.assign((size_t)L.nchunks * L.K, 0);
+    for (const EdgeRec& r : hm.edges) {
+        m->h_obits[r.el[0]] |= (uint8_t)(1u << (r.side[0] - 1));
+        for (int k = 1; k <= N - 2; ++k) {
+            int i0, j0;
+            side_ij(r.side[0], k, i0, j0);
+            int p0 = (int)L.pos(r.el[0], i0, j0);
+            if (r.type == BOUNDARY) {
+                bnd.push_back(p0);
+            } else {
+                int kk = r.start + (k - 1) * r.delta, i1, j1;
+                side_ij(std::abs(r.side[1]), kk, i1, j1);
+                pairs.push_back(make_int2(p0, (int)L.pos(r.el[1], i1, j1)));
+            }
+        }
+    }
+    for (int c = 0; c < hm.n_nodes; ++c) {
+        const int b = hm.conn_off[c], e = hm.conn_off[c + 1];
+        if (e > b) {
+            int first = hm.conn[b];
+            m->h_obits[first / 4] |= (uint8_t)(1u << (4 + first % 4));
+        }
+        if (hm.node_type[c] == BOUNDARY) {
+            for (int q = b; q < e; ++q) bnd.push_back(corner_pos(hm.conn[q]));
+        } else {
+            for (int q = b; q < e; ++q) c_pos.push_back(corner_pos(hm.conn[q]));
+            c_off.push_back((int)c_pos.size());
+        }
+    }
+    // every DSS position must address the padded device arrays
+    for (const int2& pr : pairs)
+        if (pr.x < 0 || pr.y < 0 || (size_t)pr.x >= L.len || (size_t)pr.y >= L.len)
+            throw std::runtime_error("internal error: DSS pair position out of range");
+    for (int v : bnd)
+        if (v < 0 || (size_t)v >= L.len) throw std::runtime_error("internal error: Dirichlet position out of range");
+    for (int v : c_pos)
+        if (v < 0 || (size_t)v >= L.len) throw std::runtime_error("internal error: corner position out of range");
+    m->n_pairs = (int)pairs.size();
+    m->n_corners = (int)c_off.size() - 1;
+    m->n_bnd = (int)bnd.size();
+    auto up = [&](auto*& dst, const auto& v) {
+        using E = typename std::decay_t<decltype(v)>::value_type;
+        SEM_HIP(hipMalloc(&dst, std::max<size_t>(v.size(), 1) * sizeof(E)));
+        if (!v.empty()) SEM_HIP(hipMemcpy(dst, v.data(), v.size() * sizeof(E), hipMemcpyHostToDevice));
+    };
+    up(m->pairs, pairs);
+    up(m->c_off, c_off);
+    up(m->c_pos, c_pos);
+    up(m->bnd, bnd);
+    up(m->obits, m->h_obits);
+    const long total = (long)m->n_pairs + m->n_corners + m->n_bnd;
+    m->dss_nb = (int)std::max<long>(1, (total + 255) / 256);
+
+    SEM_HIP(hipMalloc(&m->sc, sizeof(KrylovScalars)));
+    SEM_HIP(hipMemset(m->sc, 0, sizeof(KrylovScalars)));
+    SEM_HIP(hipHostMalloc(&m->h_sc, sizeof(KrylovScalars), hipHostMallocDefault));
+    m->n_partial = L.nchunks + m->dss_nb + 1024;
+    SEM_HIP(hipMalloc(&m->partial, (size_t)m->n_partial * sizeof(double)));
+}
+
+template <typename T>
+static DssArgs<T> dss_args(sem_mesh* m, T* y, int ncomp)
+{
+    DssArgs<T> d{};
+    d.y = y;
+    d.len = m->len();
+    d.ncomp = ncomp;
+    d.pairs = m->pairs;
+    d.n_pairs = m->n_pairs;
+    d.c_off = m->c_off;
+    d.c_pos = m->c_pos;
+    d.n_corners = m->n_corners;
+    d.bnd = m->bnd;
+    d.n_bnd = m->n_bnd;
+    d.post[0] = d.post[1] = 1;
+    d.sc = reinterpret_cast<const double*>(m->sc);
+    d.partial = m->partial;
+    return d;
+}
+
+// element op (raw ring entries) + DSS with Dirichlet zeroing: the consistent
+// form of mask(global_sum(op(unmask(in)))).
+template <typename T>
+static void apply_op(sem_mesh* m, int kind, const T* in, T* out, const T* cf, double alpha, double post = 1.0)
+{
+    OpArgs<T> a{};
+    a.in = in;
+    a.out = out;
+    a.geo = m->geo_t<T>();
+    a.cf = cf;
+    a.alpha = (T)alpha;
+    a.post = (T)post;
+    a.len = m->len();
+    a.nchunks = m->L.nchunks;
+    launch_element_op_d<T>(m->L.N, kind, a, m->dmat<T>(), m->stream);
+    DssArgs<T> d = dss_args<T>(m, out, op_ncomp(kind));
+    d.post[0] = d.post[1] = (T)post;
+    launch_dss<T>(DSS_SUM_ZB, d, m->stream);
+}
+
+template <typename T>
+static void dss(sem_mesh* m, int mode, T* y, int ncomp)
+{
+    DssArgs<T> d = dss_args<T>(m, y, ncomp);
+    launch_dss<T>(mode, d, m->stream);
+}
+
+template <typename T>
+static void lincomb(sem_mesh* m, T* out, int mode, std::initializer_list<std::pair<double, const T*>> terms, size_t n)
+{
+    LinComb<T> a{};
+    a.out = out;
+    a.mode = mode;
+    a.n = n;
+    int t = 0;
+    for (auto& pr : terms) {
+        a.c[t] = pr.first;
+        a.x[t] = pr.second;
+        ++t;
+    }
+    a.m = t;
+    launch_lincomb<T>(a, m->stream);
+}
+
+// owner-weighted dot into sc->red[slot] (device)
+template <typename T>
+static void wdot_dev(sem_mesh* m, const T* a, const T* b, int ncomp, int slot)
+{
+    launch_wdot<T>(a, b, m->len(), ncomp, m->L, m->obits, m->partial, m->stream);
+    launch_fin_sum(m->partial, m->L.nchunks, m->sc->red + slot, m->stream);
+}
+
+template <typename T>
+static double wdot_host(sem_mesh* m, const T* a, const T* b, int ncomp)
+{
+    wdot_dev<T>(m, a, b, ncomp, 7);
+    m->read_scalars();
+    return m->h_sc->red[7];
+}
+
+// ---------------------------------------------------------------- MINRES --
+// minres.hpp:19-107 with IdentityPreconditioner, three fused launches plus
+// two single-block scalar updates per Lanczos step. Vectors are T (FP64 or
+// FP32); every reduction and every recurrence scalar is FP64.
+template <typename T>
+static Results minres_fused(sem_mesh* m, int kind, const T* cf, double alpha, const T* b, T* x, bool x_zero,
+                            const KrylovOpts& o)
+{
+    const int nc = op_ncomp(kind);
+    const size_t n = (size_t)nc * m->len();
+    const std::string tp = sizeof(T) == 8 ? "d" : "f";
+    T* R[3] = {m->buf<T>("mr_R0" + tp, n), m->buf<T>("mr_R1" + tp, n), m->buf<T>("mr_R2" + tp, n)};
+    T* W[3] = {m->buf<T>("mr_W0" + tp, n), m->buf<T>("mr_W1" + tp, n), m->buf<T>("mr_W2" + tp, n)};
+    hipStream_t s = m->stream;
+    if (x_zero) {
+        launch_fill<T>(x, T(0), n, s);
+        launch_cast<T, T>(R[0], b, n, s);
+    } else {
+        T* Ax = m->buf<T>("mr_Ax" + tp, n);
+        apply_op<T>(m, kind, x, Ax, cf, alpha);
+        lincomb<T>(m, R[0], 0, {{1.0, b}, {-1.0, Ax}}, n);  // r1 = b - A(x)
+    }
+    launch_fill<T>(W[0], T(0), n, s);
+    launch_fill<T>(W[2], T(0), n, s);
+    wdot_dev<T>(m, R[0], R[0], nc, 0);
+    launch_minres_init(m->sc, 0, o.maxit, o.tol, s);
+    if (o.rel_to_b) {  // b_norm = ||b|| (fair stop test for nonzero guesses)
+        wdot_dev<T>(m, b, b, nc, 1);
+    }
+    if (o.rel_to_b || o.atol != o.tol) {
+        // stop test phibar < tol * b_ref + atol (minres.hpp:92 has b_ref = ||r0||, atol = tol)
+        m->read_scalars();
+        if (o.rel_to_b) m->h_sc->b_norm = std::sqrt(m->h_sc->red[1]);
+        m->h_sc->atol = o.atol;
+        SEM_HIP(hipMemcpyAsync(m->sc, m->h_sc, sizeof(KrylovScalars), hipMemcpyHostToDevice, s));
+    }
+    const int nch = m->L.nchunks;
+    const int ti = sizeof(T) == 8 ? 0 : 1;
+    if (m->timing) {
+        while ((int)m->ev.size() < 2 * m->batch) {
+            hipEvent_t e;
+            SEM_HIP(hipEventCreate(&e));
+            m->ev.push_back(e);
+        }
+        // algorithmic bytes of one launch: the operator apply (inputs, geometry, outputs)
+        // plus the fused w/x update (r1, w1, w2, x in; w, x out), per component
+        const double EN2 = (double)m->L.E * m->L.N * m->L.N;
+        const double op_vals = (kind == K_RES) ? 9 : (kind == K_HELMHOLTZ ? 7 : 5);
+        m->kt_bytes[ti] = (op_vals + 6.0 * nc) * sizeof(T) * EN2;
+    }
+    int i = 1;
+    bool done = o.maxit < 1;
+    while (!done) {
+        int launched = 0;
+        const int i0 = i;
+        for (int b_ = 0; b_ < m->batch && i <= o.maxit + 1; ++b_, ++i) {
+            MinresOpArgs<T> a{};
+            a.r2 = R[(i - 1) % 3];
+            a.r1 = R[(i + 1) % 3];
+            a.y = R[i % 3];
+            a.wa = W[i % 3];
+            a.wb = W[(i + 1) % 3];
+            a.wn = W[(i + 2) % 3];
+            a.x = x;
+            a.geo = m->geo_t<T>();
+            a.cf = cf;
+            a.alpha = (T)alpha;
+            a.len = m->len();
+            a.nchunks = nch;
+            a.partial = m->partial;
+            a.sc = reinterpret_cast<const double*>(m->sc);
+            if (m->timing) SEM_HIP(hipEventRecord(m->ev[2 * b_], s));
+            launch_minres_op_d<T>(m->L.N, kind, a, m->dmat<T>(), s);
+            if (m->timing) SEM_HIP(hipEventRecord(m->ev[2 * b_ + 1], s));
+            ++launched;
+            DssArgs<T> d = dss_args<T>(m, R[i % 3], nc);
+            d.r1 = R[(i + 1) % 3];
+            d.r2 = R[(i - 1) % 3];
+            d.partial = m->partial + nch;
+            launch_dss<T>(DSS_MINRES, d, s);
+            launch_minres_fin_alpha(m->sc, m->partial, nch + m->dss_nb, s);
+            launch_minres_p2<T>(R[i % 3], R[(i - 1) % 3], m->len(), nc, m->L, m->obits,
+                                reinterpret_cast<const double*>(m->sc), m->partial, s);
+            launch_minres_fin_beta(m->sc, m->partial, nch, s);
+        }
+        m->read_scalars();
+        if (m->timing) {
+            // launches that ran a full Lanczos step (later ones in the batch exit early)
+            for (int b_ = 0; b_ < launched && i0 + b_ <= m->h_sc->iter; ++b_) {
+                float ms = 0;
+                SEM_HIP(hipEventElapsedTime(&ms, m->ev[2 * b_], m->ev[2 * b_ + 1]));
+                m->kt_us[ti] += 1000.0 * ms;
+                m->kt_launches[ti] += 1;
+            }
+        }
+        done = m->h_sc->done != 0 || i > o.maxit + 1;
+    }
+    if (m->h_sc->p3_pending) {
+        const int j = m->h_sc->iter;  // last Lanczos step
+        launch_minres_flush<T>(R[(j + 2) % 3], W[(j + 1) % 3], W[(j + 2) % 3], W[j % 3], x, n, m->sc, s);
+        m->read_scalars();
+    }
+    Results r;
+    r.success = m->h_sc->success;
+    r.n_iter = r.success ? m->h_sc->iter : o.maxit + 1;
+    if (o.maxit < 1) r.n_iter = 1;
+    r.residual = m->h_sc->phibar;
+    return r;
+}
+
+// ------------------------------------------------------------------- PCG --
+// pcg.hpp:59-97 (identity preconditioner), FP64.
+static Results pcg_fused(sem_mesh* m, const double* b, double* x, bool x_zero, int maxit, double tol)
+{
+    const size_t n = m->len();
+    hipStream_t s = m->stream;
+    double* r = m->buf<double>("pcg_r", n);
+    double* p = m->buf<double>("pcg_p", n);
+    double* ap = m->buf<double>("pcg_ap", n);
+    if (x_zero) {
+        launch_fill<double>(x, 0.0, n, s);
+        launch_cast<double, double>(r, b, n, s);
+    } else {
+        apply_op<double>(m, K_LAPLACE, x, ap, nullptr, 0);
+        lincomb<double>(m, r, 0, {{1.0, b}, {-1.0, ap}}, n);
+    }
+    wdot_dev<double>(m, b, b, 1, 0);
+    wdot_dev<double>(m, r, r, 1, 1);
+    launch_pcg_init(m->sc, maxit, tol, s);
+    const int nch = m->L.nchunks;
+    int i = 0;
+    bool done = false;
+    while (!done) {
+        for (int b_ = 0; b_ < m->batch && i < maxit + 1; ++b_, ++i) {
+            PcgOpArgs<double> a{};
+            a.r = r;
+            a.p = p;
+            a.ap = ap;
+            a.geo = m->geo;
+            a.len = n;
+            a.partial = m->partial;
+            a.sc = reinterpret_cast<const double*>(m->sc);
+            launch_pcg_op_d<double>(m->L.N, a, nch, m->D64.data(), s);
+            DssArgs<double> d = dss_args<double>(m, ap, 1);
+            d.r2 = p;
+            d.partial = m->partial + nch;
+            launch_dss<double>(DSS_PCG, d, s);
+            launch_pcg_fin_alpha(m->sc, m->partial, nch + m->dss_nb, s);
+            launch_pcg_upd(x, r, p, ap, n, m->L, m->obits, reinterpret_cast<const double*>(m->sc), m->partial, s);
+            launch_pcg_fin_rho(m->sc, m->partial, nch, s);
+        }
+        m->read_scalars();
+        done = m->h_sc->done != 0 || i >= maxit + 1;
+    }
+    Results res;
+    res.success = m->h_sc->success;
+    res.n_iter = m->h_sc->iter;
+    res.residual = m->h_sc->resid;
+    return res;
+}
+
+// ------------------------------------------------ mixed precision solve --
+// FP64 iterative refinement around an FP32 MINRES: r = b - A x in FP64, the
+// correction A d = r is solved in FP32 (FP64 recurrence scalars and dots),
+// x += d in FP64. Stops on the true FP64 residual, ||r|| < tol*||r0|| + atol
+// (or tol*||b|| + atol when rel_to_b), the reference's test applied to the
+// true residual instead of MINRES's recurrence estimate.
+static Results solve_mixed(sem_mesh* m, int kind, const double* cf64, const float* cf32, double alpha, const double* b,
+                           double* x, bool x_zero, const KrylovOpts& o, double inner_tol)
+{
+    const int nc = op_ncomp(kind);
+    const size_t n = (size_t)nc * m->len();
+    hipStream_t s = m->stream;
+    double* r = m->buf<double>("mx_r", n);
+    double* Ax = m->buf<double>("mx_Ax", n);
+    float* r32 = m->buf<float>("mx_r32", n);
+    float* d32 = m->buf<float>("mx_d32", n);
+    if (x_zero) {
+        launch_fill<double>(x, 0.0, n, s);
+        launch_cast<double, double>(r, b, n, s);
+    } else {
+        apply_op<double>(m, kind, x, Ax, cf64, alpha);
+        lincomb<double>(m, r, 0, {{1.0, b}, {-1.0, Ax}}, n);
+    }
+    double r0 = std::sqrt(wdot_host<double>(m, r, r, nc));
+    double ref = r0;
+    if (o.rel_to_b) ref = std::sqrt(wdot_host<double>(m, b, b, nc));
+    const double target = o.tol * ref + o.atol;
+    Results res;
+    res.residual = r0;
+    long inner_total = 0;
+    int sweeps = 0;
+    double rn = r0;
+    while (!(rn < target) && inner_total < o.maxit && sweeps < 50) {
+        launch_cast<float, double>(r32, r, n, s);
+        KrylovOpts io;
+        io.maxit = (int)std::max<long>(1, o.maxit - inner_total);
+        // relative reduction of this sweep; never ask FP32 for more than it can give
+        io.tol = std::max(inner_tol, 0.5 * target / std::max(rn, 1e-300));
+        io.atol = 0;
+        Results ir = minres_fused<float>(m, kind, cf32, alpha, r32, d32, true, io);
+        inner_total += ir.n_iter;
+        launch_cast<double, float>(Ax, d32, n, s);
+        lincomb<double>(m, x, 2, {{1.0, Ax}}, n);  // x += d
+        apply_op<double>(m, kind, x, Ax, cf64, alpha);
+        lincomb<double>(m, r, 0, {{1.0, b}, {-1.0, Ax}}, n);
+        rn = std::sqrt(wdot_host<double>(m, r, r, nc));
+        ++sweeps;
+        m->stats.refinements++;
+    }
+    res.success = rn < target;
+    res.n_iter = (int)inner_total;
+    res.residual = rn;
+    return res;
+}
+
+// ---------------------------------------------------------- PDE drivers --
+struct Field {
+    double* c64 = nullptr;
+    float* c32 = nullptr;
+};
+
+// premultiplied coefficient field gJ * f on the device, from a host field in
+// the reference layout
+static Field make_field(sem_mesh* m, const double* host, const std::string& name)
+{
+    Field f;
+    const size_t n = m->len();
+    f.c64 = m->buf<double>(name + "_64", n);
+    f.c32 = m->buf<float>(name + "_32", n);
+    double* tmp = m->buf<double>("field_tmp", n);
+    m->upload(tmp, host, 1);
+    launch_mul<double>(f.c64, m->geo + 3 * n, tmp, n, m->stream);
+    launch_cast<float, double>(f.c32, f.c64, n, m->stream);
+    return f;
+}
+
+// masked output of the reference's poisson/helmholtz: owner copies carry the
+// solution, non-owner copies keep the caller's input (poisson.hpp:56)
+static void merge_owner(sem_mesh* m, const double* xdev, double* u_host)
+{
+    const Layout& L = m->L;
+    const int N = L.N;
+    std::vector<double> xr((size_t)L.E * N * N);
+    m->download(xr.data(), xdev, 1);
+    for (int e = 0; e < L.E; ++e) {
+        const unsigned bits = m->h_obits[e];
+        for (int j = 0; j < N; ++j)
+            for (int i = 0; i < N; ++i) {
+                const bool ri = (i == 0 || i == N - 1), rj = (j == 0 || j == N - 1);
+                bool own = true;
+                if (ri && rj) own = (bits >> (4 + ((i == 0) ? ((j == 0) ? 0 : 3) : ((j == 0) ? 1 : 2)))) & 1u;
+                else if (j == 0) own = bits & 1u;
+                else if (i == N - 1) own = (bits >> 1) & 1u;
+                else if (j == N - 1) own = (bits >> 2) & 1u;
+                else if (i == 0) own = (bits >> 3) & 1u;
+                const size_t k = L.ref(e, i, j);
+                if (own) u_host[k] = xr[k];
+            }
+    }
+}
+
+// Schrodinger stepper session: the state psi, the potential and the stage
+// vectors stay resident on the GPU between advance() calls.
+struct Stepper {
+    sem_mesh* m = nullptr;
+    int scheme = 1;  // 0 bdf2, 1 dirk4s3, 2 dirk4s5
+    sem_solver_opts opts{};
+    Field V;
+    size_t n = 0;  // 2 * len
+    std::vector<void*> owned;
+    double* p = nullptr;
+    double* v = nullptr;
+    double* q = nullptr;
+    std::vector<double*> z, zprev, zprev2;
+    bool have_prev = false, have_prev2 = false;
+    double t = 0;
+    double pnorm = 0;          // dirk4s5 (schrodinger.hpp:214)
+    double* ps[2] = {nullptr, nullptr};  // bdf2 history (schrodinger.hpp:85)
+    int nps = 0;
+    double alpha_bdf = 0;
+    sem_run_stats stats{};
+
+    double* alloc()
+    {
+        void* d = nullptr;
+        SEM_HIP(hipMalloc(&d, n * sizeof(double)));
+        owned.push_back(d);
+        return (double*)d;
+    }
+    ~Stepper()
+    {
+        if (m) hipStreamSynchronize(m->stream);
+        for (void* d : owned) hipFree(d);
+    }
+};
+
+// v = dt * f(q) (schrodinger.hpp:163-183, :221): f's DSS sums scaled by dt
+static void stage_rhs(Stepper& st, const double* q, double* out)
+{
+    apply_op<double>(st.m, K_F, q, out, st.V.c64, 0.0, st.opts.dt);
+}
+
+static Results solve_res(Stepper& st, double alpha, const double* b, double* x, bool zero, bool rel_to_b)
+{
+    sem_mesh* m = st.m;
+    KrylovOpts ko;
+    ko.maxit = st.opts.max_iter;
+    ko.tol = st.opts.tol;
+    ko.atol = st.opts.tol;
+    ko.rel_to_b = rel_to_b;
+    Results r;
+    const long ref0 = m->stats.refinements;
+    if (st.opts.precision == SEM_PREC_MIXED)
+        r = solve_mixed(m, K_RES, st.V.c64, st.V.c32, alpha, b, x, zero, ko, 2e-5);
+    else
+        r = minres_fused<double>(m, K_RES, st.V.c64, alpha, b, x, zero, ko);
+    st.stats.refinements += m->stats.refinements - ref0;
+    st.stats.stage_solves++;
+    st.stats.total_iters += r.n_iter;
+    return r;
+}
+
+static Results stage_solve(Stepper& st, double alpha, const double* b, double* z, int stage)
+{
+    bool zero = true;
+    if (st.opts.guess == 1 && st.have_prev) {  // extrapolation from the previous steps' same stage
+        if (st.have_prev2)
+            lincomb<double>(st.m, z, 1, {{2.0, st.zprev[stage]}, {-1.0, st.zprev2[stage]}}, st.n);
+        else
+            launch_cast<double, double>(z, st.zprev[stage], st.n, st.m->stream);
+        zero = false;
+    }
+    return solve_res(st, alpha, b, z, zero, !zero);
+}
+
+static void record_history(Stepper& st, int nstages)
+{
+    if (st.opts.guess != 1) return;
+    for (int k = 0; k < nstages; ++k) {
+        if (st.have_prev) std::swap(st.zprev2[k], st.zprev[k]);
+        launch_cast<double, double>(st.zprev[k], st.z[k], st.n, st.m->stream);
+    }
+    st.have_prev2 = st.have_prev;
+    st.have_prev = true;
+}
+
+static bool do_callback(Stepper& st, sem_step_cb cb, void* user)
+{
+    if (!cb) return false;
+    sem_mesh* m = st.m;
+    const size_t rl = (size_t)m->L.E * m->L.N * m->L.N;
+    std::vector<double> h(2 * rl);
+    m->download(h.data(), st.p, 2);
+    return cb(h.data(), h.data() + rl, user) != 0;
+}
+
+// VerboseTracker (verbose.hpp:25-44)
+static void progress(double t, double T, double dt, const double* err)
+{
+    int prec = (int)(2 - std::log10(dt));
+    if (prec < 0) prec = 0;
+    const int w = 30;
+    int q = (int)(w * (t / T));
+    if (q > w) q = w;
+    if (q < 0) q = 0;
+    std::string bar(w + 2, ' ');
+    bar[0] = '[';
+    for (int i = 0; i < q; ++i) bar[i + 1] = '-';
+    bar[q + 1] = '>';
+    bar[w + 1] = ']';
+    if (err)
+        std::printf("%.*f :: %s :: error ~ %e\r", prec, t, bar.c_str(), *err);
+    else
+        std::printf("%.*f :: %s\r", prec, t, bar.c_str());
+    std::fflush(stdout);
+}
+
+static Stepper* stepper_create(sem_mesh* m, int scheme, const double* re, const double* im, const double* V,
+                               const sem_solver_opts* o)
+{
+    if (!(o->dt > 0)) throw std::invalid_argument("schrodinger: dt must be positive");
+    if (o->guess < 0 || o->guess > 1) throw std::invalid_argument("schrodinger: unsupported guess mode");
+    if (o->precision < 0 || o->precision > 1) throw std::invalid_argument("schrodinger: unsupported precision");
+    std::unique_ptr<Stepper> st(new Stepper);
+    st->m = m;
+    st->scheme = scheme;
+    st->opts = *o;
+    st->n = 2 * m->len();
+    st->V = make_field(m, V, "V");
+    // the field buffers of make_field live in the mesh pool under a fixed
+    // name; copy them into the session so several sessions can coexist
+    {
+        double* c64 = st->alloc();
+        SEM_HIP(hipMemcpyAsync(c64, st->V.c64, m->len() * sizeof(double), hipMemcpyDeviceToDevice, m->stream));
+        float* c32 = (float*)st->alloc();
+        SEM_HIP(hipMemcpyAsync(c32, st->V.c32, m->len() * sizeof(float), hipMemcpyDeviceToDevice, m->stream));
+        st->V.c64 = c64;
+        st->V.c32 = c32;
+    }
+    st->p = st->alloc();
+    st->v = st->alloc();
+    st->q = st->alloc();
+    const int nz = scheme == 0 ? 0 : (scheme == 1 ? 3 : 5);
+    for (int k = 0; k < nz; ++k) st->z.push_back(st->alloc());
+    if (o->guess == 1)
+        for (int k = 0; k < nz; ++k) {
+            st->zprev.push_back(st->alloc());
+            st->zprev2.push_back(st->alloc());
+        }
+    if (scheme == 0) {
+        st->ps[0] = st->alloc();
+        st->ps[1] = st->alloc();
+        st->alpha_bdf = o->dt;
+    }
+    const size_t rl = (size_t)m->L.E * m->L.N * m->L.N;
+    std::vector<double> h(2 * rl);
+    std::copy(re, re + rl, h.begin());
+    std::copy(im, im + rl, h.begin() + rl);
+    m->upload(st->p, h.data(), 2);
+    // psi arrives unmasked; unmask copies the owner values, the device form is consistent
+    dss<double>(m, DSS_UNMASK, st->p, 2);
+    if (scheme == 2) st->pnorm = std::sqrt(wdot_host<double>(m, st->p, st->p, 2));
+    return st.release();
+}
+
+// one DIRK4S3 step, schrodinger.hpp:376-406
+static bool step_dirk4s3(Stepper& st)
+{
+    static const double gamma = std::cos(M_PI / 18.0) / std::sqrt(3.0) + 0.5;
+    static const double delta = 1.0 / 6.0 / std::pow(2 * gamma - 1, 2);
+    sem_mesh* m = st.m;
+    const double alpha = gamma * st.opts.dt;
+    const size_t n = st.n;
+    double *p = st.p, *v = st.v, *q = st.q, **z = st.z.data();
+    stage_rhs(st, p, v);
+    if (!stage_solve(st, alpha, v, z[0], 0).success) return false;
+    lincomb<double>(m, q, 0, {{1.0, p}, {0.5 - gamma, z[0]}}, n);
+    stage_rhs(st, q, v);
+    if (!stage_solve(st, alpha, v, z[1], 1).success) return false;
+    lincomb<double>(m, q, 0, {{1.0, p}, {2 * gamma, z[0]}, {1 - 4 * gamma, z[1]}}, n);
+    stage_rhs(st, q, v);
+    if (!stage_solve(st, alpha, v, z[2], 2).success) return false;
+    lincomb<double>(m, p, 2, {{delta, z[0]}, {1 - 2 * delta, z[1]}, {delta, z[2]}}, n);
+    record_history(st, 3);
+    return true;
+}
+
+// one DIRK4S5 step, schrodinger.hpp:219-268
+static bool step_dirk4s5(Stepper& st)
+{
+    sem_mesh* m = st.m;
+    const double alpha = 0.25 * st.opts.dt;
+    const size_t n = st.n;
+    double *p = st.p, *v = st.v, *q = st.q, **z = st.z.data();
+    stage_rhs(st, p, v);
+    if (!stage_solve(st, alpha, v, z[0], 0).success) return false;
+    lincomb<double>(m, q, 0, {{1.0, p}, {0.5, z[0]}}, n);
+    stage_rhs(st, q, v);
+    if (!stage_solve(st, alpha, v, z[1], 1).success) return false;
+    lincomb<double>(m, q, 0, {{1.0, p}, {17.0 / 50.0, z[0]}, {-1.0 / 25.0, z[1]}}, n);
+    stage_rhs(st, q, v);
+    if (!stage_solve(st, alpha, v, z[2], 2).success) return false;
+    lincomb<double>(m, q, 0, {{1.0, p}, {371.0 / 1360.0, z[0]}, {-137.0 / 2720.0, z[1]}, {15.0 / 544.0, z[2]}}, n);
+    stage_rhs(st, q, v);
+    if (!stage_solve(st, alpha, v, z[3], 3).success) return false;
+    lincomb<double>(m, q, 0,
+                    {{1.0, p}, {25.0 / 24.0, z[0]}, {-49.0 / 48.0, z[1]}, {125.0 / 16.0, z[2]}, {-85.0 / 12.0, z[3]}}, n);
+    stage_rhs(st, q, v);
+    if (!stage_solve(st, alpha, v, z[4], 4).success) return false;
+    lincomb<double>(m, p, 2,
+                    {{25.0 / 24.0, z[0]}, {-49.0 / 48.0, z[1]}, {125.0 / 16.0, z[2]}, {-85.0 / 12.0, z[3]}, {0.25, z[4]}},
+                    n);
+    lincomb<double>(m, q, 1, {{-3.0 / 16.0, z[0]}, {-27.0 / 32.0, z[1]}, {25.0 / 32.0, z[2]}, {0.25, z[4]}}, n);
+    st.stats.last_error_estimate = std::sqrt(wdot_host<double>(m, q, q, 2)) / st.pnorm;
+    record_history(st, 5);
+    return true;
+}
+
+// one BDF2 step, schrodinger.hpp:91-102 (implicit Euler first); the
+// previous psi is the initial guess (solved in place)
+static bool step_bdf2(Stepper& st)
+{
+    sem_mesh* m = st.m;
+    const size_t len = m->len(), n = st.n;
+    double* p = st.p;
+    if (st.nps == 2) std::swap(st.ps[0], st.ps[1]);
+    double* slot = st.ps[st.nps == 2 ? 1 : st.nps];
+    // proj(p) = [gproj(re), -gproj(im)] (schrodinger.hpp:69-78)
+    apply_op<double>(m, K_MASS, p, slot, nullptr, 0.0, 1.0);
+    apply_op<double>(m, K_MASS, p + len, slot + len, nullptr, 0.0, -1.0);
+    if (st.nps < 2) ++st.nps;
+    Results r;
+    if (st.nps == 1) {
+        r = solve_res(st, st.alpha_bdf, st.ps[0], p, false, false);
+        st.alpha_bdf = (2.0 / 3.0) * st.opts.dt;
+    } else {
+        lincomb<double>(m, st.v, 1, {{-1.0 / 3.0, st.ps[0]}, {4.0 / 3.0, st.ps[1]}}, n);
+        r = solve_res(st, st.alpha_bdf, st.v, p, false, false);
+    }
+    return r.success != 0;
+}
+
+// while (t < T) loop of the steppers; max_steps > 0 also bounds the count.
+static bool stepper_advance(Stepper& st, double T, long max_steps, sem_step_cb cb, void* user)
+{
+    bool success = true;
+    long k = 0;
+    const double dt = st.opts.dt;
+    while (st.t < T && (max_steps <= 0 || k < max_steps)) {
+        bool ok;
+        if (st.scheme == 0) ok = step_bdf2(st);
+        else if (st.scheme == 1) ok = step_dirk4s3(st);
+        else ok = step_dirk4s5(st);
+        if (st.scheme == 0) {
+            // bdf2 counts the step before testing success (schrodinger.hpp:107-114)
+            st.t += dt;
+            st.stats.steps++;
+            ++k;
+            if (st.opts.verbose) progress(st.t, T, dt, nullptr);
+            if (!ok) { success = false; break; }
+        } else {
+            if (!ok) { success = false; break; }
+            st.t += dt;
+            st.stats.steps++;
+            ++k;
+            if (st.opts.verbose) progress(st.t, T, dt, st.scheme == 2 ? &st.stats.last_error_estimate : nullptr);
+        }
+        if (do_callback(st, cb, user)) break;
+    }
+    if (st.opts.verbose) std::printf("\n");
+    st.m->sync();
+    return success;
+}
+
+static void stepper_read(Stepper& st, double* re, double* im)
+{
+    sem_mesh* m = st.m;
+    const size_t rl = (size_t)m->L.E * m->L.N * m->L.N;
+    std::vector<double> h(2 * rl);
+    m->download(h.data(), st.p, 2);
+    if (re) std::copy(h.begin(), h.begin() + rl, re);
+    if (im) std::copy(h.begin() + rl, h.end(), im);
+}
+
+// ------------------------------------------------------- generic solvers --
+// minres.hpp / pcg.hpp over plain device vectors with a tridiagonal operator
+// and the Euclidean dot; host-driven (one scalar readback per reduction).
+struct GenericCtx {
+    hipStream_t s;
+    long n;
+    double lo, di, up;
+    double* part;
+    double* red;
+    double* h_red;
+    double dot(const double* a, const double* b)
+    {
+        int nb = launch_edot(a, b, (size_t)n, part, s);
+        launch_fin_sum(part, nb, red, s);
+        SEM_HIP(hipMemcpyAsync(h_red, red, sizeof(double), hipMemcpyDeviceToHost, s));
+        SEM_HIP(hipStreamSynchronize(s));
+        return *h_red;
+    }
+    void A(const double* x, double* y) { launch_tridiag(x, y, n, lo, di, up, s); }
+    void comb(double* out, int mode, std::initializer_list<std::pair<double, const double*>> t)
+    {
+        LinComb<double> a{};
+        a.out = out;
+        a.mode = mode;
+        a.n = (size_t)n;
+        int k = 0;
+        for (auto& pr : t) {
+            a.c[k] = pr.first;
+            a.x[k] = pr.second;
+            ++k;
+        }
+        a.m = k;
+        launch_lincomb<double>(a, s);
+    }
+};
+
+static Results generic_minres(GenericCtx& g, double* x, const double* b, int maxit, double tol, double* buf)
+{
+    const long n = g.n;
+    double *r1 = buf, *r2 = buf + n, *y = buf + 2 * n, *v = buf + 3 * n, *w = buf + 4 * n, *w1 = buf + 5 * n,
+           *w2 = buf + 6 * n, *t = buf + 7 * n;
+    g.A(x, t);
+    g.comb(r1, 0, {{1.0, b}, {-1.0, t}});
+    g.comb(y, 0, {{1.0, r1}});
+    g.comb(r2, 0, {{1.0, r1}});
+    const double b_norm = std::sqrt(g.dot(r1, r1));
+    const double beta1 = std::sqrt(g.dot(y, r1));
+    double beta = beta1, beta_old = 0, tnorm = 0, dbar = 0, eps = 0, eps_old = 0, phibar = beta1, cs = -1, sn = 0;
+    g.comb(w, 1, {{0.0, x}});
+    g.comb(w2, 0, {{1.0, w}});
+    bool success = false;
+    int i;
+    for (i = 1; i <= maxit; ++i) {
+        g.comb(v, 1, {{1.0 / beta, y}});
+        g.A(v, y);
+        if (i >= 2) g.comb(y, 0, {{1.0, y}, {-(beta / beta_old), r1}});
+        const double alpha = g.dot(v, y);
+        g.comb(y, 0, {{1.0, y}, {-(alpha / beta), r2}});
+        std::swap(r1, r2);
+        g.comb(r2, 0, {{1.0, y}});
+        beta_old = beta;
+        beta = std::sqrt(g.dot(y, y));
+        tnorm += alpha * alpha + beta * beta + beta_old * beta_old;
+        eps_old = eps;
+        const double delta = cs * dbar + sn * alpha;
+        const double gbar = sn * dbar - cs * alpha;
+        eps = sn * beta;
+        dbar = -cs * beta;
+        const double gamma = std::max(std::sqrt(gbar * gbar + beta * beta), 2.220446049250313e-16);
+        cs = gbar / gamma;
+        sn = beta / gamma;
+        const double phi = cs * phibar;
+        phibar *= sn;
+        std::swap(w1, w2);  // w1 <- w2
+        std::swap(w2, w);   // w2 <- w
+        g.comb(w, 1, {{1.0 / gamma, v}, {-eps_old / gamma, w1}, {-delta / gamma, w2}});
+        g.comb(x, 2, {{phi, w}});
+        if (phibar < tol * b_norm + tol) {
+            success = true;
+            break;
+        }
+    }
+    return {success ? 1 : 0, i, phibar};
+}
+
+static Results generic_pcg(GenericCtx& g, double* x, const double* b, int maxit, double tol, double* buf)
+{
+    const long n = g.n;
+    double *r = buf, *p = buf + n, *ap = buf + 2 * n;
+    g.A(x, ap);
+    g.comb(r, 0, {{1.0, b}, {-1.0, ap}});
+    const double bnorm = std::sqrt(g.dot(b, b));
+    bool success = false;
+    double rho = 0, rho_prev = 0;
+    int i;
+    for (i = 0; i < maxit; ++i) {
+        if (i == 0) {
+            g.comb(p, 0, {{1.0, r}});
+            rho = g.dot(r, r);
+        } else {
+            rho = g.dot(r, r);
+            g.comb(p, 0, {{1.0, r}, {rho / rho_prev, p}});
+        }
+        g.A(p, ap);
+        const double alpha = rho / g.dot(p, ap);
+        g.comb(x, 2, {{alpha, p}});
+        g.comb(r, 2, {{-alpha, ap}});
+        rho_prev = rho;
+        if (std::sqrt(rho) < tol * bnorm) {
+            success = true;
+            break;
+        }
+    }
+    return {success ? 1 : 0, i, std::sqrt(rho)};
+}
+
+}  // namespace sem
+
+// ================================================================ C-ABI ==
+namespace {
+
+template <typename F>
+int guarded(F&& f)
+{
+    try {
+        return f();
+    } catch (const HipError& e) {
+        set_error(std::string("HIP error ") + hipGetErrorString(e.code) + " in " + e.where);
+        return SEM_EHIP;
+    } catch (const std::bad_alloc&) {
+        set_error("out of memory");
+        return SEM_ENOMEM;
+    } catch (const std::invalid_argument& e) {
+        set_error(e.what());
+        return SEM_EINVAL;
+    } catch (const std::exception& e) {
+        set_error(e.what());
+        return SEM_EINVAL;
+    }
+}
+
+bool check_gpu()
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n < 1) {
+        set_error("no HIP device available (libsem_hip needs an MI355X / gfx950 GPU)");
+        return false;
+    }
+    return true;
+}
+
+size_t ref_len(const sem_mesh* m) { return (size_t)m->L.E * m->L.N * m->L.N; }
+
+void fill_results(sem_solver_results* out, const Results& r)
+{
+    if (!out) return;
+    out->success = r.success;
+    out->n_iter = r.n_iter;
+    out->residual = r.residual;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* sem_last_error(void) { return g_error.c_str(); }
+int sem_version(void) { return 100; }
+
+int sem_device_info(char* name, int name_len, int* n_cu, long* hbm_bytes)
+{
+    return guarded([&] {
+        if (!check_gpu()) return (int)SEM_EHIP;
+        hipDeviceProp_t p;
+        SEM_HIP(hipGetDeviceProperties(&p, 0));
+        if (name && name_len > 0) std::snprintf(name, name_len, "%s (%s)", p.name, p.gcnArchName);
+        if (n_cu) *n_cu = p.multiProcessorCount;
+        if (hbm_bytes) *hbm_bytes = (long)p.totalGlobalMem;
+        return (int)SEM_OK;
+    });
+}
+
+int sem_mesh_create(int p, const double* xy, int n_nodes, const int* quads, int n_elements, sem_mesh** out)
+{
+    return guarded([&] {
+        if (!out || !xy || !quads) throw std::invalid_argument("sem_mesh_create: null argument");
+        *out = nullptr;
+        if (!check_gpu()) return (int)SEM_EHIP;
+        std::unique_ptr<sem_mesh> m(new sem_mesh);
+        build_host_mesh(m->hm, p, xy, n_nodes, quads, n_elements);
+        setup_device(m.get());
+        *out = m.release();
+        return (int)SEM_OK;
+    });
+}
+
+int sem_mesh_load(int p, const char* folder, sem_mesh** out)
+{
+    std::vector<double> xy;
+    std::vector<int> q;
+    try {
+        if (!folder) throw std::invalid_argument("null folder");
+        read_mesh_folder(folder, xy, q);
+    } catch (const std::invalid_argument& e) {
+        set_error(e.what());
+        return SEM_EINVAL;
+    } catch (const std::exception& e) {
+        set_error(e.what());
+        return SEM_EIO;
+    }
+    return sem_mesh_create(p, xy.data(), (int)xy.size() / 2, q.data(), (int)q.size() / 4, out);
+}
+
+void sem_mesh_destroy(sem_mesh* m) { delete m; }
+
+int sem_mesh_info(const sem_mesh* m, long* out)
+{
+    if (!m || !out) return SEM_EINVAL;
+    out[0] = m->hm.N;
+    out[1] = m->hm.n_el;
+    out[2] = m->hm.n_nodes;
+    out[3] = (long)m->hm.edges.size();
+    out[4] = m->hm.dof;
+    out[5] = m->L.K;
+    return SEM_OK;
+}
+
+int sem_mesh_quadrature(const sem_mesh* m, double* x, double* w, double* D)
+{
+    if (!m) return SEM_EINVAL;
+    if (x) std::copy(m->hm.x.begin(), m->hm.x.end(), x);
+    if (w) std::copy(m->hm.w.begin(), m->hm.w.end(), w);
+    if (D) std::copy(m->hm.D.begin(), m->hm.D.end(), D);
+    return SEM_OK;
+}
+
+int sem_mesh_coords(const sem_mesh* m, double* X, double* Y)
+{
+    if (!m || !X || !Y) return SEM_EINVAL;
+    const HostMesh& hm = m->hm;
+    const int N = hm.N;
+    for (int e = 0; e < hm.n_el; ++e) {
+        double xs[4], ys[4];
+        for (int k = 0; k < 4; ++k) {
+            xs[k] = hm.node_xy[2 * hm.quads[4 * e + k]];
+            ys[k] = hm.node_xy[2 * hm.quads[4 * e + k] + 1];
+        }
+        for (int j = 0; j < N; ++j)
+            for (int i = 0; i < N; ++i) {
+                const double xi = hm.x[i], eta = hm.x[j];
+                const size_t k = (size_t)e * N * N + i + (size_t)j * N;
+                X[k] = 0.25 * (xs[0] * (1 - xi) * (1 - eta) + xs[1] * (1 + xi) * (1 - eta) + xs[2] * (1 + xi) * (1 + eta) +
+                               xs[3] * (1 - xi) * (1 + eta));
+                Y[k] = 0.25 * (ys[0] * (1 - xi) * (1 - eta) + ys[1] * (1 + xi) * (1 - eta) + ys[2] * (1 + xi) * (1 + eta) +
+                               ys[3] * (1 - xi) * (1 + eta));
+            }
+    }
+    return SEM_OK;
+}
+
+int sem_mesh_metrics(const sem_mesh* m, double* J, double* A, double* B, double* C)
+{
+    if (!m) return SEM_EINVAL;
+    if (J) std::copy(m->hm.J.begin(), m->hm.J.end(), J);
+    if (A) std::copy(m->hm.A.begin(), m->hm.A.end(), A);
+    if (B) std::copy(m->hm.B.begin(), m->hm.B.end(), B);
+    if (C) std::copy(m->hm.C.begin(), m->hm.C.end(), C);
+    return SEM_OK;
+}
+
+int sem_element_apply(sem_mesh* m, int kind, const double* coef, double alpha, const double* in, double* out)
+{
+    return guarded([&] {
+        if (!m || !in || !out || kind < 0 || kind > 5) throw std::invalid_argument("sem_element_apply: bad argument");
+        if ((kind == K_HELMHOLTZ || kind == K_RES || kind == K_F || kind == K_L) && !coef)
+            throw std::invalid_argument("sem_element_apply: this operator needs a coefficient field");
+        const int nc = op_ncomp(kind);
+        const size_t n = (size_t)nc * m->len();
+        double* din = m->buf<double>("api_in", n);
+        double* dout = m->buf<double>("api_out", n);
+        m->upload(din, in, nc);
+        Field f;
+        if (coef) f = make_field(m, coef, "api_cf");
+        OpArgs<double> a{};
+        a.in = din;
+        a.out = dout;
+        a.geo = m->geo;
+        a.cf = f.c64;
+        a.alpha = alpha;
+        a.post = 1.0;
+        a.len = m->len();
+        a.nchunks = m->L.nchunks;
+        launch_element_op_d<double>(m->L.N, kind, a, m->D64.data(), m->stream);
+        m->download(out, dout, nc);
+        return (int)SEM_OK;
+    });
+}
+
+int sem_glaplace_apply(sem_mesh* m, const double* u, double* y)
+{
+    return sem_element_apply(m, SEM_OP_LAPLACE, nullptr, 0.0, u, y);
+}
+
+int sem_galerkin_apply(sem_mesh* m, int kind, const double* coef, double alpha, const double* in, double* out)
+{
+    return guarded([&] {
+        if (!m || !in || !out || kind < 0 || kind > 5) throw std::invalid_argument("sem_galerkin_apply: bad argument");
+        if ((kind == K_HELMHOLTZ || kind == K_RES || kind == K_F || kind == K_L) && !coef)
+            throw std::invalid_argument("sem_galerkin_apply: this operator needs a coefficient field");
+        const int nc = op_ncomp(kind);
+        const size_t n = (size_t)nc * m->len();
+        double* din = m->buf<double>("api_in", n);
+        double* dout = m->buf<double>("api_out", n);
+        m->upload(din, in, nc);
+        Field f;
+        if (coef) f = make_field(m, coef, "api_cf");
+        dss<double>(m, DSS_UNMASK, din, nc);
+        apply_op<double>(m, kind, din, dout, f.c64, alpha);
+        dss<double>(m, DSS_MASK, dout, nc);
+        m->download(out, dout, nc);
+        return (int)SEM_OK;
+    });
+}
+
+int sem_gproj(sem_mesh* m, const double* u, double* out)
+{
+    return sem_galerkin_apply(m, SEM_OP_MASS, nullptr, 0.0, u, out);
+}
+
+static int dss_host(sem_mesh* m, double* a, int mode)
+{
+    return guarded([&] {
+        if (!m || !a) throw std::invalid_argument("null argument");
+        double* d = m->buf<double>("api_in", m->len());
+        m->upload(d, a, 1);
+        dss<double>(m, mode, d, 1);
+        m->download(a, d, 1);
+        return (int)SEM_OK;
+    });
+}
+
+int sem_unmask(sem_mesh* m, double* a) { return dss_host(m, a, DSS_UNMASK); }
+int sem_global_sum(sem_mesh* m, double* a) { return dss_host(m, a, DSS_SUM); }
+int sem_mask(sem_mesh* m, double* a) { return dss_host(m, a, DSS_MASK); }
+
+int sem_dot(sem_mesh* m, const double* a, const double* b, double* out)
+{
+    return guarded([&] {
+        if (!m || !a || !b || !out) throw std::invalid_argument("null argument");
+        double* da = m->buf<double>("api_in", m->len());
+        double* db = m->buf<double>("api_out", m->len());
+        m->upload(da, a, 1);
+        m->upload(db, b, 1);
+        *out = wdot_host<double>(m, da, db, 1);
+        return (int)SEM_OK;
+    });
+}
+
+int sem_poisson(sem_mesh* m, double* u, const double* f, int max_iter, double tol, sem_solver_results* out)
+{
+    return guarded([&] {
+        if (!m || !u || !f) throw std::invalid_argument("sem_poisson: null argument");
+        const size_t n = m->len();
+        double* b = m->buf<double>("ps_b", n);
+        double* x = m->buf<double>("ps_x", n);
+        m->upload(b, f, 1);
+        dss<double>(m, DSS_UNMASK, b, 1);
+        double* bb = m->buf<double>("ps_bb", n);
+        apply_op<double>(m, K_MASS, b, bb, nullptr, 0.0);  // gproj(f)
+        m->upload(x, u, 1);
+        dss<double>(m, DSS_UNMASK, x, 1);
+        Results r = pcg_fused(m, bb, x, false, max_iter, tol);
+        merge_owner(m, x, u);
+        fill_results(out, r);
+        return (int)SEM_OK;
+    });
+}
+
+int sem_helmholtz(sem_mesh* m, double* u, const double* c, const double* f, int max_iter, double tol,
+                  sem_solver_results* out)
+{
+    return guarded([&] {
+        if (!m || !u || !c || !f) throw std::invalid_argument("sem_helmholtz: null argument");
+        const size_t n = m->len();
+        double* b = m->buf<double>("hh_b", n);
+        double* x = m->buf<double>("hh_x", n);
+        m->upload(b, f, 1);
+        dss<double>(m, DSS_UNMASK, b, 1);
+        double* bb = m->buf<double>("hh_bb", n);
+        apply_op<double>(m, K_MASS, b, bb, nullptr, 0.0);
+        Field cf = make_field(m, c, "hh_c");
+        m->upload(x, u, 1);
+        dss<double>(m, DSS_UNMASK, x, 1);
+        KrylovOpts o;
+        o.maxit = max_iter;
+        o.tol = tol;
+        o.atol = tol;
+        Results r = minres_fused<double>(m, K_HELMHOLTZ, cf.c64, 0.0, bb, x, false, o);
+        merge_owner(m, x, u);
+        fill_results(out, r);
+        return (int)SEM_OK;
+    });
+}
+
+static int run_stepper(int scheme, sem_mesh* m, double* re, double* im, const double* V, double T,
+                       const sem_solver_opts* opts, sem_step_cb cb, void* user, int* success)
+{
+    return guarded([&] {
+        if (!m || !re || !im || !V || !opts) throw std::invalid_argument("schrodinger: null argument");
+        std::unique_ptr<Stepper> st(stepper_create(m, scheme, re, im, V, opts));
+        const bool ok = stepper_advance(*st, T, 0, cb, user);
+        stepper_read(*st, re, im);
+        m->stats = st->stats;
+        if (success) *success = ok ? 1 : 0;
+        return (int)SEM_OK;
+    });
+}
+
+int sem_schrodinger_bdf2(sem_mesh* m, double* re, double* im, const double* V, double T, const sem_solver_opts* opts,
+                         sem_step_cb cb, void* user, int* success)
+{
+    return run_stepper(0, m, re, im, V, T, opts, cb, user, success);
+}
+
+int sem_schrodinger_dirk4s3(sem_mesh* m, double* re, double* im, const double* V, double T,
+                            const sem_solver_opts* opts, sem_step_cb cb, void* user, int* success)
+{
+    return run_stepper(1, m, re, im, V, T, opts, cb, user, success);
+}
+
+int sem_schrodinger_dirk4s5(sem_mesh* m, double* re, double* im, const double* V, double T,
+                            const sem_solver_opts* opts, sem_step_cb cb, void* user, int* success)
+{
+    return run_stepper(2, m, re, im, V, T, opts, cb, user, success);
+}
+
+int sem_last_run_stats(const sem_mesh* m, sem_run_stats* out)
+{
+    if (!m || !out) return SEM_EINVAL;
+    *out = m->stats;
+    return SEM_OK;
+}
+
+struct sem_stepper {
+    std::unique_ptr<Stepper> st;
+};
+
+int sem_stepper_create(sem_mesh* m, int scheme, const double* re, const double* im, const double* V,
+                       const sem_solver_opts* opts, sem_stepper** out)
+{
+    return guarded([&] {
+        if (!m || !re || !im || !V || !opts || !out || scheme < 0 || scheme > 2)
+            throw std::invalid_argument("sem_stepper_create: bad argument");
+        *out = nullptr;
+        std::unique_ptr<sem_stepper> h(new sem_stepper);
+        h->st.reset(stepper_create(m, scheme, re, im, V, opts));
+        *out = h.release();
+        return (int)SEM_OK;
+    });
+}
+
+int sem_stepper_advance(sem_stepper* h, double T, long max_steps, sem_step_cb cb, void* user, int* success)
+{
+    return guarded([&] {
+        if (!h) throw std::invalid_argument("sem_stepper_advance: null handle");
+        const bool ok = stepper_advance(*h->st, T, max_steps, cb, user);
+        if (success) *success = ok ? 1 : 0;
+        return (int)SEM_OK;
+    });
+}
+
+int sem_stepper_read(sem_stepper* h, double* re, double* im, double* t)
+{
+    return guarded([&] {
+        if (!h) throw std::invalid_argument("sem_stepper_read: null handle");
+        stepper_read(*h->st, re, im);
+        if (t) *t = h->st->t;
+        return (int)SEM_OK;
+    });
+}
+
+int sem_stepper_stats(const sem_stepper* h, sem_run_stats* out)
+{
+    if (!h || !out) return SEM_EINVAL;
+    *out = h->st->stats;
+    return SEM_OK;
+}
+
+void sem_stepper_destroy(sem_stepper* h) { delete h; }
+
+int sem_stage_solve(sem_mesh* m, const double* V, double alpha, const double* b, double* z, int max_iter, double tol,
+                    int precision, sem_solver_results* out)
+{
+    return guarded([&] {
+        if (!m || !V || !b || !z) throw std::invalid_argument("sem_stage_solve: null argument");
+        const size_t n = 2 * m->len();
+        Field f = make_field(m, V, "ss_V");
+        double* db = m->buf<double>("ss_b", n);
+        double* dz = m->buf<double>("ss_z", n);
+        m->upload(db, b, 2);
+        m->upload(dz, z, 2);
+        dss<double>(m, DSS_UNMASK, dz, 2);
+        KrylovOpts o;
+        o.maxit = max_iter;
+        o.tol = tol;
+        o.atol = tol;
+        Results r;
+        if (precision == SEM_PREC_MIXED) r = solve_mixed(m, K_RES, f.c64, f.c32, alpha, db, dz, false, o, 2e-5);
+        else r = minres_fused<double>(m, K_RES, f.c64, alpha, db, dz, false, o);
+        // unmasked (consistent) solution
+        m->download(z, dz, 2);
+        fill_results(out, r);
+        return (int)SEM_OK;
+    });
+}
+
+static int tridiag_solve(bool use_minres, long n, double lo, double di, double up, const double* b, double* x,
+                         int maxit, double tol, sem_solver_results* out)
+{
+    return guarded([&] {
+        if (n < 1 || !b || !x) throw std::invalid_argument("tridiag solver: bad argument");
+        if (!check_gpu()) return (int)SEM_EHIP;
+        GenericCtx g{};
+        SEM_HIP(hipStreamCreateWithFlags(&g.s, hipStreamNonBlocking));
+        g.n = n;
+        g.lo = lo;
+        g.di = di;
+        g.up = up;
+        double *db = nullptr, *dx = nullptr, *buf = nullptr;
+        int rc = SEM_OK;
+        try {
+            SEM_HIP(hipMalloc(&db, n * sizeof(double)));
+            SEM_HIP(hipMalloc(&dx, n * sizeof(double)));
+            SEM_HIP(hipMalloc(&buf, 8 * n * sizeof(double)));
+            SEM_HIP(hipMalloc(&g.part, 1024 * sizeof(double)));
+            SEM_HIP(hipMalloc(&g.red, sizeof(double)));
+            SEM_HIP(hipHostMalloc(&g.h_red, sizeof(double), hipHostMallocDefault));
+            SEM_HIP(hipMemcpy(db, b, n * sizeof(double), hipMemcpyHostToDevice));
+            SEM_HIP(hipMemcpy(dx, x, n * sizeof(double), hipMemcpyHostToDevice));
+            Results r = use_minres ? generic_minres(g, dx, db, maxit, tol, buf) : generic_pcg(g, dx, db, maxit, tol, buf);
+            SEM_HIP(hipStreamSynchronize(g.s));
+            SEM_HIP(hipMemcpy(x, dx, n * sizeof(double), hipMemcpyDeviceToHost));
+            fill_results(out, r);
+        } catch (...) {
+            rc = SEM_EHIP;
+        }
+        hipFree(db);
+        hipFree(dx);
+        hipFree(buf);
+        hipFree(g.part);
+        hipFree(g.red);
+        if (g.h_red) hipHostFree(g.h_red);
+        hipStreamDestroy(g.s);
+        if (rc != SEM_OK) throw HipError{hipErrorUnknown, "tridiag solver"};
+        return rc;
+    });
+}
+
+int sem_minres_tridiag(long n, double lo, double di, double up, const double* b, double* x, int maxit, double tol,
+                       sem_solver_results* out)
+{
+    return tridiag_solve(true, n, lo, di, up, b, x, maxit, tol, out);
+}
+
+int sem_pcg_tridiag(long n, double lo, double di, double up, const double* b, double* x, int maxit, double tol,
+                    sem_solver_results* out)
+{
+    return tridiag_solve(false, n, lo, di, up, b, x, maxit, tol, out);
+}
+
+int sem_set_kernel_timing(sem_mesh* m, int on)
+{
+    if (!m) return SEM_EINVAL;
+    m->timing = on != 0;
+    for (int k = 0; k < 2; ++k) {
+        m->kt_launches[k] = 0;
+        m->kt_us[k] = 0;
+    }
+    return SEM_OK;
+}
+
+int sem_kernel_timing(const sem_mesh* m, int which, long* launches, double* total_us, double* bytes_per_launch)
+{
+    if (!m || which < 0 || which > 1) return SEM_EINVAL;
+    if (launches) *launches = m->kt_launches[which];
+    if (total_us) *total_us = m->kt_us[which];
+    if (bytes_per_launch) *bytes_per_launch = m->kt_bytes[which];
+    return SEM_OK;
+}
+
+int sem_time_element_op(sem_mesh* m, int kind, int precision, int reps, double* us_per_launch, double* bytes_per_launch)
+{
+    return guarded([&] {
+        if (!m || reps < 1 || kind < 0 || kind > 5) throw std::invalid_argument("sem_time_element_op: bad argument");
+        const int nc = op_ncomp(kind);
+        const size_t n = (size_t)nc * m->len();
+        const int N = m->L.N;
+        const double EN2 = (double)m->L.E * N * N;
+        // algorithmic bytes per launch in the reference data model, 8 B (FP64) or 4 B (FP32) values
+        const double vb = precision ? 4.0 : 8.0;
+        double fields;
+        switch (kind) {
+        case K_LAPLACE: fields = 5; break;    // u, A, B, C in; y out
+        case K_HELMHOLTZ: fields = 7; break;  // u, c, J, A, B, C in; y out
+        case K_MASS: fields = 3; break;       // u, J in; y out
+        case K_RES: fields = 9; break;        // u, v, V, J, A, B, C in; 2 out
+        case K_F: fields = 9; break;          // u, v, V, J, A, B, C in; 2 out
+        default: fields = 7; break;
+        }
+        if (bytes_per_launch) *bytes_per_launch = fields * vb * EN2;
+        hipEvent_t e0, e1;
+        SEM_HIP(hipEventCreate(&e0));
+        SEM_HIP(hipEventCreate(&e1));
+        float ms = 0;
+        if (precision) {
+            float* in = m->buf<float>("t_in32", n);
+            float* out = m->buf<float>("t_out32", n);
+            float* cf = m->buf<float>("t_cf32", m->len());
+            launch_fill<float>(in, 0.5f, n, m->stream);
+            launch_fill<float>(cf, 0.25f, m->len(), m->stream);
+            OpArgs<float> a{in, out, m->geo32, cf, 0.1f, 1.0f, m->len(), m->L.nchunks};
+            launch_element_op_d<float>(N, kind, a, m->D32.data(), m->stream);
+            SEM_HIP(hipEventRecord(e0, m->stream));
+            for (int r = 0; r < reps; ++r) launch_element_op_d<float>(N, kind, a, m->D32.data(), m->stream);
+            SEM_HIP(hipEventRecord(e1, m->stream));
+        } else {
+            double* in = m->buf<double>("t_in", n);
+            double* out = m->buf<double>("t_out", n);
+            double* cf = m->buf<double>("t_cf", m->len());
+            launch_fill<double>(in, 0.5, n, m->stream);
+            launch_fill<double>(cf, 0.25, m->len(), m->stream);
+            OpArgs<double> a{in, out, m->geo, cf, 0.1, 1.0, m->len(), m->L.nchunks};
+            launch_element_op_d<double>(N, kind, a, m->D64.data(), m->stream);
+            SEM_HIP(hipEventRecord(e0, m->stream));
+            for (int r = 0; r < reps; ++r) launch_element_op_d<double>(N, kind, a, m->D64.data(), m->stream);
+            SEM_HIP(hipEventRecord(e1, m->stream));
+        }
+        SEM_HIP(hipEventSynchronize(e1));
+        SEM_HIP(hipEventElapsedTime(&ms, e0, e1));
+        hipEventDestroy(e0);
+        hipEventDestroy(e1);
+        if (us_per_launch) *us_per_launch = 1000.0 * ms / reps;
+        return (int)SEM_OK;
+    });
+}
+
+}  // extern "C"

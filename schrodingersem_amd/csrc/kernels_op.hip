@@ -1,0 +1,459 @@
+// kernels_op.hip -- element-operator kernels for gfx950.
+//
+// One 64-lane wavefront (one workgroup) per chunk of K = 64/N elements; lane
+// l = el*N + j holds column j of element el in registers (the device layout
+// makes plane i of a chunk one contiguous K*N-value run, so every load and
+// store is coalesced). The sum-factorised contractions of glaplace
+// (glaplace.hpp:16-28) run in registers with D read from the kernel-argument
+// segment (uniform scalar loads); the two index transposes per direction go
+// through LDS. Geometric factors arrive premultiplied by w_i w_j:
+//   gA = w_i w_j A, gB = w_i w_j B, gC = w_i w_j C, gJ = w_i w_j J, so
+//   K u (i,l) = sum_k D(k,i) G1(k,l) + sum_k D(k,l) G2(i,k),
+//   G1 = gA u_xi - gB u_eta, G2 = gC u_eta - gB u_xi,
+// which is F W + W G of glaplace.hpp:22-25 with the weights moved inside.
+#include "sem_internal.hpp"
+
+namespace sem {
+
+template <int N>
+struct Shape {
+    static constexpr int K = 64 / N;
+    static constexpr int KN = K * N;
+    static constexpr int P = (N % 2) ? N : N + 1;  // LDS column pitch (odd)
+    static constexpr int LDS = K * N * P;
+};
+
+template <int N, typename T>
+struct Dm {
+    T v[N * N];  // D(i,k) at i + k*N
+};
+
+__device__ __forceinline__ double fmad(double a, double b, double c) { return __builtin_fma(a, b, c); }
+__device__ __forceinline__ float fmad(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+
+// thread holds column l of X: col[i] = X(i,l); returns row[k] = X(l,k)
+template <int N, typename T>
+__device__ __forceinline__ void xpose_c2r(T* buf, int el, int l, bool act, const T (&col)[N], T (&row)[N])
+{
+    constexpr int P = Shape<N>::P;
+    T* b = buf + el * N * P;
+    if (act) {
+#pragma unroll
+        for (int i = 0; i < N; ++i) b[l * P + i] = col[i];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < N; ++k) row[k] = act ? b[k * P + l] : T(0);
+}
+
+// thread holds row l of X: row[j] = X(l,j); returns col[i] = X(i,l)
+template <int N, typename T>
+__device__ __forceinline__ void xpose_r2c(T* buf, int el, int l, bool act, const T (&row)[N], T (&col)[N])
+{
+    constexpr int P = Shape<N>::P;
+    T* b = buf + el * N * P;
+    if (act) {
+#pragma unroll
+        for (int j = 0; j < N; ++j) b[j * P + l] = row[j];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < N; ++i) col[i] = act ? b[l * P + i] : T(0);
+}
+
+// out(:,l) = (K q)(:,l), the element stiffness action of glaplace.
+template <int N, typename T>
+__device__ __forceinline__ void stiffness(const Dm<N, T>& D, T* lds0, T* lds1, int el, int l, bool act, const T (&q)[N],
+                                          const T (&ga)[N], const T (&gb)[N], const T (&gc)[N], T (&out)[N])
+{
+    T qxi[N], tmp[N], qer[N], qe[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) {  // u_xi = D u
+        T s = 0;
+#pragma unroll
+        for (int k = 0; k < N; ++k) s = fmad(D.v[i + k * N], q[k], s);
+        qxi[i] = s;
+    }
+    xpose_c2r<N>(lds0, el, l, act, q, tmp);
+#pragma unroll
+    for (int j = 0; j < N; ++j) {  // u_eta = u D^T (row l)
+        T s = 0;
+#pragma unroll
+        for (int k = 0; k < N; ++k) s = fmad(D.v[j + k * N], tmp[k], s);
+        qer[j] = s;
+    }
+    xpose_r2c<N>(lds1, el, l, act, qer, qe);
+    T g1[N], g2[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        g1[i] = ga[i] * qxi[i] - gb[i] * qe[i];
+        g2[i] = gc[i] * qe[i] - gb[i] * qxi[i];
+    }
+#pragma unroll
+    for (int i = 0; i < N; ++i) {  // D^T G1
+        T s = 0;
+#pragma unroll
+        for (int k = 0; k < N; ++k) s = fmad(D.v[k + i * N], g1[k], s);
+        out[i] = s;
+    }
+    xpose_c2r<N>(lds0, el, l, act, g2, tmp);
+#pragma unroll
+    for (int j = 0; j < N; ++j) {  // G2 D (row l)
+        T s = 0;
+#pragma unroll
+        for (int k = 0; k < N; ++k) s = fmad(D.v[k + j * N], tmp[k], s);
+        qer[j] = s;
+    }
+    xpose_r2c<N>(lds1, el, l, act, qer, qe);
+#pragma unroll
+    for (int i = 0; i < N; ++i) out[i] += qe[i];
+}
+
+template <int N>
+__device__ __forceinline__ bool is_interior(int i, int l)
+{
+    return i > 0 && i < N - 1 && l > 0 && l < N - 1;
+}
+
+// Column loaders / storers of the device layout.
+template <int N, typename T>
+struct Col {
+    size_t base;
+    bool act;
+    __device__ __forceinline__ void ld(const T* __restrict__ p, T (&r)[N]) const
+    {
+#pragma unroll
+        for (int i = 0; i < N; ++i) r[i] = act ? p[base + (size_t)i * Shape<N>::KN] : T(0);
+    }
+    __device__ __forceinline__ void st(T* __restrict__ p, const T (&r)[N]) const
+    {
+        if (act) {
+#pragma unroll
+            for (int i = 0; i < N; ++i) p[base + (size_t)i * Shape<N>::KN] = r[i];
+        }
+    }
+};
+
+// Element operator of every sem_op_kind; ring entries are written raw (the DSS
+// kernel assembles them), interior entries are final and scaled by `post`.
+template <int N, typename T, int KIND>
+__global__ __launch_bounds__(64) void k_element_op(OpArgs<T> a, Dm<N, T> D)
+{
+    using S = Shape<N>;
+    __shared__ T lds0[S::LDS], lds1[S::LDS];
+    const int lane = threadIdx.x;
+    const bool act = lane < S::KN;
+    const int el = act ? lane / N : 0;
+    const int l = act ? lane % N : 0;
+    const Col<N, T> c{(size_t)blockIdx.x * S::K * N * N + lane, act};
+    const size_t len = a.len;
+    T ga[N], gb[N], gc[N];
+    c.ld(a.geo, ga);
+    c.ld(a.geo + len, gb);
+    c.ld(a.geo + 2 * len, gc);
+    auto finish = [&](T* dst, T (&y)[N]) {
+#pragma unroll
+        for (int i = 0; i < N; ++i)
+            if (is_interior<N>(i, l)) y[i] = a.post * y[i];
+        c.st(dst, y);
+    };
+    if constexpr (KIND == K_RES || KIND == K_F) {
+        T q0[N], q1[N], cf[N], ky[N], y[N];
+        c.ld(a.in, q0);
+        c.ld(a.in + len, q1);
+        c.ld(a.cf, cf);
+        // component 0 output from K q1
+        stiffness<N>(D, lds0, lds1, el, l, act, q1, ga, gb, gc, ky);
+        if constexpr (KIND == K_RES) {
+            T gj[N];
+            c.ld(a.geo + 3 * len, gj);
+#pragma unroll
+            for (int i = 0; i < N; ++i) y[i] = gj[i] * q0[i] + a.alpha * (ky[i] + cf[i] * q1[i]);
+        } else {
+#pragma unroll
+            for (int i = 0; i < N; ++i) y[i] = -(ky[i] + cf[i] * q1[i]);
+        }
+        finish(a.out, y);
+        stiffness<N>(D, lds0, lds1, el, l, act, q0, ga, gb, gc, ky);
+        if constexpr (KIND == K_RES) {
+            T gj[N];
+            c.ld(a.geo + 3 * len, gj);
+#pragma unroll
+            for (int i = 0; i < N; ++i) y[i] = a.alpha * (ky[i] + cf[i] * q0[i]) - gj[i] * q1[i];
+        } else {
+#pragma unroll
+            for (int i = 0; i < N; ++i) y[i] = -(ky[i] + cf[i] * q0[i]);
+        }
+        finish(a.out + len, y);
+    } else {
+        T q[N], y[N];
+        c.ld(a.in, q);
+        if constexpr (KIND == K_MASS) {
+            T gj[N];
+            c.ld(a.geo + 3 * len, gj);
+#pragma unroll
+            for (int i = 0; i < N; ++i) y[i] = gj[i] * q[i];
+        } else {
+            T ky[N];
+            stiffness<N>(D, lds0, lds1, el, l, act, q, ga, gb, gc, ky);
+            if constexpr (KIND == K_LAPLACE) {
+#pragma unroll
+                for (int i = 0; i < N; ++i) y[i] = ky[i];
+            } else if constexpr (KIND == K_HELMHOLTZ) {
+                T cf[N];
+                c.ld(a.cf, cf);
+#pragma unroll
+                for (int i = 0; i < N; ++i) y[i] = cf[i] * q[i] + ky[i];
+            } else {  // K_L
+                T cf[N];
+                c.ld(a.cf, cf);
+#pragma unroll
+                for (int i = 0; i < N; ++i) y[i] = ky[i] + cf[i] * q[i];
+            }
+        }
+        finish(a.out, y);
+    }
+}
+
+// Deterministic block sum: butterfly inside the wave, waves in index order.
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// MINRES Lanczos step i, fused with the w/x update of step i-1
+// (minres.hpp:48-86): v = ib*r2; y = A v - cr1*r1 (interior entries; the DSS
+// kernel finishes ring entries); alpha partial = sum v.y over interior
+// entries. Pending update: w_new = ig*((ibp*r1 - eo*wa) - dl*wb); x += phi*w_new.
+template <int N, typename T, int KIND>
+__global__ __launch_bounds__(64) void k_minres_op(MinresOpArgs<T> a, Dm<N, T> D)
+{
+    using S = Shape<N>;
+    constexpr int NC = (KIND == K_RES) ? 2 : 1;
+    __shared__ T lds0[S::LDS], lds1[S::LDS];
+    const KrylovScalars* sc = reinterpret_cast<const KrylovScalars*>(a.sc);
+    const int done = sc->done;
+    const int p3 = sc->p3_pending;
+    const int lane = threadIdx.x;
+    const bool act = lane < S::KN;
+    const int el = act ? lane / N : 0;
+    const int l = act ? lane % N : 0;
+    const Col<N, T> c{(size_t)blockIdx.x * S::K * N * N + lane, act};
+    const size_t len = a.len;
+    if (p3) {
+        const T ibp = (T)sc->p3_ibp, ig = (T)sc->p3_ig, eo = (T)sc->p3_eo, dl = (T)sc->p3_dl, phi = (T)sc->p3_phi;
+#pragma unroll
+        for (int cc = 0; cc < NC; ++cc) {
+            T r[N], wa[N], wb[N], x[N];
+            c.ld(a.r1 + cc * len, r);
+            c.ld(a.wa + cc * len, wa);
+            c.ld(a.wb + cc * len, wb);
+            c.ld(a.x + cc * len, x);
+#pragma unroll
+            for (int i = 0; i < N; ++i) {
+                T v = ibp * r[i];
+                T wn = ig * ((v - eo * wa[i]) - dl * wb[i]);
+                wa[i] = wn;
+                x[i] += phi * wn;
+            }
+            c.st(a.wn + cc * len, wa);
+            c.st(a.x + cc * len, x);
+        }
+    }
+    double part = 0;
+    if (!done) {
+        const T ib = (T)sc->ib, cr1 = (T)sc->cr1;
+        const int has_r1 = sc->has_r1;
+        T ga[N], gb[N], gc[N], cf[N];
+        c.ld(a.geo, ga);
+        c.ld(a.geo + len, gb);
+        c.ld(a.geo + 2 * len, gc);
+        if constexpr (KIND != K_LAPLACE) c.ld(a.cf, cf);
+        T q0[N];
+        c.ld(a.r2, q0);
+#pragma unroll
+        for (int i = 0; i < N; ++i) q0[i] = ib * q0[i];
+        auto finish = [&](int cc, T (&y)[N], const T (&q)[N]) {
+            T r[N];
+            if (has_r1) c.ld(a.r1 + cc * len, r);
+#pragma unroll
+            for (int i = 0; i < N; ++i)
+                if (is_interior<N>(i, l)) {
+                    if (has_r1) y[i] = y[i] - cr1 * r[i];
+                    part += (double)q[i] * (double)y[i];
+                }
+            c.st(a.y + cc * len, y);
+        };
+        if constexpr (KIND == K_RES) {
+            T q1[N], ky[N], y[N], gj[N];
+            c.ld(a.r2 + len, q1);
+#pragma unroll
+            for (int i = 0; i < N; ++i) q1[i] = ib * q1[i];
+            c.ld(a.geo + 3 * len, gj);
+            stiffness<N>(D, lds0, lds1, el, l, act, q1, ga, gb, gc, ky);
+#pragma unroll
+            for (int i = 0; i < N; ++i) y[i] = gj[i] * q0[i] + a.alpha * (ky[i] + cf[i] * q1[i]);
+            finish(0, y, q0);
+            stiffness<N>(D, lds0, lds1, el, l, act, q0, ga, gb, gc, ky);
+#pragma unroll
+            for (int i = 0; i < N; ++i) y[i] = a.alpha * (ky[i] + cf[i] * q0[i]) - gj[i] * q1[i];
+            finish(1, y, q1);
+        } else {
+            T ky[N], y[N];
+            stiffness<N>(D, lds0, lds1, el, l, act, q0, ga, gb, gc, ky);
+            if constexpr (KIND == K_HELMHOLTZ) {
+#pragma unroll
+                for (int i = 0; i < N; ++i) y[i] = cf[i] * q0[i] + ky[i];
+            } else {
+#pragma unroll
+                for (int i = 0; i < N; ++i) y[i] = ky[i];
+            }
+            finish(0, y, q0);
+        }
+    }
+    part = wave_sum(part);
+    if (threadIdx.x == 0) a.partial[blockIdx.x] = part;
+}
+
+// PCG step (pcg.hpp:70-80, identity preconditioner): p = r (first step) or
+// r + beta*p; Ap on interior entries; pAp partial over interior entries.
+
+template <int N, typename T>
+__global__ __launch_bounds__(64) void k_pcg_op(PcgOpArgs<T> a, Dm<N, T> D)
+{
+    using S = Shape<N>;
+    __shared__ T lds0[S::LDS], lds1[S::LDS];
+    const KrylovScalars* sc = reinterpret_cast<const KrylovScalars*>(a.sc);
+    const int lane = threadIdx.x;
+    const bool act = lane < S::KN;
+    const int el = act ? lane / N : 0;
+    const int l = act ? lane % N : 0;
+    const Col<N, T> c{(size_t)blockIdx.x * S::K * N * N + lane, act};
+    double part = 0;
+    if (!sc->done) {
+        const int first = sc->first;
+        const T beta = (T)sc->pcg_beta;
+        T q[N], ga[N], gb[N], gc[N], y[N];
+        c.ld(a.r, q);
+        if (!first) {
+            T pp[N];
+            c.ld(a.p, pp);
+#pragma unroll
+            for (int i = 0; i < N; ++i) q[i] = q[i] + beta * pp[i];
+        }
+        c.st(a.p, q);
+        c.ld(a.geo, ga);
+        c.ld(a.geo + a.len, gb);
+        c.ld(a.geo + 2 * a.len, gc);
+        stiffness<N>(D, lds0, lds1, el, l, act, q, ga, gb, gc, y);
+#pragma unroll
+        for (int i = 0; i < N; ++i)
+            if (is_interior<N>(i, l)) part += (double)q[i] * (double)y[i];
+        c.st(a.ap, y);
+    }
+    part = wave_sum(part);
+    if (threadIdx.x == 0) a.partial[blockIdx.x] = part;
+}
+
+// ---------------------------------------------------------------- launch --
+template <int N, typename T>
+static Dm<N, T> make_dm(const T* d)
+{
+    Dm<N, T> m;
+    for (int k = 0; k < N * N; ++k) m.v[k] = d[k];
+    return m;
+}
+
+template <int N, typename T>
+static void element_op_N(int kind, const OpArgs<T>& a, const T* d, hipStream_t s)
+{
+    auto D = make_dm<N, T>(d);
+    dim3 g(a.nchunks), b(64);
+    switch (kind) {
+    case K_LAPLACE: hipLaunchKernelGGL((k_element_op<N, T, K_LAPLACE>), g, b, 0, s, a, D); break;
+    case K_HELMHOLTZ: hipLaunchKernelGGL((k_element_op<N, T, K_HELMHOLTZ>), g, b, 0, s, a, D); break;
+    case K_MASS: hipLaunchKernelGGL((k_element_op<N, T, K_MASS>), g, b, 0, s, a, D); break;
+    case K_RES: hipLaunchKernelGGL((k_element_op<N, T, K_RES>), g, b, 0, s, a, D); break;
+    case K_F: hipLaunchKernelGGL((k_element_op<N, T, K_F>), g, b, 0, s, a, D); break;
+    case K_L: hipLaunchKernelGGL((k_element_op<N, T, K_L>), g, b, 0, s, a, D); break;
+    }
+}
+
+template <int N, typename T>
+static void minres_op_N(int kind, const MinresOpArgs<T>& a, const T* d, hipStream_t s)
+{
+    auto D = make_dm<N, T>(d);
+    dim3 g(a.nchunks), b(64);
+    switch (kind) {
+    case K_LAPLACE: hipLaunchKernelGGL((k_minres_op<N, T, K_LAPLACE>), g, b, 0, s, a, D); break;
+    case K_HELMHOLTZ: hipLaunchKernelGGL((k_minres_op<N, T, K_HELMHOLTZ>), g, b, 0, s, a, D); break;
+    case K_RES: hipLaunchKernelGGL((k_minres_op<N, T, K_RES>), g, b, 0, s, a, D); break;
+    default: throw std::runtime_error("minres: unsupported operator kind");
+    }
+}
+
+template <int N, typename T>
+static void pcg_op_N(const PcgOpArgs<T>& a, int nchunks, const T* d, hipStream_t s)
+{
+    auto D = make_dm<N, T>(d);
+    hipLaunchKernelGGL((k_pcg_op<N, T>), dim3(nchunks), dim3(64), 0, s, a, D);
+}
+
+#define SEM_DISPATCH_N(N, CALL)                                   \
+    switch (N) {                                                  \
+    case 2: CALL(2); break;                                       \
+    case 3: CALL(3); break;                                       \
+    case 4: CALL(4); break;                                       \
+    case 5: CALL(5); break;                                       \
+    case 6: CALL(6); break;                                       \
+    case 7: CALL(7); break;                                       \
+    case 8: CALL(8); break;                                       \
+    case 9: CALL(9); break;                                       \
+    case 10: CALL(10); break;                                     \
+    case 11: CALL(11); break;                                     \
+    case 12: CALL(12); break;                                     \
+    case 13: CALL(13); break;                                     \
+    case 14: CALL(14); break;                                     \
+    case 15: CALL(15); break;                                     \
+    case 16: CALL(16); break;                                     \
+    default: throw std::runtime_error("unsupported N");           \
+    }
+
+template <typename T>
+void launch_element_op_d(int N, int kind, const OpArgs<T>& a, const T* d, hipStream_t s)
+{
+#define CALL_EO(n) element_op_N<n, T>(kind, a, d, s)
+    SEM_DISPATCH_N(N, CALL_EO)
+#undef CALL_EO
+}
+
+template <typename T>
+void launch_minres_op_d(int N, int kind, const MinresOpArgs<T>& a, const T* d, hipStream_t s)
+{
+#define CALL_MO(n) minres_op_N<n, T>(kind, a, d, s)
+    SEM_DISPATCH_N(N, CALL_MO)
+#undef CALL_MO
+}
+
+template <typename T>
+void launch_pcg_op_d(int N, const PcgOpArgs<T>& a, int nchunks, const T* d, hipStream_t s)
+{
+#define CALL_PO(n) pcg_op_N<n, T>(a, nchunks, d, s)
+    SEM_DISPATCH_N(N, CALL_PO)
+#undef CALL_PO
+}
+
+// The translation unit is compiled once per value type (SEM_OP_FP32 set or
+// not) so the two halves of the kernel set build in parallel.
+#ifndef SEM_OP_FP32
+template void launch_element_op_d<double>(int, int, const OpArgs<double>&, const double*, hipStream_t);
+template void launch_minres_op_d<double>(int, int, const MinresOpArgs<double>&, const double*, hipStream_t);
+template void launch_pcg_op_d<double>(int, const PcgOpArgs<double>&, int, const double*, hipStream_t);
+#else
+template void launch_element_op_d<float>(int, int, const OpArgs<float>&, const float*, hipStream_t);
+template void launch_minres_op_d<float>(int, int, const MinresOpArgs<float>&, const float*, hipStream_t);
+#endif
+
+}  // namespace sem

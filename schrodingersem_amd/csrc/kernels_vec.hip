@@ -1,0 +1,590 @@
+// kernels_vec.hip -- direct-stiffness summation, vector updates and
+// deterministic reductions.
+//
+// DSS is a gather over precomputed index lists: every interface node is one
+// work item that reads all its copies, sums them in the reference's order and
+// writes the result to every copy. Nothing is accumulated with atomics, so
+// repeated runs are bit-identical. Edge sums are owner + other
+// (global_sum.hpp:28-34); corner sums are a left fold from +0.0 in
+// connected_elements order (global_sum.hpp:54-63).
+//
+// Reductions are two-stage: every block writes one partial (butterfly inside
+// the wave, waves in index order), then a single-block kernel adds the
+// partials in a fixed order. The result depends only on the launch shape.
+#include <cfloat>
+
+#include "sem_internal.hpp"
+#include "sem_kernels.hpp"
+
+namespace sem {
+
+__device__ __forceinline__ double wsum(double v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// block sum for blockDim = 256; result valid in thread 0
+__device__ __forceinline__ double block_sum256(double v)
+{
+    __shared__ double sh[4];
+    v = wsum(v);
+    const int w = threadIdx.x >> 6;
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) sh[w] = v;
+    __syncthreads();
+    double s = 0;
+    if (threadIdx.x == 0) s = ((sh[0] + sh[1]) + sh[2]) + sh[3];
+    return s;
+}
+
+// owner weight of entry (i, j) of an element with ownership bits `bits`
+__device__ __forceinline__ double owner_weight(int i, int j, int N, unsigned bits)
+{
+    const bool ri = (i == 0 || i == N - 1), rj = (j == 0 || j == N - 1);
+    if (!ri && !rj) return 1.0;
+    int bit;
+    if (ri && rj) bit = 4 + ((i == 0) ? ((j == 0) ? 0 : 3) : ((j == 0) ? 1 : 2));
+    else if (j == 0) bit = 0;
+    else if (i == N - 1) bit = 1;
+    else if (j == N - 1) bit = 2;
+    else bit = 3;
+    return ((bits >> bit) & 1u) ? 1.0 : 0.0;
+}
+
+// ------------------------------------------------------------------- DSS --
+template <typename T, int MODE>
+__global__ __launch_bounds__(256) void k_dss(DssArgs<T> a)
+{
+    const long t = (long)blockIdx.x * 256 + threadIdx.x;
+    const long np = a.n_pairs, nc = a.n_corners, nb = a.n_bnd;
+    double part = 0;
+    bool skip = false;
+    double ib = 1, cr1 = 0;
+    int has_r1 = 0;
+    if constexpr (MODE == DSS_MINRES || MODE == DSS_PCG) {
+        const KrylovScalars* sc = reinterpret_cast<const KrylovScalars*>(a.sc);
+        skip = sc->done != 0;
+        ib = sc->ib;
+        cr1 = sc->cr1;
+        has_r1 = sc->has_r1;
+    }
+    if (!skip) {
+        for (int c = 0; c < a.ncomp; ++c) {
+            T* y = a.y + (size_t)c * a.len;
+            const T post = a.post[c];
+            auto finish = [&](T s, int p0) -> T {
+                if constexpr (MODE == DSS_SUM_ZB) return post * s;
+                if constexpr (MODE == DSS_MINRES) {
+                    T v = s;
+                    if (has_r1) v = v - (T)cr1 * a.r1[(size_t)c * a.len + p0];
+                    part += (double)((T)ib * a.r2[(size_t)c * a.len + p0]) * (double)v;
+                    return v;
+                }
+                if constexpr (MODE == DSS_PCG) {
+                    part += (double)a.r2[(size_t)c * a.len + p0] * (double)s;
+                    return s;
+                }
+                return s;
+            };
+            if (t < np) {
+                const int2 pr = a.pairs[t];
+                if constexpr (MODE == DSS_UNMASK) {
+                    y[pr.y] = y[pr.x];
+                } else if constexpr (MODE == DSS_MASK) {
+                    y[pr.y] = 0;
+                } else {
+                    T s = y[pr.x] + y[pr.y];
+                    s = finish(s, pr.x);
+                    y[pr.x] = s;
+                    y[pr.y] = s;
+                }
+            } else if (t < np + nc) {
+                const int k = (int)(t - np);
+                const int b = a.c_off[k], e = a.c_off[k + 1];
+                if constexpr (MODE == DSS_UNMASK) {
+                    const T v = y[a.c_pos[b]];
+                    for (int q = b + 1; q < e; ++q) y[a.c_pos[q]] = v;
+                } else if constexpr (MODE == DSS_MASK) {
+                    for (int q = b + 1; q < e; ++q) y[a.c_pos[q]] = 0;
+                } else {
+                    T s = 0;
+                    for (int q = b; q < e; ++q) s += y[a.c_pos[q]];
+                    s = finish(s, a.c_pos[b]);
+                    for (int q = b; q < e; ++q) y[a.c_pos[q]] = s;
+                }
+            } else if (t < np + nc + nb) {
+                if constexpr (MODE == DSS_SUM_ZB || MODE == DSS_MASK || MODE == DSS_MINRES || MODE == DSS_PCG)
+                    y[a.bnd[t - np - nc]] = 0;
+            }
+        }
+    }
+    if constexpr (MODE == DSS_MINRES || MODE == DSS_PCG) {
+        const double s = block_sum256(part);
+        if (threadIdx.x == 0) a.partial[blockIdx.x] = s;
+    }
+}
+
+template <typename T>
+void launch_dss(int mode, const DssArgs<T>& a, hipStream_t s)
+{
+    const long total = (long)a.n_pairs + a.n_corners + a.n_bnd;
+    const int nb = dss_blocks(a);
+    if (total == 0 && mode != DSS_MINRES && mode != DSS_PCG) return;
+    dim3 g(nb), b(256);
+    switch (mode) {
+    case DSS_SUM: hipLaunchKernelGGL((k_dss<T, DSS_SUM>), g, b, 0, s, a); break;
+    case DSS_SUM_ZB: hipLaunchKernelGGL((k_dss<T, DSS_SUM_ZB>), g, b, 0, s, a); break;
+    case DSS_UNMASK: hipLaunchKernelGGL((k_dss<T, DSS_UNMASK>), g, b, 0, s, a); break;
+    case DSS_MASK: hipLaunchKernelGGL((k_dss<T, DSS_MASK>), g, b, 0, s, a); break;
+    case DSS_MINRES: hipLaunchKernelGGL((k_dss<T, DSS_MINRES>), g, b, 0, s, a); break;
+    case DSS_PCG: hipLaunchKernelGGL((k_dss<T, DSS_PCG>), g, b, 0, s, a); break;
+    }
+}
+template void launch_dss<double>(int, const DssArgs<double>&, hipStream_t);
+template void launch_dss<float>(int, const DssArgs<float>&, hipStream_t);
+
+// ---------------------------------------------------- elementwise kernels --
+// mode 0: out = ((x0 + c1 x1) + c2 x2) + ...
+// mode 1: out = ((c0 x0 + c1 x1) + c2 x2) + ...
+// mode 2: out = out + (((c0 x0 + c1 x1) + c2 x2) + ...)
+template <typename T>
+__global__ __launch_bounds__(256) void k_lincomb(LinComb<T> a)
+{
+    for (size_t k = (size_t)blockIdx.x * 256 + threadIdx.x; k < a.n; k += (size_t)gridDim.x * 256) {
+        T s;
+        if (a.mode == 0) s = a.x[0][k];
+        else s = (T)a.c[0] * a.x[0][k];
+        for (int t = 1; t < a.m; ++t) s = s + (T)a.c[t] * a.x[t][k];
+        if (a.mode == 2) a.out[k] = a.out[k] + s;
+        else a.out[k] = s;
+    }
+}
+
+template <typename T>
+void launch_lincomb(const LinComb<T>& a, hipStream_t s)
+{
+    if (a.n == 0) return;
+    const int nb = (int)std::min<size_t>((a.n + 255) / 256, 8192);
+    hipLaunchKernelGGL((k_lincomb<T>), dim3(nb), dim3(256), 0, s, a);
+}
+template void launch_lincomb<double>(const LinComb<double>&, hipStream_t);
+template void launch_lincomb<float>(const LinComb<float>&, hipStream_t);
+
+template <typename D, typename S>
+__global__ __launch_bounds__(256) void k_cast(D* out, const S* in, size_t n)
+{
+    for (size_t k = (size_t)blockIdx.x * 256 + threadIdx.x; k < n; k += (size_t)gridDim.x * 256) out[k] = (D)in[k];
+}
+
+template <typename D, typename S>
+void launch_cast(D* out, const S* in, size_t n, hipStream_t s)
+{
+    if (n == 0) return;
+    const int nb = (int)std::min<size_t>((n + 255) / 256, 8192);
+    hipLaunchKernelGGL((k_cast<D, S>), dim3(nb), dim3(256), 0, s, out, in, n);
+}
+template void launch_cast<float, double>(float*, const double*, size_t, hipStream_t);
+template void launch_cast<double, float>(double*, const float*, size_t, hipStream_t);
+template void launch_cast<double, double>(double*, const double*, size_t, hipStream_t);
+template void launch_cast<float, float>(float*, const float*, size_t, hipStream_t);
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_fill(T* out, T v, size_t n)
+{
+    for (size_t k = (size_t)blockIdx.x * 256 + threadIdx.x; k < n; k += (size_t)gridDim.x * 256) out[k] = v;
+}
+
+template <typename T>
+void launch_fill(T* out, T v, size_t n, hipStream_t s)
+{
+    if (n == 0) return;
+    const int nb = (int)std::min<size_t>((n + 255) / 256, 8192);
+    hipLaunchKernelGGL((k_fill<T>), dim3(nb), dim3(256), 0, s, out, v, n);
+}
+template void launch_fill<double>(double*, double, size_t, hipStream_t);
+template void launch_fill<float>(float*, float, size_t, hipStream_t);
+
+// x (+)= y elementwise, y = x * z (pointwise product) used to premultiply fields
+template <typename T>
+__global__ __launch_bounds__(256) void k_mul(T* out, const T* a, const T* b, size_t n)
+{
+    for (size_t k = (size_t)blockIdx.x * 256 + threadIdx.x; k < n; k += (size_t)gridDim.x * 256) out[k] = a[k] * b[k];
+}
+
+template <typename T>
+void launch_mul(T* out, const T* a, const T* b, size_t n, hipStream_t s)
+{
+    if (n == 0) return;
+    const int nb = (int)std::min<size_t>((n + 255) / 256, 8192);
+    hipLaunchKernelGGL((k_mul<T>), dim3(nb), dim3(256), 0, s, out, a, b, n);
+}
+template void launch_mul<double>(double*, const double*, const double*, size_t, hipStream_t);
+
+// ------------------------------------------------------------ reductions --
+// owner-weighted sum over ncomp components of a*b (unique-DOF dot,
+// openmp_impl/dot.hpp:11-61); one block of 64 threads per chunk, lane = column.
+template <typename T>
+__global__ __launch_bounds__(64) void k_wdot(const T* a, const T* b, size_t len, int ncomp, int N, int K,
+                                            const uint8_t* obits, double* partial)
+{
+    const int lane = threadIdx.x, KN = K * N;
+    double part = 0;
+    if (lane < KN) {
+        const int el = lane / N, j = lane % N;
+        const int e = blockIdx.x * K + el;
+        const unsigned bits = obits[e];
+        const size_t base = (size_t)blockIdx.x * K * N * N + lane;
+        for (int c = 0; c < ncomp; ++c)
+            for (int i = 0; i < N; ++i) {
+                const size_t k = (size_t)c * len + base + (size_t)i * KN;
+                part += owner_weight(i, j, N, bits) * ((double)a[k] * (double)b[k]);
+            }
+    }
+    part = wsum(part);
+    if (lane == 0) partial[blockIdx.x] = part;
+}
+
+template <typename T>
+void launch_wdot(const T* a, const T* b, size_t len, int ncomp, const Layout& L, const uint8_t* obits, double* partial,
+                 hipStream_t s)
+{
+    hipLaunchKernelGGL((k_wdot<T>), dim3(L.nchunks), dim3(64), 0, s, a, b, len, ncomp, L.N, L.K, obits, partial);
+}
+template void launch_wdot<double>(const double*, const double*, size_t, int, const Layout&, const uint8_t*, double*,
+                                  hipStream_t);
+template void launch_wdot<float>(const float*, const float*, size_t, int, const Layout&, const uint8_t*, double*,
+                                 hipStream_t);
+
+// plain Euclidean dot (generic solvers), grid-stride with fixed grid
+template <typename T>
+__global__ __launch_bounds__(256) void k_edot(const T* a, const T* b, size_t n, double* partial)
+{
+    double part = 0;
+    for (size_t k = (size_t)blockIdx.x * 256 + threadIdx.x; k < n; k += (size_t)gridDim.x * 256)
+        part += (double)a[k] * (double)b[k];
+    const double s = block_sum256(part);
+    if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+int launch_edot(const double* a, const double* b, size_t n, double* partial, hipStream_t s)
+{
+    const int nb = (int)std::max<size_t>(1, std::min<size_t>((n + 255) / 256, 1024));
+    hipLaunchKernelGGL((k_edot<double>), dim3(nb), dim3(256), 0, s, a, b, n, partial);
+    return nb;
+}
+
+// single-block fixed-order sum of n partials into *dst
+__device__ __forceinline__ double sum_partials(const double* p, int n)
+{
+    double s = 0;
+    for (int k = threadIdx.x; k < n; k += 256) s += p[k];
+    return block_sum256(s);
+}
+
+__global__ __launch_bounds__(256) void k_fin_sum(const double* partial, int n, double* dst)
+{
+    const double s = sum_partials(partial, n);
+    if (threadIdx.x == 0) *dst = s;
+}
+
+void launch_fin_sum(const double* partial, int n, double* dst, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_fin_sum, dim3(1), dim3(256), 0, s, partial, n, dst);
+}
+
+// --------------------------------------------------------- MINRES scalars --
+// minres.hpp:24-44 given red[slot] = dot(r1, r1)
+__global__ void k_minres_init(KrylovScalars* sc, int slot, int maxit, double tol)
+{
+    if (threadIdx.x != 0) return;
+    const double r = sc->red[slot];
+    sc->b_norm = sqrt(r);
+    const double beta1 = sqrt(r);
+    sc->beta = beta1;
+    sc->beta_old = 0;
+    sc->tnorm = 0;
+    sc->dbar = 0;
+    sc->eps = 0;
+    sc->eps_old = 0;
+    sc->phibar = beta1;
+    sc->cs = -1;
+    sc->sn = 0;
+    sc->tol = tol;
+    sc->atol = tol;
+    sc->alpha = 0;
+    sc->ib = 1.0 / beta1;
+    sc->cr1 = 0;
+    sc->c2 = 0;
+    sc->p3_pending = 0;
+    sc->iter = 0;
+    sc->maxit = maxit;
+    sc->has_r1 = 0;
+    sc->success = 0;
+    sc->done = (maxit < 1) ? 1 : 0;
+}
+
+void launch_minres_init(KrylovScalars* sc, int slot, int maxit, double tol, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_minres_init, dim3(1), dim3(64), 0, s, sc, slot, maxit, tol);
+}
+
+// alpha = dot(v, y) (minres.hpp:54); c2 = alpha / beta (minres.hpp:55)
+__global__ __launch_bounds__(256) void k_minres_fin_alpha(KrylovScalars* sc, const double* partial, int n)
+{
+    if (sc->done) {
+        // the operator kernel of this step applied the pending w/x update
+        if (threadIdx.x == 0) sc->p3_pending = 0;
+        return;
+    }
+    const double s = sum_partials(partial, n);
+    if (threadIdx.x == 0) {
+        sc->alpha = s;
+        sc->c2 = s / sc->beta;
+    }
+}
+
+void launch_minres_fin_alpha(KrylovScalars* sc, const double* partial, int n, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_minres_fin_alpha, dim3(1), dim3(256), 0, s, sc, partial, n);
+}
+
+// beta = sqrt(dot(y,y)) and the Givens update of minres.hpp:62-99.
+__global__ __launch_bounds__(256) void k_minres_fin_beta(KrylovScalars* sc, const double* partial, int n)
+{
+    if (sc->done) return;
+    const double s = sum_partials(partial, n);
+    if (threadIdx.x != 0) return;
+    const double alpha = sc->alpha;
+    const double beta_prev = sc->beta;
+    const double ib_prev = sc->ib;
+    sc->beta_old = beta_prev;
+    const double beta = sqrt(s);
+    sc->beta = beta;
+    sc->tnorm += alpha * alpha + beta * beta + beta_prev * beta_prev;
+    const double eps_old = sc->eps;
+    sc->eps_old = eps_old;
+    const double cs = sc->cs, sn = sc->sn, dbar = sc->dbar;
+    const double delta = cs * dbar + sn * alpha;
+    const double gbar = sn * dbar - cs * alpha;
+    sc->eps = sn * beta;
+    sc->dbar = -cs * beta;
+    const double gamma = fmax(sqrt(gbar * gbar + beta * beta), DBL_EPSILON);
+    sc->cs = gbar / gamma;
+    sc->sn = beta / gamma;
+    const double phi = sc->cs * sc->phibar;
+    sc->phibar *= sc->sn;
+    // pending w/x update of this step (minres.hpp:82-86)
+    sc->p3_ibp = ib_prev;
+    sc->p3_ig = 1.0 / gamma;
+    sc->p3_eo = eps_old;
+    sc->p3_dl = delta;
+    sc->p3_phi = phi;
+    sc->p3_pending = 1;
+    // coefficients of the next Lanczos step (minres.hpp:48-52)
+    sc->ib = 1.0 / beta;
+    sc->cr1 = beta / beta_prev;
+    sc->has_r1 = 1;
+    sc->iter += 1;
+    if (sc->phibar < sc->tol * sc->b_norm + sc->atol) {  // minres.hpp:92 (atol = tol there)
+        sc->success = 1;
+        sc->done = 1;
+    } else if (sc->iter >= sc->maxit) {
+        sc->done = 1;
+    }
+}
+
+void launch_minres_fin_beta(KrylovScalars* sc, const double* partial, int n, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_minres_fin_beta, dim3(1), dim3(256), 0, s, sc, partial, n);
+}
+
+// y -= c2 * r2 (minres.hpp:55) and the beta^2 partial (minres.hpp:63)
+template <typename T>
+__global__ __launch_bounds__(64) void k_minres_p2(T* y, const T* r2, size_t len, int ncomp, int N, int K,
+                                                 const uint8_t* obits, const double* scp, double* partial)
+{
+    const KrylovScalars* sc = reinterpret_cast<const KrylovScalars*>(scp);
+    const int lane = threadIdx.x, KN = K * N;
+    double part = 0;
+    if (!sc->done && lane < KN) {
+        const T c2 = (T)sc->c2;
+        const int el = lane / N, j = lane % N;
+        const unsigned bits = obits[blockIdx.x * K + el];
+        const size_t base = (size_t)blockIdx.x * K * N * N + lane;
+        for (int c = 0; c < ncomp; ++c)
+            for (int i = 0; i < N; ++i) {
+                const size_t k = (size_t)c * len + base + (size_t)i * KN;
+                const T v = y[k] - c2 * r2[k];
+                y[k] = v;
+                part += owner_weight(i, j, N, bits) * ((double)v * (double)v);
+            }
+    }
+    part = wsum(part);
+    if (lane == 0) partial[blockIdx.x] = part;
+}
+
+template <typename T>
+void launch_minres_p2(T* y, const T* r2, size_t len, int ncomp, const Layout& L, const uint8_t* obits,
+                      const double* sc, double* partial, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_minres_p2<T>), dim3(L.nchunks), dim3(64), 0, s, y, r2, len, ncomp, L.N, L.K, obits, sc,
+                       partial);
+}
+template void launch_minres_p2<double>(double*, const double*, size_t, int, const Layout&, const uint8_t*,
+                                       const double*, double*, hipStream_t);
+template void launch_minres_p2<float>(float*, const float*, size_t, int, const Layout&, const uint8_t*, const double*,
+                                      double*, hipStream_t);
+
+// final pending w/x update after the loop ends (minres.hpp:82-86)
+template <typename T>
+__global__ __launch_bounds__(256) void k_minres_flush(const T* r1, const T* wa, const T* wb, T* wn, T* x, size_t n,
+                                                     const double* scp)
+{
+    const KrylovScalars* sc = reinterpret_cast<const KrylovScalars*>(scp);
+    if (!sc->p3_pending) return;
+    const T ibp = (T)sc->p3_ibp, ig = (T)sc->p3_ig, eo = (T)sc->p3_eo, dl = (T)sc->p3_dl, phi = (T)sc->p3_phi;
+    for (size_t k = (size_t)blockIdx.x * 256 + threadIdx.x; k < n; k += (size_t)gridDim.x * 256) {
+        const T v = ibp * r1[k];
+        const T w = ig * ((v - eo * wa[k]) - dl * wb[k]);
+        wn[k] = w;
+        x[k] += phi * w;
+    }
+}
+
+__global__ void k_clear_pending(KrylovScalars* sc)
+{
+    if (threadIdx.x == 0) sc->p3_pending = 0;
+}
+
+template <typename T>
+void launch_minres_flush(const T* r1, const T* wa, const T* wb, T* wn, T* x, size_t n, KrylovScalars* sc,
+                         hipStream_t s)
+{
+    const int nb = (int)std::min<size_t>((n + 255) / 256, 8192);
+    hipLaunchKernelGGL((k_minres_flush<T>), dim3(nb), dim3(256), 0, s, r1, wa, wb, wn, x, n,
+                       reinterpret_cast<const double*>(sc));
+    hipLaunchKernelGGL(k_clear_pending, dim3(1), dim3(64), 0, s, sc);
+}
+template void launch_minres_flush<double>(const double*, const double*, const double*, double*, double*, size_t,
+                                          KrylovScalars*, hipStream_t);
+template void launch_minres_flush<float>(const float*, const float*, const float*, float*, float*, size_t,
+                                         KrylovScalars*, hipStream_t);
+
+// ------------------------------------------------------------ PCG scalars --
+// red[0] = dot(b,b), red[1] = dot(r,r) (pcg.hpp:62-63, :24)
+__global__ void k_pcg_init(KrylovScalars* sc, int maxit, double tol)
+{
+    if (threadIdx.x != 0) return;
+    sc->bnorm = sqrt(sc->red[0]);
+    sc->rho = sc->red[1];
+    sc->rho_prev = 0;
+    sc->pcg_beta = 0;
+    sc->pcg_alpha = 0;
+    sc->tol = tol;
+    sc->first = 1;
+    sc->iter = 0;
+    sc->maxit = maxit;
+    sc->success = 0;
+    sc->resid = sqrt(sc->rho);
+    sc->done = (maxit < 1) ? 1 : 0;
+}
+
+void launch_pcg_init(KrylovScalars* sc, int maxit, double tol, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_pcg_init, dim3(1), dim3(64), 0, s, sc, maxit, tol);
+}
+
+// alpha = rho / dot(p, Ap) (pcg.hpp:78)
+__global__ __launch_bounds__(256) void k_pcg_fin_alpha(KrylovScalars* sc, const double* partial, int n)
+{
+    if (sc->done) return;
+    const double s = sum_partials(partial, n);
+    if (threadIdx.x == 0) sc->pcg_alpha = sc->rho / s;
+}
+
+void launch_pcg_fin_alpha(KrylovScalars* sc, const double* partial, int n, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_pcg_fin_alpha, dim3(1), dim3(256), 0, s, sc, partial, n);
+}
+
+// x += alpha p; r -= alpha Ap (pcg.hpp:79-80); partial of the next rho = dot(r,r)
+template <typename T>
+__global__ __launch_bounds__(64) void k_pcg_upd(T* x, T* r, const T* p, const T* ap, size_t len, int N, int K,
+                                               const uint8_t* obits, const double* scp, double* partial)
+{
+    const KrylovScalars* sc = reinterpret_cast<const KrylovScalars*>(scp);
+    const int lane = threadIdx.x, KN = K * N;
+    double part = 0;
+    if (!sc->done && lane < KN) {
+        const T al = (T)sc->pcg_alpha;
+        const int el = lane / N, j = lane % N;
+        const unsigned bits = obits[blockIdx.x * K + el];
+        const size_t base = (size_t)blockIdx.x * K * N * N + lane;
+        for (int i = 0; i < N; ++i) {
+            const size_t k = base + (size_t)i * KN;
+            x[k] += al * p[k];
+            const T v = r[k] - al * ap[k];
+            r[k] = v;
+            part += owner_weight(i, j, N, bits) * ((double)v * (double)v);
+        }
+    }
+    part = wsum(part);
+    if (lane == 0) partial[blockIdx.x] = part;
+}
+
+void launch_pcg_upd(double* x, double* r, const double* p, const double* ap, size_t len, const Layout& L,
+                    const uint8_t* obits, const double* sc, double* partial, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_pcg_upd<double>), dim3(L.nchunks), dim3(64), 0, s, x, r, p, ap, len, L.N, L.K, obits, sc,
+                       partial);
+}
+
+// stop test on the pre-update rho (pcg.hpp:82-88), then rho for the next step
+__global__ __launch_bounds__(256) void k_pcg_fin_rho(KrylovScalars* sc, const double* partial, int n)
+{
+    if (sc->done) return;
+    const double s = sum_partials(partial, n);
+    if (threadIdx.x != 0) return;
+    const int i = sc->iter;  // 0-based index of the step just finished
+    sc->rho_prev = sc->rho;
+    sc->resid = sqrt(sc->rho);
+    if (sqrt(sc->rho) < sc->tol * sc->bnorm) {
+        sc->success = 1;
+        sc->done = 1;
+        return;
+    }
+    sc->iter = i + 1;
+    if (sc->iter >= sc->maxit) {
+        sc->done = 1;
+        return;
+    }
+    sc->rho = s;
+    sc->pcg_beta = s / sc->rho_prev;
+    sc->first = 0;
+}
+
+void launch_pcg_fin_rho(KrylovScalars* sc, const double* partial, int n, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_pcg_fin_rho, dim3(1), dim3(256), 0, s, sc, partial, n);
+}
+
+// ---------------------------------------------------- tridiagonal operator --
+__global__ __launch_bounds__(256) void k_tridiag(const double* x, double* y, long n, double lo, double di, double up)
+{
+    for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) {
+        double s = di * x[i];
+        if (i > 0) s += lo * x[i - 1];
+        if (i + 1 < n) s += up * x[i + 1];
+        y[i] = s;
+    }
+}
+
+void launch_tridiag(const double* x, double* y, long n, double lo, double di, double up, hipStream_t s)
+{
+    const int nb = (int)std::max<long>(1, std::min<long>((n + 255) / 256, 4096));
+    hipLaunchKernelGGL(k_tridiag, dim3(nb), dim3(256), 0, s, x, y, n, lo, di, up);
+}
+
+}  // namespace sem

@@ -1,0 +1,88 @@
+// sem_kernels.hpp -- argument structs and launchers of kernels_vec.hip.
+#pragma once
+
+#include "sem_internal.hpp"
+
+namespace sem {
+
+// DSS work modes (one launch covers interior-edge pairs, interior corners and
+// Dirichlet entries):
+//   DSS_SUM     global_sum (global_sum.hpp:65-77): sums to every copy, boundary untouched
+//   DSS_SUM_ZB  global_sum then zero Dirichlet entries, sums scaled by post[c]
+//   DSS_UNMASK  unmask (unmask.hpp:43-55): owner value to every copy
+//   DSS_MASK    mask (mask.hpp:16-28): zero non-owner copies and Dirichlet entries
+//   DSS_MINRES  DSS_SUM_ZB fused with y -= cr1*r1 and the alpha partial
+//   DSS_PCG     DSS_SUM_ZB fused with the p.Ap partial
+enum DssMode { DSS_SUM = 0, DSS_SUM_ZB = 1, DSS_UNMASK = 2, DSS_MASK = 3, DSS_MINRES = 4, DSS_PCG = 5 };
+
+template <typename T>
+struct DssArgs {
+    T* y;
+    size_t len;
+    int ncomp;
+    const int2* pairs;
+    int n_pairs;
+    const int* c_off;
+    const int* c_pos;
+    int n_corners;
+    const int* bnd;
+    int n_bnd;
+    T post[2];
+    const T* r1;  // MINRES: R[i-2]
+    const T* r2;  // MINRES: R[i-1] (v = ib*r2); PCG: search direction p
+    const double* sc;
+    double* partial;
+};
+
+template <typename T>
+inline int dss_blocks(const DssArgs<T>& a)
+{
+    const long total = (long)a.n_pairs + a.n_corners + a.n_bnd;
+    return (int)std::max<long>(1, (total + 255) / 256);
+}
+
+template <typename T>
+struct LinComb {
+    T* out;
+    const T* x[6];
+    double c[6];
+    int m;
+    int mode;
+    size_t n;
+};
+
+template <typename T>
+void launch_dss(int mode, const DssArgs<T>& a, hipStream_t s);
+template <typename T>
+void launch_lincomb(const LinComb<T>& a, hipStream_t s);
+template <typename D, typename S>
+void launch_cast(D* out, const S* in, size_t n, hipStream_t s);
+template <typename T>
+void launch_fill(T* out, T v, size_t n, hipStream_t s);
+template <typename T>
+void launch_mul(T* out, const T* a, const T* b, size_t n, hipStream_t s);
+template <typename T>
+void launch_wdot(const T* a, const T* b, size_t len, int ncomp, const Layout& L, const uint8_t* obits,
+                 double* partial, hipStream_t s);
+int launch_edot(const double* a, const double* b, size_t n, double* partial, hipStream_t s);
+void launch_fin_sum(const double* partial, int n, double* dst, hipStream_t s);
+
+void launch_minres_init(KrylovScalars* sc, int slot, int maxit, double tol, hipStream_t s);
+void launch_minres_fin_alpha(KrylovScalars* sc, const double* partial, int n, hipStream_t s);
+void launch_minres_fin_beta(KrylovScalars* sc, const double* partial, int n, hipStream_t s);
+template <typename T>
+void launch_minres_p2(T* y, const T* r2, size_t len, int ncomp, const Layout& L, const uint8_t* obits,
+                      const double* sc, double* partial, hipStream_t s);
+template <typename T>
+void launch_minres_flush(const T* r1, const T* wa, const T* wb, T* wn, T* x, size_t n, KrylovScalars* sc,
+                         hipStream_t s);
+
+void launch_pcg_init(KrylovScalars* sc, int maxit, double tol, hipStream_t s);
+void launch_pcg_fin_alpha(KrylovScalars* sc, const double* partial, int n, hipStream_t s);
+void launch_pcg_upd(double* x, double* r, const double* p, const double* ap, size_t len, const Layout& L,
+                    const uint8_t* obits, const double* sc, double* partial, hipStream_t s);
+void launch_pcg_fin_rho(KrylovScalars* sc, const double* partial, int n, hipStream_t s);
+
+void launch_tridiag(const double* x, double* y, long n, double lo, double di, double up, hipStream_t s);
+
+}  // namespace sem

@@ -1,0 +1,64 @@
+"""CPU checks of the C-ABI library: it loads, exports every function that
+include/sem.h declares, the host-only entry points work without a GPU, and
+GPU entry points fail loudly (no CPU fallback) when no GPU is present."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from conftest import ROOT, SMALL_MESH
+
+HEADER = os.path.join(ROOT, "include", "sem.h")
+
+
+def declared_functions():
+    txt = open(HEADER).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    names = set(re.findall(r"\b(sem_[a-z0-9_]+)\s*\(", txt))
+    return sorted(n for n in names if not n.endswith("_cb"))
+
+
+def test_library_loads_and_exports_header(sem):
+    lib = ctypes.CDLL(sem.LIB_PATH, mode=os.RTLD_NOW)
+    names = declared_functions()
+    assert len(names) > 30
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_structured_mesh_generator(sem):
+    xy, q = sem.structured_mesh(8, (0, 1, 0, 1), 0.1, 0)
+    assert xy.shape == (81, 2) and q.shape == (64, 4)
+    h = 1 / 8
+    g = np.stack(np.meshgrid(np.arange(9) * h, np.arange(9) * h), -1).reshape(-1, 2)
+    d = np.abs(xy - g)
+    assert d.max() <= 0.1 * h + 1e-15
+    border = (g[:, 0] == 0) | (g[:, 0] == 1) | (g[:, 1] == 0) | (g[:, 1] == 1)
+    assert np.all(d[border] == 0)
+    assert np.all(d[~border].max(axis=0) > 0.05 * h)
+    # counter-clockwise quads: positive shoelace area, total area 1
+    P = xy[q]
+    area = 0.5 * np.sum(P[:, :, 0] * np.roll(P[:, :, 1], -1, 1) - np.roll(P[:, :, 0], -1, 1) * P[:, :, 1], axis=1)
+    assert np.all(area > 0) and abs(area.sum() - 1) < 1e-14
+    # seeded and reproducible; golden fixture of the config-1 mesh
+    xy2, _ = sem.structured_mesh(8, (0, 1, 0, 1), 0.1, 0)
+    assert np.array_equal(xy, xy2)
+    gold = np.loadtxt(os.path.join(ROOT, "tests", "golden", "config1_mesh", "coordinates.txt"))
+    assert np.array_equal(gold, xy)
+
+
+def test_mesh_write_roundtrip(sem, orc, tmp_path):
+    xy, q = sem.structured_mesh(5, (-1, 1, -1, 1), 0.1, 3)
+    sem.write_mesh(str(tmp_path), xy, q)
+    m = orc.Mesh.load(3, str(tmp_path))
+    assert m.n_elements == 25 and m.n_nodes == 36 and m.dof() == 36 + 60 * 2 + 25 * 4
+
+
+def test_gpu_entry_points_fail_loudly_without_gpu(sem):
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    with pytest.raises(sem.SemError, match="HIP|device"):
+        sem.load_mesh(3, SMALL_MESH)
