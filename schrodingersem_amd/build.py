@@ -59,6 +59,20 @@ def build_lib(jobs=None):
     return LIB
 
 
+def build_cpp_tests():
+    """The C++ drop-in test (reference test sequence against include/sem.hpp)."""
+    out_dir = os.path.join(ROOT, "tests", "cpp", "bin")
+    os.makedirs(out_dir, exist_ok=True)
+    out = os.path.join(out_dir, "test_drop_in")
+    cmd = ["g++", "-std=c++20", "-O2", "-I" + os.path.join(ROOT, "include"),
+           os.path.join(ROOT, "tests", "cpp", "test_drop_in.cpp"), "-o", out, "-L" + HERE, "-lsem_hip",
+           "-Wl,-rpath,$ORIGIN/../../../schrodingersem_amd"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("C++ drop-in test build failed:\n%s" % r.stderr[-4000:])
+    return out
+
+
 def build_oracle():
     r = subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle")], capture_output=True, text=True)
     if r.returncode != 0:
@@ -72,6 +86,7 @@ def main(argv=None):
     ap.add_argument("--skip-oracle", action="store_true")
     a = ap.parse_args(argv)
     print(build_lib(a.jobs))
+    print(build_cpp_tests())
     if not a.skip_oracle:
         print(build_oracle())
 
