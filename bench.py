@@ -130,6 +130,10 @@ def main():
                 "kernel": "k_minres_op<9,%s,RES>" % ("double" if best == "fp64" else "float"),
                 "avg_us": round(avg_us, 2), "launches": launches, "bytes_per_launch": bpl}
 
+    copy_us = sem.time_copy(1 << 30, 10)
+    copy_gbs = 2.0 * (1 << 30) / copy_us / 1e3
+    log("device copy 1 GiB: %.1f us -> %.0f GB/s (read + write)" % (copy_us, copy_gbs))
+
     glap = None
     if not args.skip_glaplace:
         xyb, qb = sem.structured_mesh(512, (0.0, 1.0, 0.0, 1.0), 0.0, 0)
@@ -187,6 +191,7 @@ def main():
         "roofline": roofline,
         "glaplace": glap,
         "cpu_baseline": cpu,
+        "copy_GBs_measured": round(copy_gbs, 1),
         "device": dev[0],
     }
     print(json.dumps(line), flush=True)

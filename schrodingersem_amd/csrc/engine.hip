@@ -1383,6 +1383,10 @@ int sem_stage_solve(sem_mesh* m, const double* V, double alpha, const double* b,
         double* db = m->buf<double>("ss_b", n);
         double* dz = m->buf<double>("ss_z", n);
         m->upload(db, b, 2);
+        // b arrives masked (as galerkin_op returns it); the solver works on its
+        // consistent form: owner values on every copy, Dirichlet entries zero
+        dss<double>(m, DSS_UNMASK, db, 2);
+        dss<double>(m, DSS_ZERO_BND, db, 2);
         m->upload(dz, z, 2);
         dss<double>(m, DSS_UNMASK, dz, 2);
         KrylovOpts o;
@@ -1451,6 +1455,44 @@ int sem_pcg_tridiag(long n, double lo, double di, double up, const double* b, do
                     sem_solver_results* out)
 {
     return tridiag_solve(false, n, lo, di, up, b, x, maxit, tol, out);
+}
+
+int sem_time_copy(long bytes, int reps, double* us_per_launch)
+{
+    return guarded([&] {
+        if (bytes < 8 || reps < 1) throw std::invalid_argument("sem_time_copy: bad argument");
+        if (!check_gpu()) return (int)SEM_EHIP;
+        const size_t n = (size_t)bytes / sizeof(double);
+        double *a = nullptr, *b = nullptr;
+        hipStream_t s = nullptr;
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        int rc = SEM_OK;
+        try {
+            SEM_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+            SEM_HIP(hipMalloc(&a, n * sizeof(double)));
+            SEM_HIP(hipMalloc(&b, n * sizeof(double)));
+            SEM_HIP(hipEventCreate(&e0));
+            SEM_HIP(hipEventCreate(&e1));
+            launch_fill<double>(a, 1.0, n, s);
+            launch_cast<double, double>(b, a, n, s);
+            SEM_HIP(hipEventRecord(e0, s));
+            for (int r = 0; r < reps; ++r) launch_cast<double, double>(b, a, n, s);
+            SEM_HIP(hipEventRecord(e1, s));
+            SEM_HIP(hipEventSynchronize(e1));
+            float ms = 0;
+            SEM_HIP(hipEventElapsedTime(&ms, e0, e1));
+            if (us_per_launch) *us_per_launch = 1000.0 * ms / reps;
+        } catch (const HipError&) {
+            rc = SEM_EHIP;
+        }
+        if (e0) hipEventDestroy(e0);
+        if (e1) hipEventDestroy(e1);
+        hipFree(a);
+        hipFree(b);
+        if (s) hipStreamDestroy(s);
+        if (rc != SEM_OK) throw HipError{hipErrorUnknown, "sem_time_copy"};
+        return rc;
+    });
 }
 
 int sem_set_kernel_timing(sem_mesh* m, int on)

@@ -88,6 +88,10 @@ __global__ __launch_bounds__(256) void k_dss(DssArgs<T> a)
                 }
                 return s;
             };
+            if constexpr (MODE == DSS_ZERO_BND) {
+                if (t >= np + nc && t < np + nc + nb) y[a.bnd[t - np - nc]] = 0;
+                continue;
+            }
             if (t < np) {
                 const int2 pr = a.pairs[t];
                 if constexpr (MODE == DSS_UNMASK) {
@@ -140,6 +144,7 @@ void launch_dss(int mode, const DssArgs<T>& a, hipStream_t s)
     case DSS_MASK: hipLaunchKernelGGL((k_dss<T, DSS_MASK>), g, b, 0, s, a); break;
     case DSS_MINRES: hipLaunchKernelGGL((k_dss<T, DSS_MINRES>), g, b, 0, s, a); break;
     case DSS_PCG: hipLaunchKernelGGL((k_dss<T, DSS_PCG>), g, b, 0, s, a); break;
+    case DSS_ZERO_BND: hipLaunchKernelGGL((k_dss<T, DSS_ZERO_BND>), g, b, 0, s, a); break;
     }
 }
 template void launch_dss<double>(int, const DssArgs<double>&, hipStream_t);

@@ -13,7 +13,8 @@ namespace sem {
 //   DSS_MASK    mask (mask.hpp:16-28): zero non-owner copies and Dirichlet entries
 //   DSS_MINRES  DSS_SUM_ZB fused with y -= cr1*r1 and the alpha partial
 //   DSS_PCG     DSS_SUM_ZB fused with the p.Ap partial
-enum DssMode { DSS_SUM = 0, DSS_SUM_ZB = 1, DSS_UNMASK = 2, DSS_MASK = 3, DSS_MINRES = 4, DSS_PCG = 5 };
+//   DSS_ZERO_BND zero Dirichlet entries only
+enum DssMode { DSS_SUM = 0, DSS_SUM_ZB = 1, DSS_UNMASK = 2, DSS_MASK = 3, DSS_MINRES = 4, DSS_PCG = 5, DSS_ZERO_BND = 6 };
 
 template <typename T>
 struct DssArgs {
