@@ -56,6 +56,8 @@ struct sem_mesh {
     int* c_pos = nullptr;
     int* bnd = nullptr;
     int dss_nb = 1;
+    int* eside = nullptr;    // per element side: neighbour descriptor or -1
+    int* ecorner = nullptr;  // per element corner: interior corner index or -1
     // D matrix
     std::vector<double> D64;
     std::vector<float> D32;
@@ -86,6 +88,8 @@ struct sem_mesh {
         hipFree(c_off);
         hipFree(c_pos);
         hipFree(bnd);
+        hipFree(eside);
+        hipFree(ecorner);
         hipFree(sc);
         hipFree(partial);
         if (h_sc) hipHostFree(h_sc);
@@ -110,6 +114,7 @@ struct sem_mesh {
     }
 
     size_t len() const { return L.len; }
+    unsigned* sc_counter(int k) { return &sc->cnt[k]; }
     template <typename T>
     const T* geo_t() const;
     template <typename T>
@@ -236,6 +241,60 @@ static void setup_device(sem_mesh* m)
         if (v < 0 || (size_t)v >= L.len) throw std::runtime_error("internal error: Dirichlet position out of range");
     for (int v : c_pos)
         if (v < 0 || (size_t)v >= L.len) throw std::runtime_error("internal error: corner position out of range");
+    // gather descriptors of the MINRES update kernel
+    std::vector<int> eside((size_t)L.nchunks * L.K * 4, -1), ecorner((size_t)L.nchunks * L.K * 4, -1);
+    for (const EdgeRec& r : hm.edges) {
+        if (r.type == BOUNDARY) continue;
+        const int s0 = r.side[0], s1 = std::abs(r.side[1]), rev = r.side[1] < 0 ? 1 : 0;
+        eside[4 * (size_t)r.el[0] + s0 - 1] = (r.el[1] << 3) | ((s1 - 1) << 1) | rev;
+        eside[4 * (size_t)r.el[1] + s1 - 1] = (r.el[0] << 3) | ((s0 - 1) << 1) | rev;
+    }
+    {
+        int ci = 0;
+        for (int c = 0; c < hm.n_nodes; ++c) {
+            if (hm.node_type[c] == BOUNDARY) continue;
+            for (int q = hm.conn_off[c]; q < hm.conn_off[c + 1]; ++q) ecorner[hm.conn[q]] = ci;
+            ++ci;
+        }
+    }
+    if ((size_t)hm.n_el >= (1u << 27)) throw std::runtime_error("mesh too large for packed side descriptors");
+    {
+        // the gather descriptors must name exactly the partners of the DSS pair lists
+        std::unordered_map<int, int> partner;
+        partner.reserve(2 * pairs.size());
+        for (const int2& pr : pairs) {
+            partner[pr.x] = pr.y;
+            partner[pr.y] = pr.x;
+        }
+        size_t seen = 0;
+        for (int e = 0; e < hm.n_el; ++e)
+            for (int sd = 1; sd <= 4; ++sd) {
+                const int d = eside[4 * (size_t)e + sd - 1];
+                if (d < 0) continue;
+                const int e2 = d >> 3, s2 = ((d >> 1) & 3) + 1;
+                for (int k = 1; k <= N - 2; ++k) {
+                    const int k2 = (d & 1) ? N - 1 - k : k;
+                    int i0, j0, i1, j1;
+                    side_ij(sd, k, i0, j0);
+                    side_ij(s2, k2, i1, j1);
+                    auto it = partner.find((int)L.pos(e, i0, j0));
+                    if (it == partner.end() || it->second != (int)L.pos(e2, i1, j1))
+                        throw std::runtime_error("internal error: side descriptor disagrees with the DSS pairs");
+                    ++seen;
+                }
+            }
+        if (seen != 2 * pairs.size()) throw std::runtime_error("internal error: side descriptors miss DSS pairs");
+        for (int e = 0; e < hm.n_el; ++e)
+            for (int k = 0; k < 4; ++k) {
+                const int ci = ecorner[4 * (size_t)e + k];
+                if (ci < -1 || ci >= (int)c_off.size() - 1) throw std::runtime_error("internal error: corner index");
+                if (ci >= 0) {
+                    bool found = false;
+                    for (int q = c_off[ci]; q < c_off[ci + 1]; ++q) found |= c_pos[q] == corner_pos(4 * e + k);
+                    if (!found) throw std::runtime_error("internal error: corner descriptor");
+                }
+            }
+    }
     m->n_pairs = (int)pairs.size();
     m->n_corners = (int)c_off.size() - 1;
     m->n_bnd = (int)bnd.size();
@@ -249,13 +308,15 @@ static void setup_device(sem_mesh* m)
     up(m->c_pos, c_pos);
     up(m->bnd, bnd);
     up(m->obits, m->h_obits);
+    up(m->eside, eside);
+    up(m->ecorner, ecorner);
     const long total = (long)m->n_pairs + m->n_corners + m->n_bnd;
     m->dss_nb = (int)std::max<long>(1, (total + 255) / 256);
 
     SEM_HIP(hipMalloc(&m->sc, sizeof(KrylovScalars)));
     SEM_HIP(hipMemset(m->sc, 0, sizeof(KrylovScalars)));
     SEM_HIP(hipHostMalloc(&m->h_sc, sizeof(KrylovScalars), hipHostMallocDefault));
-    m->n_partial = L.nchunks + m->dss_nb + 1024;
+    m->n_partial = 2 * L.nchunks + m->dss_nb + 1024;
     SEM_HIP(hipMalloc(&m->partial, (size_t)m->n_partial * sizeof(double)));
 }
 
@@ -352,6 +413,7 @@ static Results minres_fused(sem_mesh* m, int kind, const T* cf, double alpha, co
     const std::string tp = sizeof(T) == 8 ? "d" : "f";
     T* R[3] = {m->buf<T>("mr_R0" + tp, n), m->buf<T>("mr_R1" + tp, n), m->buf<T>("mr_R2" + tp, n)};
     T* W[3] = {m->buf<T>("mr_W0" + tp, n), m->buf<T>("mr_W1" + tp, n), m->buf<T>("mr_W2" + tp, n)};
+    T* Yraw = m->buf<T>("mr_Yraw" + tp, n);
     hipStream_t s = m->stream;
     if (x_zero) {
         launch_fill<T>(x, T(0), n, s);
@@ -398,7 +460,7 @@ static Results minres_fused(sem_mesh* m, int kind, const T* cf, double alpha, co
             MinresOpArgs<T> a{};
             a.r2 = R[(i - 1) % 3];
             a.r1 = R[(i + 1) % 3];
-            a.y = R[i % 3];
+            a.y = Yraw;
             a.wa = W[i % 3];
             a.wb = W[(i + 1) % 3];
             a.wn = W[(i + 2) % 3];
@@ -408,21 +470,31 @@ static Results minres_fused(sem_mesh* m, int kind, const T* cf, double alpha, co
             a.alpha = (T)alpha;
             a.len = m->len();
             a.nchunks = nch;
+            a.obits = m->obits;
             a.partial = m->partial;
-            a.sc = reinterpret_cast<const double*>(m->sc);
+            a.counter = m->sc_counter(0);
+            a.sc = reinterpret_cast<double*>(m->sc);
             if (m->timing) SEM_HIP(hipEventRecord(m->ev[2 * b_], s));
             launch_minres_op_d<T>(m->L.N, kind, a, m->dmat<T>(), s);
             if (m->timing) SEM_HIP(hipEventRecord(m->ev[2 * b_ + 1], s));
             ++launched;
-            DssArgs<T> d = dss_args<T>(m, R[i % 3], nc);
-            d.r1 = R[(i + 1) % 3];
-            d.r2 = R[(i - 1) % 3];
-            d.partial = m->partial + nch;
-            launch_dss<T>(DSS_MINRES, d, s);
-            launch_minres_fin_alpha(m->sc, m->partial, nch + m->dss_nb, s);
-            launch_minres_p2<T>(R[i % 3], R[(i - 1) % 3], m->len(), nc, m->L, m->obits,
-                                reinterpret_cast<const double*>(m->sc), m->partial, s);
-            launch_minres_fin_beta(m->sc, m->partial, nch, s);
+            MinresUpdArgs<T> u{};
+            u.yraw = Yraw;
+            u.y = R[i % 3];
+            u.r2 = R[(i - 1) % 3];
+            u.len = m->len();
+            u.ncomp = nc;
+            u.N = m->L.N;
+            u.K = m->L.K;
+            u.obits = m->obits;
+            u.eside = m->eside;
+            u.ecorner = m->ecorner;
+            u.c_off = m->c_off;
+            u.c_pos = m->c_pos;
+            u.partial = m->partial + nch;
+            u.counter = m->sc_counter(1);
+            u.sc = reinterpret_cast<double*>(m->sc);
+            launch_minres_upd<T>(u, nch, s);
         }
         m->read_scalars();
         if (m->timing) {

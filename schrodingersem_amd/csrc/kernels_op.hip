@@ -223,17 +223,39 @@ __device__ __forceinline__ double wave_sum(double v)
     return v;
 }
 
+// owner test of entry (i, l) of an element with ownership bits (bit s-1:
+// side s owned; bit 4+k: corner k owned)
+template <int N>
+__device__ __forceinline__ bool owns(int i, int l, unsigned bits)
+{
+    const bool ri = (i == 0 || i == N - 1), rj = (l == 0 || l == N - 1);
+    if (!ri && !rj) return true;
+    int bit;
+    if (ri && rj) bit = 4 + ((i == 0) ? ((l == 0) ? 0 : 3) : ((l == 0) ? 1 : 2));
+    else if (l == 0) bit = 0;
+    else if (i == N - 1) bit = 1;
+    else if (l == N - 1) bit = 2;
+    else bit = 3;
+    return (bits >> bit) & 1u;
+}
+
 // MINRES Lanczos step i, fused with the w/x update of step i-1
-// (minres.hpp:48-86): v = ib*r2; y = A v - cr1*r1 (interior entries; the DSS
-// kernel finishes ring entries); alpha partial = sum v.y over interior
-// entries. Pending update: w_new = ig*((ibp*r1 - eo*wa) - dl*wb); x += phi*w_new.
+// (minres.hpp:48-86): v = ib*r2, y = A v - cr1*r1. The subtraction is applied
+// on element-interior entries and on the owner copy of every interface node;
+// the update kernel then sums the copies, which gives the assembled
+// (A v)_owner - cr1*r1 everywhere. alpha = dot(v, y) is therefore the sum of
+// v*y over interior entries plus v*(copy value) over every interface copy,
+// computed here; the last block to finish adds the partials in block order and
+// sets c2 = alpha / beta (minres.hpp:54-55).
+// Pending update: w_new = ig*((ibp*r1 - eo*wa) - dl*wb); x += phi*w_new.
 template <int N, typename T, int KIND>
 __global__ __launch_bounds__(64) void k_minres_op(MinresOpArgs<T> a, Dm<N, T> D)
 {
     using S = Shape<N>;
     constexpr int NC = (KIND == K_RES) ? 2 : 1;
     __shared__ T lds0[S::LDS], lds1[S::LDS];
-    const KrylovScalars* sc = reinterpret_cast<const KrylovScalars*>(a.sc);
+    __shared__ int am_last;
+    KrylovScalars* sc = reinterpret_cast<KrylovScalars*>(a.sc);
     const int done = sc->done;
     const int p3 = sc->p3_pending;
     const int lane = threadIdx.x;
@@ -266,6 +288,7 @@ __global__ __launch_bounds__(64) void k_minres_op(MinresOpArgs<T> a, Dm<N, T> D)
     if (!done) {
         const T ib = (T)sc->ib, cr1 = (T)sc->cr1;
         const int has_r1 = sc->has_r1;
+        const unsigned bits = act ? a.obits[blockIdx.x * S::K + el] : 0u;
         T ga[N], gb[N], gc[N], cf[N];
         c.ld(a.geo, ga);
         c.ld(a.geo + len, gb);
@@ -279,11 +302,10 @@ __global__ __launch_bounds__(64) void k_minres_op(MinresOpArgs<T> a, Dm<N, T> D)
             T r[N];
             if (has_r1) c.ld(a.r1 + cc * len, r);
 #pragma unroll
-            for (int i = 0; i < N; ++i)
-                if (is_interior<N>(i, l)) {
-                    if (has_r1) y[i] = y[i] - cr1 * r[i];
-                    part += (double)q[i] * (double)y[i];
-                }
+            for (int i = 0; i < N; ++i) {
+                if (has_r1 && owns<N>(i, l, bits)) y[i] = y[i] - cr1 * r[i];
+                part += (double)q[i] * (double)y[i];
+            }
             c.st(a.y + cc * len, y);
         };
         if constexpr (KIND == K_RES) {
@@ -313,8 +335,30 @@ __global__ __launch_bounds__(64) void k_minres_op(MinresOpArgs<T> a, Dm<N, T> D)
             finish(0, y, q0);
         }
     }
+    // last-block reduction of the alpha partials
     part = wave_sum(part);
-    if (threadIdx.x == 0) a.partial[blockIdx.x] = part;
+    if (threadIdx.x == 0) {
+        a.partial[blockIdx.x] = part;
+        __threadfence();
+        const unsigned prev = atomicAdd(a.counter, 1u);
+        am_last = (prev == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (am_last) {
+        __threadfence();
+        double s = 0;
+        for (int k = threadIdx.x; k < (int)gridDim.x; k += 64) s += a.partial[k];
+        s = wave_sum(s);
+        if (threadIdx.x == 0) {
+            if (done) {
+                sc->p3_pending = 0;  // this launch applied the pending w/x update
+            } else {
+                sc->alpha = s;
+                sc->c2 = s / sc->beta;
+            }
+            *a.counter = 0;
+        }
+    }
 }
 
 // PCG step (pcg.hpp:70-80, identity preconditioner): p = r (first step) or

@@ -61,14 +61,9 @@ __global__ __launch_bounds__(256) void k_dss(DssArgs<T> a)
     const long np = a.n_pairs, nc = a.n_corners, nb = a.n_bnd;
     double part = 0;
     bool skip = false;
-    double ib = 1, cr1 = 0;
-    int has_r1 = 0;
-    if constexpr (MODE == DSS_MINRES || MODE == DSS_PCG) {
+    if constexpr (MODE == DSS_PCG) {
         const KrylovScalars* sc = reinterpret_cast<const KrylovScalars*>(a.sc);
         skip = sc->done != 0;
-        ib = sc->ib;
-        cr1 = sc->cr1;
-        has_r1 = sc->has_r1;
     }
     if (!skip) {
         for (int c = 0; c < a.ncomp; ++c) {
@@ -76,12 +71,6 @@ __global__ __launch_bounds__(256) void k_dss(DssArgs<T> a)
             const T post = a.post[c];
             auto finish = [&](T s, int p0) -> T {
                 if constexpr (MODE == DSS_SUM_ZB) return post * s;
-                if constexpr (MODE == DSS_MINRES) {
-                    T v = s;
-                    if (has_r1) v = v - (T)cr1 * a.r1[(size_t)c * a.len + p0];
-                    part += (double)((T)ib * a.r2[(size_t)c * a.len + p0]) * (double)v;
-                    return v;
-                }
                 if constexpr (MODE == DSS_PCG) {
                     part += (double)a.r2[(size_t)c * a.len + p0] * (double)s;
                     return s;
@@ -119,12 +108,12 @@ __global__ __launch_bounds__(256) void k_dss(DssArgs<T> a)
                     for (int q = b; q < e; ++q) y[a.c_pos[q]] = s;
                 }
             } else if (t < np + nc + nb) {
-                if constexpr (MODE == DSS_SUM_ZB || MODE == DSS_MASK || MODE == DSS_MINRES || MODE == DSS_PCG)
+                if constexpr (MODE == DSS_SUM_ZB || MODE == DSS_MASK || MODE == DSS_PCG)
                     y[a.bnd[t - np - nc]] = 0;
             }
         }
     }
-    if constexpr (MODE == DSS_MINRES || MODE == DSS_PCG) {
+    if constexpr (MODE == DSS_PCG) {
         const double s = block_sum256(part);
         if (threadIdx.x == 0) a.partial[blockIdx.x] = s;
     }
@@ -135,14 +124,13 @@ void launch_dss(int mode, const DssArgs<T>& a, hipStream_t s)
 {
     const long total = (long)a.n_pairs + a.n_corners + a.n_bnd;
     const int nb = dss_blocks(a);
-    if (total == 0 && mode != DSS_MINRES && mode != DSS_PCG) return;
+    if (total == 0 && mode != DSS_PCG) return;
     dim3 g(nb), b(256);
     switch (mode) {
     case DSS_SUM: hipLaunchKernelGGL((k_dss<T, DSS_SUM>), g, b, 0, s, a); break;
     case DSS_SUM_ZB: hipLaunchKernelGGL((k_dss<T, DSS_SUM_ZB>), g, b, 0, s, a); break;
     case DSS_UNMASK: hipLaunchKernelGGL((k_dss<T, DSS_UNMASK>), g, b, 0, s, a); break;
     case DSS_MASK: hipLaunchKernelGGL((k_dss<T, DSS_MASK>), g, b, 0, s, a); break;
-    case DSS_MINRES: hipLaunchKernelGGL((k_dss<T, DSS_MINRES>), g, b, 0, s, a); break;
     case DSS_PCG: hipLaunchKernelGGL((k_dss<T, DSS_PCG>), g, b, 0, s, a); break;
     case DSS_ZERO_BND: hipLaunchKernelGGL((k_dss<T, DSS_ZERO_BND>), g, b, 0, s, a); break;
     }
@@ -335,32 +323,10 @@ void launch_minres_init(KrylovScalars* sc, int slot, int maxit, double tol, hipS
     hipLaunchKernelGGL(k_minres_init, dim3(1), dim3(64), 0, s, sc, slot, maxit, tol);
 }
 
-// alpha = dot(v, y) (minres.hpp:54); c2 = alpha / beta (minres.hpp:55)
-__global__ __launch_bounds__(256) void k_minres_fin_alpha(KrylovScalars* sc, const double* partial, int n)
+// beta = sqrt(dot(y,y)) and the Givens update of minres.hpp:62-99, given
+// s = dot(y, y) of the step just finished.
+__device__ void minres_beta_update(KrylovScalars* sc, double s)
 {
-    if (sc->done) {
-        // the operator kernel of this step applied the pending w/x update
-        if (threadIdx.x == 0) sc->p3_pending = 0;
-        return;
-    }
-    const double s = sum_partials(partial, n);
-    if (threadIdx.x == 0) {
-        sc->alpha = s;
-        sc->c2 = s / sc->beta;
-    }
-}
-
-void launch_minres_fin_alpha(KrylovScalars* sc, const double* partial, int n, hipStream_t s)
-{
-    hipLaunchKernelGGL(k_minres_fin_alpha, dim3(1), dim3(256), 0, s, sc, partial, n);
-}
-
-// beta = sqrt(dot(y,y)) and the Givens update of minres.hpp:62-99.
-__global__ __launch_bounds__(256) void k_minres_fin_beta(KrylovScalars* sc, const double* partial, int n)
-{
-    if (sc->done) return;
-    const double s = sum_partials(partial, n);
-    if (threadIdx.x != 0) return;
     const double alpha = sc->alpha;
     const double beta_prev = sc->beta;
     const double ib_prev = sc->ib;
@@ -400,47 +366,103 @@ __global__ __launch_bounds__(256) void k_minres_fin_beta(KrylovScalars* sc, cons
     }
 }
 
-void launch_minres_fin_beta(KrylovScalars* sc, const double* partial, int n, hipStream_t s)
-{
-    hipLaunchKernelGGL(k_minres_fin_beta, dim3(1), dim3(256), 0, s, sc, partial, n);
-}
-
-// y -= c2 * r2 (minres.hpp:55) and the beta^2 partial (minres.hpp:63)
+// The second kernel of a Lanczos step. Every entry of y is finished here:
+// interface copies gather the unassembled values of all copies of their node
+// (edge: owner + other, global_sum.hpp:28-34; corner: left fold from +0.0 in
+// connected_elements order, global_sum.hpp:54-63) -- each copy computes the
+// same sum, bit for bit -- Dirichlet entries become zero, then
+// y -= c2*r2 (minres.hpp:55) and the beta^2 partial (minres.hpp:63) with
+// unique-DOF weights. The last block runs the Givens recurrence.
 template <typename T>
-__global__ __launch_bounds__(64) void k_minres_p2(T* y, const T* r2, size_t len, int ncomp, int N, int K,
-                                                 const uint8_t* obits, const double* scp, double* partial)
+__global__ __launch_bounds__(64) void k_minres_upd(MinresUpdArgs<T> a)
 {
-    const KrylovScalars* sc = reinterpret_cast<const KrylovScalars*>(scp);
-    const int lane = threadIdx.x, KN = K * N;
+    __shared__ int am_last;
+    KrylovScalars* sc = reinterpret_cast<KrylovScalars*>(a.sc);
+    const int done = sc->done;
+    const int lane = threadIdx.x, N = a.N, K = a.K, KN = K * N;
     double part = 0;
-    if (!sc->done && lane < KN) {
+    if (!done && lane < KN) {
         const T c2 = (T)sc->c2;
         const int el = lane / N, j = lane % N;
-        const unsigned bits = obits[blockIdx.x * K + el];
+        const int e = blockIdx.x * K + el;
+        const unsigned bits = a.obits[e];
         const size_t base = (size_t)blockIdx.x * K * N * N + lane;
-        for (int c = 0; c < ncomp; ++c)
+        const size_t NN = (size_t)N * N;
+        // element-local (i', j') of side s, entry k
+        auto side_pos = [&](int e2, int s2, int k2) -> size_t {
+            int i2, j2;
+            if (s2 == 1) { i2 = k2; j2 = 0; }
+            else if (s2 == 2) { i2 = N - 1; j2 = k2; }
+            else if (s2 == 3) { i2 = k2; j2 = N - 1; }
+            else { i2 = 0; j2 = k2; }
+            return (size_t)(e2 / K) * K * NN + (size_t)i2 * KN + (size_t)(e2 % K) * N + j2;
+        };
+        for (int c = 0; c < a.ncomp; ++c) {
+            const T* yr = a.yraw + (size_t)c * a.len;
             for (int i = 0; i < N; ++i) {
-                const size_t k = (size_t)c * len + base + (size_t)i * KN;
-                const T v = y[k] - c2 * r2[k];
-                y[k] = v;
+                const size_t p = base + (size_t)i * KN;
+                const bool ri = (i == 0 || i == N - 1), rj = (j == 0 || j == N - 1);
+                T v;
+                if (!ri && !rj) {
+                    v = yr[p];
+                } else if (ri && rj) {
+                    const int kc = (i == 0) ? ((j == 0) ? 0 : 3) : ((j == 0) ? 1 : 2);
+                    const int ci = a.ecorner[4 * e + kc];
+                    if (ci < 0) {
+                        v = 0;
+                    } else {
+                        T s = 0;
+                        for (int q = a.c_off[ci]; q < a.c_off[ci + 1]; ++q) s += yr[a.c_pos[q]];
+                        v = s;
+                    }
+                } else {
+                    int sd, k;
+                    if (j == 0) { sd = 1; k = i; }
+                    else if (j == N - 1) { sd = 3; k = i; }
+                    else if (i == 0) { sd = 4; k = j; }
+                    else { sd = 2; k = j; }
+                    const int d = a.eside[4 * e + sd - 1];
+                    if (d < 0) {
+                        v = 0;
+                    } else {
+                        const int e2 = d >> 3, s2 = ((d >> 1) & 3) + 1;
+                        const int k2 = (d & 1) ? N - 1 - k : k;
+                        v = yr[p] + yr[side_pos(e2, s2, k2)];
+                    }
+                }
+                v = v - c2 * a.r2[(size_t)c * a.len + p];
+                a.y[(size_t)c * a.len + p] = v;
                 part += owner_weight(i, j, N, bits) * ((double)v * (double)v);
             }
+        }
     }
     part = wsum(part);
-    if (lane == 0) partial[blockIdx.x] = part;
+    if (threadIdx.x == 0) {
+        a.partial[blockIdx.x] = part;
+        __threadfence();
+        const unsigned prev = atomicAdd(a.counter, 1u);
+        am_last = (prev == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (am_last) {
+        __threadfence();
+        double s = 0;
+        for (int k = threadIdx.x; k < (int)gridDim.x; k += 64) s += a.partial[k];
+        s = wsum(s);
+        if (threadIdx.x == 0) {
+            if (!done) minres_beta_update(sc, s);
+            *a.counter = 0;
+        }
+    }
 }
 
 template <typename T>
-void launch_minres_p2(T* y, const T* r2, size_t len, int ncomp, const Layout& L, const uint8_t* obits,
-                      const double* sc, double* partial, hipStream_t s)
+void launch_minres_upd(const MinresUpdArgs<T>& a, int nchunks, hipStream_t s)
 {
-    hipLaunchKernelGGL((k_minres_p2<T>), dim3(L.nchunks), dim3(64), 0, s, y, r2, len, ncomp, L.N, L.K, obits, sc,
-                       partial);
+    hipLaunchKernelGGL((k_minres_upd<T>), dim3(nchunks), dim3(64), 0, s, a);
 }
-template void launch_minres_p2<double>(double*, const double*, size_t, int, const Layout&, const uint8_t*,
-                                       const double*, double*, hipStream_t);
-template void launch_minres_p2<float>(float*, const float*, size_t, int, const Layout&, const uint8_t*, const double*,
-                                      double*, hipStream_t);
+template void launch_minres_upd<double>(const MinresUpdArgs<double>&, int, hipStream_t);
+template void launch_minres_upd<float>(const MinresUpdArgs<float>&, int, hipStream_t);
 
 // final pending w/x update after the loop ends (minres.hpp:82-86)
 template <typename T>

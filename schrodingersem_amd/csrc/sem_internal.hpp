@@ -147,7 +147,7 @@ template <typename T>
 struct MinresOpArgs {
     const T* r2;  // R[i-1]: v_i = ib * r2
     const T* r1;  // R[i-2]
-    T* y;         // R[i]: raw element result on ring entries, final value inside
+    T* y;         // element results: final inside, ring copies unassembled (owner copy minus cr1*r1)
     const T* wa;  // W[i-3] (w1 for the pending update)
     const T* wb;  // W[i-2] (w2)
     T* wn;        // W[i-1]
@@ -157,8 +157,29 @@ struct MinresOpArgs {
     T alpha;
     size_t len;
     int nchunks;
-    double* partial;  // per-block alpha partial sums
-    const double* sc; // device scalars (KrylovScalars)
+    const uint8_t* obits;  // ownership bits per element
+    double* partial;       // per-block alpha partial sums
+    unsigned* counter;     // blocks finished (last block reduces)
+    double* sc;            // device scalars (KrylovScalars)
+};
+
+// Arguments of the MINRES update kernel: interface assembly by gather,
+// y -= c2*r2, beta^2 partial, Givens recurrence in the last block.
+template <typename T>
+struct MinresUpdArgs {
+    const T* yraw;
+    T* y;
+    const T* r2;
+    size_t len;
+    int ncomp, N, K;
+    const uint8_t* obits;
+    const int* eside;    // per element side: (neighbour << 3) | (neighbour side - 1) << 1 | reversed, or -1
+    const int* ecorner;  // per element corner: interior corner index, or -1 on the Dirichlet boundary
+    const int* c_off;
+    const int* c_pos;
+    double* partial;
+    unsigned* counter;
+    double* sc;
 };
 
 // Device-resident Krylov scalars (FP64), written by single-block finalize kernels.
@@ -174,6 +195,8 @@ struct KrylovScalars {
     int pad_;
     // generic reduction results
     double red[8];
+    // block counters of the last-block reductions
+    unsigned cnt[4];
 };
 
 // Arguments of the PCG-fused operator kernel (pcg.hpp:70-80).

@@ -11,10 +11,9 @@ namespace sem {
 //   DSS_SUM_ZB  global_sum then zero Dirichlet entries, sums scaled by post[c]
 //   DSS_UNMASK  unmask (unmask.hpp:43-55): owner value to every copy
 //   DSS_MASK    mask (mask.hpp:16-28): zero non-owner copies and Dirichlet entries
-//   DSS_MINRES  DSS_SUM_ZB fused with y -= cr1*r1 and the alpha partial
 //   DSS_PCG     DSS_SUM_ZB fused with the p.Ap partial
 //   DSS_ZERO_BND zero Dirichlet entries only
-enum DssMode { DSS_SUM = 0, DSS_SUM_ZB = 1, DSS_UNMASK = 2, DSS_MASK = 3, DSS_MINRES = 4, DSS_PCG = 5, DSS_ZERO_BND = 6 };
+enum DssMode { DSS_SUM = 0, DSS_SUM_ZB = 1, DSS_UNMASK = 2, DSS_MASK = 3, DSS_PCG = 5, DSS_ZERO_BND = 6 };
 
 template <typename T>
 struct DssArgs {
@@ -29,8 +28,7 @@ struct DssArgs {
     const int* bnd;
     int n_bnd;
     T post[2];
-    const T* r1;  // MINRES: R[i-2]
-    const T* r2;  // MINRES: R[i-1] (v = ib*r2); PCG: search direction p
+    const T* r2;  // PCG: search direction p
     const double* sc;
     double* partial;
 };
@@ -69,11 +67,8 @@ int launch_edot(const double* a, const double* b, size_t n, double* partial, hip
 void launch_fin_sum(const double* partial, int n, double* dst, hipStream_t s);
 
 void launch_minres_init(KrylovScalars* sc, int slot, int maxit, double tol, hipStream_t s);
-void launch_minres_fin_alpha(KrylovScalars* sc, const double* partial, int n, hipStream_t s);
-void launch_minres_fin_beta(KrylovScalars* sc, const double* partial, int n, hipStream_t s);
 template <typename T>
-void launch_minres_p2(T* y, const T* r2, size_t len, int ncomp, const Layout& L, const uint8_t* obits,
-                      const double* sc, double* partial, hipStream_t s);
+void launch_minres_upd(const MinresUpdArgs<T>& a, int nchunks, hipStream_t s);
 template <typename T>
 void launch_minres_flush(const T* r1, const T* wa, const T* wb, T* wn, T* x, size_t n, KrylovScalars* sc,
                          hipStream_t s);
