@@ -4,7 +4,9 @@
 Workload (one "step" = one 3-stage SDIRK4 time step = 3 MINRES stage solves):
 Schrodinger on a 128 x 128 quad mesh of [-10, 10]^2 at degree 8 (1.05 M nodes,
 complex FP64 state), potential 0.5|x|^2 + 16 Gaussian bumps (seed 2), Gaussian
-wavepacket of width 1 with momentum (2, 1), dt = 1e-3, MINRES tolerance 1e-10.
+wavepacket of width 1 with momentum (2, 1), dt = 1e-3, MINRES tolerance 1e-12
+(at 1e-10 even the plain FP64 solve lands 5e-9 away from a tol-1e-14 solve,
+so 1e-12 is the loosest tolerance that meets the 1e-9 relative-L2 bar).
 The reference integrates psi_t = +i(-Lap + V)psi; the equation
 i psi_t = (-Lap/2 + V) psi maps onto it exactly with V_ref = 2V, dt_ref = dt/2
 and psi conjugated on input and output.
@@ -13,8 +15,9 @@ Modes timed (same inputs, state resident in HBM before the timed region):
   fp64   reference arithmetic (FP64 fused MINRES, zero stage guesses)
   mixed  FP32 inner MINRES inside FP64 iterative refinement, extrapolated
          stage guesses, true FP64 residual as the stop test
-value = stage solves/s of the faster mode whose final psi agrees with the
-fp64 run to 1e-9 relative L2.
+A third, untimed run at tolerance 1e-14 (FP64) is the accuracy reference:
+value = stage solves/s of the fastest mode whose final psi agrees with it to
+1e-9 relative L2.
 
 Also measured: live HIP-event timing of the dominant kernel (the fused
 MINRES Lanczos-step kernel) over the timed region -> roofline; glaplace
@@ -86,7 +89,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=128)
     ap.add_argument("--p", type=int, default=8)
-    ap.add_argument("--tol", type=float, default=1e-10)
+    ap.add_argument("--tol", type=float, default=1e-12)
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-glaplace", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
@@ -104,20 +107,24 @@ def main():
     dt_ref = 1e-3 / 2.0
     log("mesh: E=%d N=%d dof=%d" % (m.n_elements, m.N, m.dof()))
 
+    log("accuracy reference: fp64, tol 1e-14")
+    truth = run_mode(sem, m, Vref, re, im, dt_ref, 1e-14, args.steps, args.warmup, sem.PREC_FP64,
+                     sem.GUESS_REFERENCE)["psi"]
     res = {}
     for name, prec, guess in (("fp64", sem.PREC_FP64, sem.GUESS_REFERENCE),
                               ("mixed", sem.PREC_MIXED, sem.GUESS_EXTRAPOLATED)):
         log("timing mode", name)
         r = run_mode(sem, m, Vref, re, im, dt_ref, args.tol, args.steps, args.warmup, prec, guess)
-        log("  %s: %.3f s for %d steps, %d stage solves, %d inner iterations, ok=%s" %
-            (name, r["seconds"], args.steps, r["stage_solves"], r["iters"], r["ok"]))
+        r["err"] = float(np.linalg.norm(r["psi"] - truth) / np.linalg.norm(truth))
+        log("  %s: %.3f s for %d steps, %d stage solves, %d inner iterations, rel-L2 vs tol-1e-14 run %.2e, ok=%s" %
+            (name, r["seconds"], args.steps, r["stage_solves"], r["iters"], r["err"], r["ok"]))
         res[name] = r
 
-    ref_psi = res["fp64"]["psi"]
-    mixed_err = float(np.linalg.norm(res["mixed"]["psi"] - ref_psi) / np.linalg.norm(ref_psi))
-    norm_drift = float(abs(np.linalg.norm(ref_psi) / np.linalg.norm(np.concatenate([re, im])) - 1.0))
+    mixed_err = res["mixed"]["err"]
+    norm_drift = float(abs(np.linalg.norm(truth) / np.linalg.norm(np.concatenate([re, im])) - 1.0))
     rate = {k: v["stage_solves"] / v["seconds"] for k, v in res.items()}
-    best = "mixed" if (mixed_err < 1e-9 and res["mixed"]["ok"] and rate["mixed"] > rate["fp64"]) else "fp64"
+    good = [k for k in res if res[k]["ok"] and res[k]["err"] < 1e-9]
+    best = max(good, key=lambda k: rate[k]) if good else "fp64"
     value = rate[best]
     ms_per_step = 1000.0 * res[best]["seconds"] / args.steps
 
@@ -186,8 +193,8 @@ def main():
                       "inner_iters_per_solve": round(res[k]["iters"] / max(res[k]["stage_solves"], 1), 1),
                       "refinements": res[k]["refinements"]} for k in res},
         "speedup_mixed_vs_fp64": round(rate["mixed"] / rate["fp64"], 3),
-        "mixed_rel_l2_vs_fp64": mixed_err,
-        "psi_norm_drift_fp64": norm_drift,
+        "rel_l2_vs_tol1e-14": {k: res[k]["err"] for k in res},
+        "psi_norm_drift": norm_drift,
         "roofline": roofline,
         "glaplace": glap,
         "cpu_baseline": cpu,

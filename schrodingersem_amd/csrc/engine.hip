@@ -114,7 +114,6 @@ struct sem_mesh {
     }
 
     size_t len() const { return L.len; }
-    unsigned* sc_counter(int k) { return &sc->cnt[k]; }
     template <typename T>
     const T* geo_t() const;
     template <typename T>
@@ -445,11 +444,11 @@ static Results minres_fused(sem_mesh* m, int kind, const T* cf, double alpha, co
             SEM_HIP(hipEventCreate(&e));
             m->ev.push_back(e);
         }
-        // algorithmic bytes of one launch: the operator apply (inputs, geometry, outputs)
-        // plus the fused w/x update (r1, w1, w2, x in; w, x out), per component
+        // algorithmic bytes of one launch: the operator apply (inputs, geometry,
+        // outputs; SURVEY-model fields) plus r1 read for y -= cr1*r1, per component
         const double EN2 = (double)m->L.E * m->L.N * m->L.N;
         const double op_vals = (kind == K_RES) ? 9 : (kind == K_HELMHOLTZ ? 7 : 5);
-        m->kt_bytes[ti] = (op_vals + 6.0 * nc) * sizeof(T) * EN2;
+        m->kt_bytes[ti] = (op_vals + 1.0 * nc) * sizeof(T) * EN2;
     }
     int i = 1;
     bool done = o.maxit < 1;
@@ -461,10 +460,6 @@ static Results minres_fused(sem_mesh* m, int kind, const T* cf, double alpha, co
             a.r2 = R[(i - 1) % 3];
             a.r1 = R[(i + 1) % 3];
             a.y = Yraw;
-            a.wa = W[i % 3];
-            a.wb = W[(i + 1) % 3];
-            a.wn = W[(i + 2) % 3];
-            a.x = x;
             a.geo = m->geo_t<T>();
             a.cf = cf;
             a.alpha = (T)alpha;
@@ -472,29 +467,33 @@ static Results minres_fused(sem_mesh* m, int kind, const T* cf, double alpha, co
             a.nchunks = nch;
             a.obits = m->obits;
             a.partial = m->partial;
-            a.counter = m->sc_counter(0);
-            a.sc = reinterpret_cast<double*>(m->sc);
+            a.sc = reinterpret_cast<const double*>(m->sc);
             if (m->timing) SEM_HIP(hipEventRecord(m->ev[2 * b_], s));
             launch_minres_op_d<T>(m->L.N, kind, a, m->dmat<T>(), s);
             if (m->timing) SEM_HIP(hipEventRecord(m->ev[2 * b_ + 1], s));
             ++launched;
+            launch_minres_fin_alpha(m->sc, m->partial, nch, s);
             MinresUpdArgs<T> u{};
             u.yraw = Yraw;
             u.y = R[i % 3];
             u.r2 = R[(i - 1) % 3];
+            u.r1 = R[(i + 1) % 3];
+            u.wa = W[i % 3];
+            u.wb = W[(i + 1) % 3];
+            u.wn = W[(i + 2) % 3];
+            u.x = x;
             u.len = m->len();
             u.ncomp = nc;
-            u.N = m->L.N;
-            u.K = m->L.K;
+            u.nchunks = nch;
             u.obits = m->obits;
             u.eside = m->eside;
             u.ecorner = m->ecorner;
             u.c_off = m->c_off;
             u.c_pos = m->c_pos;
             u.partial = m->partial + nch;
-            u.counter = m->sc_counter(1);
-            u.sc = reinterpret_cast<double*>(m->sc);
-            launch_minres_upd<T>(u, nch, s);
+            u.sc = reinterpret_cast<const double*>(m->sc);
+            const int nb2 = launch_minres_upd_n<T>(m->L.N, u, s);
+            launch_minres_fin_beta(m->sc, m->partial + nch, nb2, s);
         }
         m->read_scalars();
         if (m->timing) {

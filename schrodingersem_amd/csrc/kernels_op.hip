@@ -29,6 +29,18 @@ struct Dm {
 };
 
 __device__ __forceinline__ double fmad(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
+// LDS exchange among the lanes of one wavefront. Every wave works in its own
+// LDS region, so no workgroup barrier is needed: LDS instructions of a wave
+// execute in order; the fences keep the compiler from moving the reads above
+// the writes, and (unlike __syncthreads) they do not order global memory
+// traffic, so independent global loads stay in flight across the exchange.
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 __device__ __forceinline__ float fmad(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 
 // thread holds column l of X: col[i] = X(i,l); returns row[k] = X(l,k)
@@ -41,7 +53,7 @@ __device__ __forceinline__ void xpose_c2r(T* buf, int el, int l, bool act, const
 #pragma unroll
         for (int i = 0; i < N; ++i) b[l * P + i] = col[i];
     }
-    __syncthreads();
+    wave_lds_sync();
 #pragma unroll
     for (int k = 0; k < N; ++k) row[k] = act ? b[k * P + l] : T(0);
 }
@@ -56,7 +68,7 @@ __device__ __forceinline__ void xpose_r2c(T* buf, int el, int l, bool act, const
 #pragma unroll
         for (int j = 0; j < N; ++j) b[j * P + l] = row[j];
     }
-    __syncthreads();
+    wave_lds_sync();
 #pragma unroll
     for (int i = 0; i < N; ++i) col[i] = act ? b[l * P + i] : T(0);
 }
@@ -239,126 +251,198 @@ __device__ __forceinline__ bool owns(int i, int l, unsigned bits)
     return (bits >> bit) & 1u;
 }
 
-// MINRES Lanczos step i, fused with the w/x update of step i-1
-// (minres.hpp:48-86): v = ib*r2, y = A v - cr1*r1. The subtraction is applied
-// on element-interior entries and on the owner copy of every interface node;
-// the update kernel then sums the copies, which gives the assembled
-// (A v)_owner - cr1*r1 everywhere. alpha = dot(v, y) is therefore the sum of
-// v*y over interior entries plus v*(copy value) over every interface copy,
-// computed here; the last block to finish adds the partials in block order and
-// sets c2 = alpha / beta (minres.hpp:54-55).
-// Pending update: w_new = ig*((ibp*r1 - eo*wa) - dl*wb); x += phi*w_new.
+// MINRES Lanczos step i, operator part (minres.hpp:48-55): v = ib*r2,
+// y = A v - cr1*r1. One wavefront per output component (workgroup = NC
+// waves on the same chunk, so the second wave's geometry loads hit in cache).
+// The subtraction is applied on element-interior entries and on the owner
+// copy of every interface node; the update kernel sums the copies, which gives
+// the assembled (A v)_owner - cr1*r1 on every copy. alpha = dot(v, y) is
+// therefore the sum of v*y over interior entries plus v*(copy value) over
+// every interface copy, computed here as one partial per workgroup.
 template <int N, typename T, int KIND>
-__global__ __launch_bounds__(64) void k_minres_op(MinresOpArgs<T> a, Dm<N, T> D)
+__global__ __launch_bounds__(128) void k_minres_op(MinresOpArgs<T> a, Dm<N, T> D)
 {
     using S = Shape<N>;
     constexpr int NC = (KIND == K_RES) ? 2 : 1;
-    __shared__ T lds0[S::LDS], lds1[S::LDS];
-    __shared__ int am_last;
-    KrylovScalars* sc = reinterpret_cast<KrylovScalars*>(a.sc);
+    __shared__ T lds[NC][2][S::LDS];
+    __shared__ double wpart[NC];
+    const KrylovScalars* sc = reinterpret_cast<const KrylovScalars*>(a.sc);
     const int done = sc->done;
-    const int p3 = sc->p3_pending;
-    const int lane = threadIdx.x;
+    const int w = threadIdx.x >> 6;  // output component of this wave
+    const int lane = threadIdx.x & 63;
     const bool act = lane < S::KN;
     const int el = act ? lane / N : 0;
     const int l = act ? lane % N : 0;
     const Col<N, T> c{(size_t)blockIdx.x * S::K * N * N + lane, act};
     const size_t len = a.len;
-    if (p3) {
-        const T ibp = (T)sc->p3_ibp, ig = (T)sc->p3_ig, eo = (T)sc->p3_eo, dl = (T)sc->p3_dl, phi = (T)sc->p3_phi;
-#pragma unroll
-        for (int cc = 0; cc < NC; ++cc) {
-            T r[N], wa[N], wb[N], x[N];
-            c.ld(a.r1 + cc * len, r);
-            c.ld(a.wa + cc * len, wa);
-            c.ld(a.wb + cc * len, wb);
-            c.ld(a.x + cc * len, x);
-#pragma unroll
-            for (int i = 0; i < N; ++i) {
-                T v = ibp * r[i];
-                T wn = ig * ((v - eo * wa[i]) - dl * wb[i]);
-                wa[i] = wn;
-                x[i] += phi * wn;
-            }
-            c.st(a.wn + cc * len, wa);
-            c.st(a.x + cc * len, x);
-        }
-    }
     double part = 0;
     if (!done) {
         const T ib = (T)sc->ib, cr1 = (T)sc->cr1;
         const int has_r1 = sc->has_r1;
         const unsigned bits = act ? a.obits[blockIdx.x * S::K + el] : 0u;
-        T ga[N], gb[N], gc[N], cf[N];
+        T ga[N], gb[N], gc[N], qs[N], y[N];
+        // stiffness input: the other component for Res, the only one otherwise
+        c.ld(a.r2 + (NC == 2 ? (1 - w) : 0) * len, qs);
         c.ld(a.geo, ga);
         c.ld(a.geo + len, gb);
         c.ld(a.geo + 2 * len, gc);
-        if constexpr (KIND != K_LAPLACE) c.ld(a.cf, cf);
-        T q0[N];
-        c.ld(a.r2, q0);
 #pragma unroll
-        for (int i = 0; i < N; ++i) q0[i] = ib * q0[i];
-        auto finish = [&](int cc, T (&y)[N], const T (&q)[N]) {
-            T r[N];
-            if (has_r1) c.ld(a.r1 + cc * len, r);
+        for (int i = 0; i < N; ++i) qs[i] = ib * qs[i];
+        stiffness<N>(D, lds[w][0], lds[w][1], el, l, act, qs, ga, gb, gc, y);
+        T qm[N];  // v of the output component (alpha partial, mass term)
+        if constexpr (KIND == K_RES) {
+            T cf[N], gj[N];
+            c.ld(a.r2 + w * len, qm);
+            c.ld(a.cf, cf);
+            c.ld(a.geo + 3 * len, gj);
 #pragma unroll
             for (int i = 0; i < N; ++i) {
-                if (has_r1 && owns<N>(i, l, bits)) y[i] = y[i] - cr1 * r[i];
-                part += (double)q[i] * (double)y[i];
+                qm[i] = ib * qm[i];
+                const T t = a.alpha * (y[i] + cf[i] * qs[i]);
+                y[i] = (w == 0) ? gj[i] * qm[i] + t : t - gj[i] * qm[i];
             }
-            c.st(a.y + cc * len, y);
-        };
-        if constexpr (KIND == K_RES) {
-            T q1[N], ky[N], y[N], gj[N];
-            c.ld(a.r2 + len, q1);
+        } else if constexpr (KIND == K_HELMHOLTZ) {
+            T cf[N];
+            c.ld(a.cf, cf);
 #pragma unroll
-            for (int i = 0; i < N; ++i) q1[i] = ib * q1[i];
-            c.ld(a.geo + 3 * len, gj);
-            stiffness<N>(D, lds0, lds1, el, l, act, q1, ga, gb, gc, ky);
-#pragma unroll
-            for (int i = 0; i < N; ++i) y[i] = gj[i] * q0[i] + a.alpha * (ky[i] + cf[i] * q1[i]);
-            finish(0, y, q0);
-            stiffness<N>(D, lds0, lds1, el, l, act, q0, ga, gb, gc, ky);
-#pragma unroll
-            for (int i = 0; i < N; ++i) y[i] = a.alpha * (ky[i] + cf[i] * q0[i]) - gj[i] * q1[i];
-            finish(1, y, q1);
+            for (int i = 0; i < N; ++i) {
+                qm[i] = qs[i];
+                y[i] = cf[i] * qs[i] + y[i];
+            }
         } else {
-            T ky[N], y[N];
-            stiffness<N>(D, lds0, lds1, el, l, act, q0, ga, gb, gc, ky);
-            if constexpr (KIND == K_HELMHOLTZ) {
 #pragma unroll
-                for (int i = 0; i < N; ++i) y[i] = cf[i] * q0[i] + ky[i];
-            } else {
-#pragma unroll
-                for (int i = 0; i < N; ++i) y[i] = ky[i];
-            }
-            finish(0, y, q0);
+            for (int i = 0; i < N; ++i) qm[i] = qs[i];
         }
+        T r[N];
+        if (has_r1) c.ld(a.r1 + w * len, r);
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+            if (has_r1 && owns<N>(i, l, bits)) y[i] = y[i] - cr1 * r[i];
+            part += (double)qm[i] * (double)y[i];
+        }
+        c.st(a.y + w * len, y);
     }
-    // last-block reduction of the alpha partials
     part = wave_sum(part);
-    if (threadIdx.x == 0) {
-        a.partial[blockIdx.x] = part;
-        __threadfence();
-        const unsigned prev = atomicAdd(a.counter, 1u);
-        am_last = (prev == gridDim.x - 1);
-    }
+    if (lane == 0) wpart[w] = part;
     __syncthreads();
-    if (am_last) {
-        __threadfence();
-        double s = 0;
-        for (int k = threadIdx.x; k < (int)gridDim.x; k += 64) s += a.partial[k];
-        s = wave_sum(s);
-        if (threadIdx.x == 0) {
-            if (done) {
-                sc->p3_pending = 0;  // this launch applied the pending w/x update
-            } else {
-                sc->alpha = s;
-                sc->c2 = s / sc->beta;
+    if (threadIdx.x == 0) {
+        double s = wpart[0];
+        if constexpr (NC == 2) s += wpart[1];
+        a.partial[blockIdx.x] = s;
+    }
+}
+
+// Second kernel of a Lanczos step, four chunks per 256-thread workgroup,
+// lane = column. Every entry of y is finished here: interface copies gather
+// the unassembled values of all copies of their node (edge: owner + other,
+// global_sum.hpp:28-34; corner: left fold from +0.0 in connected_elements
+// order, global_sum.hpp:54-63), so every copy computes the same sum bit for
+// bit; Dirichlet entries become zero; then y -= c2*r2 (minres.hpp:55) and the
+// beta^2 partial with unique-DOF weights (minres.hpp:63). The pending w/x
+// update of the previous step runs here too (it reads r1 = R[i-2]).
+template <int N, typename T>
+__global__ __launch_bounds__(256) void k_minres_upd(MinresUpdArgs<T> a)
+{
+    using S = Shape<N>;
+    constexpr int K = S::K, KN = S::KN;
+    __shared__ double wpart[4];
+    const KrylovScalars* sc = reinterpret_cast<const KrylovScalars*>(a.sc);
+    const int done = sc->done, p3 = sc->p3_pending;
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int chunk = blockIdx.x * 4 + wv;
+    const bool act = lane < KN && chunk < a.nchunks;
+    double part = 0;
+    if (act) {
+        const int el = lane / N, j = lane % N;
+        const int e = chunk * K + el;
+        const size_t base = (size_t)chunk * K * N * N + lane;
+        const size_t len = a.len;
+        if (p3) {
+            const T ibp = (T)sc->p3_ibp, ig = (T)sc->p3_ig, eo = (T)sc->p3_eo, dl = (T)sc->p3_dl, phi = (T)sc->p3_phi;
+            for (int c = 0; c < a.ncomp; ++c) {
+                T r[N], wa[N], wb[N], x[N];
+                const size_t o = (size_t)c * len + base;
+#pragma unroll
+                for (int i = 0; i < N; ++i) {
+                    r[i] = a.r1[o + (size_t)i * KN];
+                    wa[i] = a.wa[o + (size_t)i * KN];
+                    wb[i] = a.wb[o + (size_t)i * KN];
+                    x[i] = a.x[o + (size_t)i * KN];
+                }
+#pragma unroll
+                for (int i = 0; i < N; ++i) {
+                    const T v = ibp * r[i];
+                    const T wn = ig * ((v - eo * wa[i]) - dl * wb[i]);
+                    a.wn[o + (size_t)i * KN] = wn;
+                    a.x[o + (size_t)i * KN] = x[i] + phi * wn;
+                }
             }
-            *a.counter = 0;
+        }
+        if (!done) {
+            const T c2 = (T)sc->c2;
+            const unsigned bits = a.obits[e];
+            // partner position of every interface entry of this column (or -1: Dirichlet)
+            auto side_pos = [&](int d, int k) -> long {
+                const int e2 = d >> 3, s2 = ((d >> 1) & 3) + 1;
+                const int k2 = (d & 1) ? N - 1 - k : k;
+                int i2, j2;
+                if (s2 == 1) { i2 = k2; j2 = 0; }
+                else if (s2 == 2) { i2 = N - 1; j2 = k2; }
+                else if (s2 == 3) { i2 = k2; j2 = N - 1; }
+                else { i2 = 0; j2 = k2; }
+                return (long)(e2 / K) * K * N * N + (long)i2 * KN + (long)(e2 % K) * N + j2;
+            };
+            const int dcol = (j == 0) ? a.eside[4 * e + 0] : ((j == N - 1) ? a.eside[4 * e + 2] : 0);
+            const int d4 = a.eside[4 * e + 3], d2 = a.eside[4 * e + 1];
+            const int c_lo = (j == 0) ? a.ecorner[4 * e + 0] : ((j == N - 1) ? a.ecorner[4 * e + 3] : -1);
+            const int c_hi = (j == 0) ? a.ecorner[4 * e + 1] : ((j == N - 1) ? a.ecorner[4 * e + 2] : -1);
+            for (int c = 0; c < a.ncomp; ++c) {
+                const T* yr = a.yraw + (size_t)c * len;
+                T v[N], r[N], pv[N];
+                const size_t o = base;
+#pragma unroll
+                for (int i = 0; i < N; ++i) {
+                    v[i] = yr[o + (size_t)i * KN];
+                    r[i] = a.r2[(size_t)c * len + o + (size_t)i * KN];
+                    pv[i] = 0;
+                }
+                const bool rj = (j == 0 || j == N - 1);
+                if (rj) {
+                    // column on side 1 (j = 0) or side 3 (j = N-1): planes 1..N-2
+                    if (dcol >= 0) {
+#pragma unroll
+                        for (int i = 1; i < N - 1; ++i) pv[i] = yr[side_pos(dcol, i)];
+                    }
+                    // corners at planes 0 and N-1
+                    T s0 = 0, s1 = 0;
+                    if (c_lo >= 0)
+                        for (int q = a.c_off[c_lo]; q < a.c_off[c_lo + 1]; ++q) s0 += yr[a.c_pos[q]];
+                    if (c_hi >= 0)
+                        for (int q = a.c_off[c_hi]; q < a.c_off[c_hi + 1]; ++q) s1 += yr[a.c_pos[q]];
+#pragma unroll
+                    for (int i = 1; i < N - 1; ++i) v[i] = (dcol >= 0) ? v[i] + pv[i] : T(0);
+                    v[0] = s0;
+                    v[N - 1] = s1;
+                } else {
+                    // plane 0 is side 4, plane N-1 is side 2
+                    const T p0 = (d4 >= 0) ? yr[side_pos(d4, j)] : T(0);
+                    const T p1 = (d2 >= 0) ? yr[side_pos(d2, j)] : T(0);
+                    v[0] = (d4 >= 0) ? v[0] + p0 : T(0);
+                    v[N - 1] = (d2 >= 0) ? v[N - 1] + p1 : T(0);
+                }
+#pragma unroll
+                for (int i = 0; i < N; ++i) {
+                    const T yv = v[i] - c2 * r[i];
+                    a.y[(size_t)c * len + o + (size_t)i * KN] = yv;
+                    if (owns<N>(i, j, bits)) part += (double)yv * (double)yv;
+                }
+            }
         }
     }
+    part = wave_sum(part);
+    if (lane == 0) wpart[wv] = part;
+    __syncthreads();
+    if (threadIdx.x == 0) a.partial[blockIdx.x] = ((wpart[0] + wpart[1]) + wpart[2]) + wpart[3];
 }
 
 // PCG step (pcg.hpp:70-80, identity preconditioner): p = r (first step) or
@@ -429,11 +513,11 @@ template <int N, typename T>
 static void minres_op_N(int kind, const MinresOpArgs<T>& a, const T* d, hipStream_t s)
 {
     auto D = make_dm<N, T>(d);
-    dim3 g(a.nchunks), b(64);
+    dim3 g(a.nchunks), b1(64), b2(128);
     switch (kind) {
-    case K_LAPLACE: hipLaunchKernelGGL((k_minres_op<N, T, K_LAPLACE>), g, b, 0, s, a, D); break;
-    case K_HELMHOLTZ: hipLaunchKernelGGL((k_minres_op<N, T, K_HELMHOLTZ>), g, b, 0, s, a, D); break;
-    case K_RES: hipLaunchKernelGGL((k_minres_op<N, T, K_RES>), g, b, 0, s, a, D); break;
+    case K_LAPLACE: hipLaunchKernelGGL((k_minres_op<N, T, K_LAPLACE>), g, b1, 0, s, a, D); break;
+    case K_HELMHOLTZ: hipLaunchKernelGGL((k_minres_op<N, T, K_HELMHOLTZ>), g, b1, 0, s, a, D); break;
+    case K_RES: hipLaunchKernelGGL((k_minres_op<N, T, K_RES>), g, b2, 0, s, a, D); break;
     default: throw std::runtime_error("minres: unsupported operator kind");
     }
 }
@@ -443,6 +527,14 @@ static void pcg_op_N(const PcgOpArgs<T>& a, int nchunks, const T* d, hipStream_t
 {
     auto D = make_dm<N, T>(d);
     hipLaunchKernelGGL((k_pcg_op<N, T>), dim3(nchunks), dim3(64), 0, s, a, D);
+}
+
+template <int N, typename T>
+static int minres_upd_N(const MinresUpdArgs<T>& a, hipStream_t s)
+{
+    const int nb = (a.nchunks + 3) / 4;
+    hipLaunchKernelGGL((k_minres_upd<N, T>), dim3(nb), dim3(256), 0, s, a);
+    return nb;
 }
 
 #define SEM_DISPATCH_N(N, CALL)                                   \
@@ -482,6 +574,16 @@ void launch_minres_op_d(int N, int kind, const MinresOpArgs<T>& a, const T* d, h
 }
 
 template <typename T>
+int launch_minres_upd_n(int N, const MinresUpdArgs<T>& a, hipStream_t s)
+{
+    int nb = 0;
+#define CALL_MU(n) nb = minres_upd_N<n, T>(a, s)
+    SEM_DISPATCH_N(N, CALL_MU)
+#undef CALL_MU
+    return nb;
+}
+
+template <typename T>
 void launch_pcg_op_d(int N, const PcgOpArgs<T>& a, int nchunks, const T* d, hipStream_t s)
 {
 #define CALL_PO(n) pcg_op_N<n, T>(a, nchunks, d, s)
@@ -494,10 +596,12 @@ void launch_pcg_op_d(int N, const PcgOpArgs<T>& a, int nchunks, const T* d, hipS
 #ifndef SEM_OP_FP32
 template void launch_element_op_d<double>(int, int, const OpArgs<double>&, const double*, hipStream_t);
 template void launch_minres_op_d<double>(int, int, const MinresOpArgs<double>&, const double*, hipStream_t);
+template int launch_minres_upd_n<double>(int, const MinresUpdArgs<double>&, hipStream_t);
 template void launch_pcg_op_d<double>(int, const PcgOpArgs<double>&, int, const double*, hipStream_t);
 #else
 template void launch_element_op_d<float>(int, int, const OpArgs<float>&, const float*, hipStream_t);
 template void launch_minres_op_d<float>(int, int, const MinresOpArgs<float>&, const float*, hipStream_t);
+template int launch_minres_upd_n<float>(int, const MinresUpdArgs<float>&, hipStream_t);
 #endif
 
 }  // namespace sem

@@ -366,103 +366,38 @@ __device__ void minres_beta_update(KrylovScalars* sc, double s)
     }
 }
 
-// The second kernel of a Lanczos step. Every entry of y is finished here:
-// interface copies gather the unassembled values of all copies of their node
-// (edge: owner + other, global_sum.hpp:28-34; corner: left fold from +0.0 in
-// connected_elements order, global_sum.hpp:54-63) -- each copy computes the
-// same sum, bit for bit -- Dirichlet entries become zero, then
-// y -= c2*r2 (minres.hpp:55) and the beta^2 partial (minres.hpp:63) with
-// unique-DOF weights. The last block runs the Givens recurrence.
-template <typename T>
-__global__ __launch_bounds__(64) void k_minres_upd(MinresUpdArgs<T> a)
+// alpha = dot(v, y) from the operator kernel's partials (minres.hpp:54),
+// c2 = alpha / beta (minres.hpp:55).
+__global__ __launch_bounds__(256) void k_minres_fin_alpha(KrylovScalars* sc, const double* partial, int n)
 {
-    __shared__ int am_last;
-    KrylovScalars* sc = reinterpret_cast<KrylovScalars*>(a.sc);
-    const int done = sc->done;
-    const int lane = threadIdx.x, N = a.N, K = a.K, KN = K * N;
-    double part = 0;
-    if (!done && lane < KN) {
-        const T c2 = (T)sc->c2;
-        const int el = lane / N, j = lane % N;
-        const int e = blockIdx.x * K + el;
-        const unsigned bits = a.obits[e];
-        const size_t base = (size_t)blockIdx.x * K * N * N + lane;
-        const size_t NN = (size_t)N * N;
-        // element-local (i', j') of side s, entry k
-        auto side_pos = [&](int e2, int s2, int k2) -> size_t {
-            int i2, j2;
-            if (s2 == 1) { i2 = k2; j2 = 0; }
-            else if (s2 == 2) { i2 = N - 1; j2 = k2; }
-            else if (s2 == 3) { i2 = k2; j2 = N - 1; }
-            else { i2 = 0; j2 = k2; }
-            return (size_t)(e2 / K) * K * NN + (size_t)i2 * KN + (size_t)(e2 % K) * N + j2;
-        };
-        for (int c = 0; c < a.ncomp; ++c) {
-            const T* yr = a.yraw + (size_t)c * a.len;
-            for (int i = 0; i < N; ++i) {
-                const size_t p = base + (size_t)i * KN;
-                const bool ri = (i == 0 || i == N - 1), rj = (j == 0 || j == N - 1);
-                T v;
-                if (!ri && !rj) {
-                    v = yr[p];
-                } else if (ri && rj) {
-                    const int kc = (i == 0) ? ((j == 0) ? 0 : 3) : ((j == 0) ? 1 : 2);
-                    const int ci = a.ecorner[4 * e + kc];
-                    if (ci < 0) {
-                        v = 0;
-                    } else {
-                        T s = 0;
-                        for (int q = a.c_off[ci]; q < a.c_off[ci + 1]; ++q) s += yr[a.c_pos[q]];
-                        v = s;
-                    }
-                } else {
-                    int sd, k;
-                    if (j == 0) { sd = 1; k = i; }
-                    else if (j == N - 1) { sd = 3; k = i; }
-                    else if (i == 0) { sd = 4; k = j; }
-                    else { sd = 2; k = j; }
-                    const int d = a.eside[4 * e + sd - 1];
-                    if (d < 0) {
-                        v = 0;
-                    } else {
-                        const int e2 = d >> 3, s2 = ((d >> 1) & 3) + 1;
-                        const int k2 = (d & 1) ? N - 1 - k : k;
-                        v = yr[p] + yr[side_pos(e2, s2, k2)];
-                    }
-                }
-                v = v - c2 * a.r2[(size_t)c * a.len + p];
-                a.y[(size_t)c * a.len + p] = v;
-                part += owner_weight(i, j, N, bits) * ((double)v * (double)v);
-            }
-        }
-    }
-    part = wsum(part);
+    if (sc->done) return;
+    const double s = sum_partials(partial, n);
     if (threadIdx.x == 0) {
-        a.partial[blockIdx.x] = part;
-        __threadfence();
-        const unsigned prev = atomicAdd(a.counter, 1u);
-        am_last = (prev == gridDim.x - 1);
-    }
-    __syncthreads();
-    if (am_last) {
-        __threadfence();
-        double s = 0;
-        for (int k = threadIdx.x; k < (int)gridDim.x; k += 64) s += a.partial[k];
-        s = wsum(s);
-        if (threadIdx.x == 0) {
-            if (!done) minres_beta_update(sc, s);
-            *a.counter = 0;
-        }
+        sc->alpha = s;
+        sc->c2 = s / sc->beta;
     }
 }
 
-template <typename T>
-void launch_minres_upd(const MinresUpdArgs<T>& a, int nchunks, hipStream_t s)
+void launch_minres_fin_alpha(KrylovScalars* sc, const double* partial, int n, hipStream_t s)
 {
-    hipLaunchKernelGGL((k_minres_upd<T>), dim3(nchunks), dim3(64), 0, s, a);
+    hipLaunchKernelGGL(k_minres_fin_alpha, dim3(1), dim3(256), 0, s, sc, partial, n);
 }
-template void launch_minres_upd<double>(const MinresUpdArgs<double>&, int, hipStream_t);
-template void launch_minres_upd<float>(const MinresUpdArgs<float>&, int, hipStream_t);
+
+__global__ __launch_bounds__(256) void k_minres_fin_beta(KrylovScalars* sc, const double* partial, int n)
+{
+    if (sc->done) {
+        // the update kernel of this step applied the pending w/x update
+        if (threadIdx.x == 0) sc->p3_pending = 0;
+        return;
+    }
+    const double s = sum_partials(partial, n);
+    if (threadIdx.x == 0) minres_beta_update(sc, s);
+}
+
+void launch_minres_fin_beta(KrylovScalars* sc, const double* partial, int n, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_minres_fin_beta, dim3(1), dim3(256), 0, s, sc, partial, n);
+}
 
 // final pending w/x update after the loop ends (minres.hpp:82-86)
 template <typename T>

@@ -141,45 +141,44 @@ struct OpArgs {
     int nchunks;
 };
 
-// Arguments of the MINRES-fused operator kernel (one Lanczos step + the
-// previous step's w/x update).
+// Arguments of the MINRES operator kernel (one Lanczos step, minres.hpp:48-55).
 template <typename T>
 struct MinresOpArgs {
     const T* r2;  // R[i-1]: v_i = ib * r2
     const T* r1;  // R[i-2]
     T* y;         // element results: final inside, ring copies unassembled (owner copy minus cr1*r1)
-    const T* wa;  // W[i-3] (w1 for the pending update)
-    const T* wb;  // W[i-2] (w2)
-    T* wn;        // W[i-1]
-    T* x;
     const T* geo;
     const T* cf;
     T alpha;
     size_t len;
     int nchunks;
     const uint8_t* obits;  // ownership bits per element
-    double* partial;       // per-block alpha partial sums
-    unsigned* counter;     // blocks finished (last block reduces)
-    double* sc;            // device scalars (KrylovScalars)
+    double* partial;       // per-workgroup alpha partial sums
+    const double* sc;      // device scalars (KrylovScalars)
 };
 
 // Arguments of the MINRES update kernel: interface assembly by gather,
-// y -= c2*r2, beta^2 partial, Givens recurrence in the last block.
+// y -= c2*r2, beta^2 partial, and the pending w/x update of the previous step
+// (w_new = ig*((ibp*r1 - eo*wa) - dl*wb); x += phi*w_new, minres.hpp:82-86).
 template <typename T>
 struct MinresUpdArgs {
     const T* yraw;
-    T* y;
-    const T* r2;
+    T* y;         // R[i]
+    const T* r2;  // R[i-1]
+    const T* r1;  // R[i-2]
+    const T* wa;  // W[i-3]
+    const T* wb;  // W[i-2]
+    T* wn;        // W[i-1]
+    T* x;
     size_t len;
-    int ncomp, N, K;
+    int ncomp, nchunks;
     const uint8_t* obits;
     const int* eside;    // per element side: (neighbour << 3) | (neighbour side - 1) << 1 | reversed, or -1
     const int* ecorner;  // per element corner: interior corner index, or -1 on the Dirichlet boundary
     const int* c_off;
     const int* c_pos;
     double* partial;
-    unsigned* counter;
-    double* sc;
+    const double* sc;
 };
 
 // Device-resident Krylov scalars (FP64), written by single-block finalize kernels.
@@ -195,8 +194,6 @@ struct KrylovScalars {
     int pad_;
     // generic reduction results
     double red[8];
-    // block counters of the last-block reductions
-    unsigned cnt[4];
 };
 
 // Arguments of the PCG-fused operator kernel (pcg.hpp:70-80).
@@ -219,5 +216,8 @@ template <typename T>
 void launch_minres_op_d(int N, int kind, const MinresOpArgs<T>& a, const T* d, hipStream_t s);
 template <typename T>
 void launch_pcg_op_d(int N, const PcgOpArgs<T>& a, int nchunks, const T* d, hipStream_t s);
+// update kernel of a Lanczos step; returns the number of partials written
+template <typename T>
+int launch_minres_upd_n(int N, const MinresUpdArgs<T>& a, hipStream_t s);
 
 }  // namespace sem
