@@ -62,8 +62,9 @@ typedef struct sem_solver_opts {
     double tol;
     double dt;
     int verbose;
-    int precision; /* enum sem_precision; 0 = reference arithmetic */
-    int guess;     /* 0 = reference guesses (zero / previous psi); 1 = extrapolated; 2 = learned */
+    int precision;   /* enum sem_precision; 0 = reference arithmetic */
+    int guess;       /* 0 = reference guesses (zero / previous psi); 1 = extrapolated; 2 = learned */
+    int learn_steps; /* guess 2: time steps during which the network trains (0 = 50), then frozen */
 } sem_solver_opts;
 
 /* Per-step observer (schrodinger.hpp:116,273,411): receives the unmasked
@@ -78,6 +79,8 @@ typedef struct sem_run_stats {
     long refinements;        /* FP64 refinement sweeps (mixed precision) */
     double last_error_estimate; /* DIRK4S5 embedded estimate (schrodinger.hpp:262-263) */
     double max_true_rel_residual; /* max over stage solves of ||b - A x|| / ||b||, FP64 */
+    long mlp_train_steps;    /* Adam steps taken by the guess network */
+    double mlp_last_loss;    /* last training loss (mean squared error, scaled targets) */
 } sem_run_stats;
 
 typedef struct sem_mesh sem_mesh;
@@ -151,6 +154,8 @@ int sem_stepper_create(sem_mesh* m, int scheme /* 0 bdf2, 1 dirk4s3, 2 dirk4s5 *
 int sem_stepper_advance(sem_stepper* s, double T, long max_steps, sem_step_cb cb, void* user, int* success);
 int sem_stepper_read(sem_stepper* s, double* re, double* im, double* t);
 int sem_stepper_stats(const sem_stepper* s, sem_run_stats* out);
+/* learned guess: Adam steps per stage solve while training (default 2) and learning rate (default 1e-3) */
+int sem_stepper_set_learning(sem_stepper* s, int epochs, float lr);
 void sem_stepper_destroy(sem_stepper* s);
 /* One Schrodinger stage solve Res(alpha, V) z = b (schrodinger.hpp:223 etc.), z in/out. */
 int sem_stage_solve(sem_mesh* m, const double* V, double alpha, const double* b, double* z, int max_iter, double tol,
@@ -179,6 +184,23 @@ int sem_set_kernel_timing(sem_mesh* m, int on);
  * and the calibration kernel for HBM byte counters. */
 int sem_time_copy(long bytes, int reps, double* us_per_launch);
 int sem_kernel_timing(const sem_mesh* m, int which, long* launches, double* total_us, double* bytes_per_launch);
+
+/* ----- per-element MLP of the learned stage guess (no counterpart in the
+ * reference: its stage guesses are zero, openmp_impl/schrodinger.hpp:222 etc.).
+ * Exposed for testing the forward, backward and Adam kernels. Row-major host
+ * arrays: X [rows x in_pad], T and Y [rows x out_pad]; parameters in one block
+ * W1 [in_pad x h1], b1, W2 [h1 x h2], b2, W3 [h2 x out_pad], b3. ----- */
+typedef struct sem_mlp sem_mlp;
+int sem_mlp_create(int in, int h1, int h2, int out, unsigned long long seed, sem_mlp** out_handle);
+void sem_mlp_destroy(sem_mlp* m);
+/* out[0..8] = in_pad, out_pad, nparams, offsets of W1, b1, W2, b2, W3, b3 */
+int sem_mlp_info(const sem_mlp* m, long* out);
+int sem_mlp_set_params(sem_mlp* m, const float* params);
+int sem_mlp_get_params(sem_mlp* m, float* params, int gradients);
+int sem_mlp_forward(sem_mlp* m, int rows, const float* X, float* Y);
+/* forward + backward of the mean squared error; gradients stay on the device */
+int sem_mlp_gradients(sem_mlp* m, int rows, const float* X, const float* T, double* loss);
+int sem_mlp_adam(sem_mlp* m, float lr);
 
 #ifdef __cplusplus
 }

@@ -215,7 +215,7 @@ bool stepper(int scheme, std::vector<M>& psi_real, std::vector<M>& psi_imag, con
 {
     std::vector<double> re = pack(psi_real, mesh.n2()), im = pack(psi_imag, mesh.n2()),
                         V = pack(potential, mesh.n2());
-    sem_solver_opts o{opts.max_iter, opts.tol, opts.dt, opts.verbose ? 1 : 0, SEM_PREC_FP64, 0};
+    sem_solver_opts o{opts.max_iter, opts.tol, opts.dt, opts.verbose ? 1 : 0, SEM_PREC_FP64, 0, 0};
     int ok = 0;
     CbCtx<M, Callback> ctx{&callback, mesh.n2(), psi_real.size(), &psi_real, &psi_imag};
     sem_step_cb cb = std::is_same_v<Callback, NullCallback> ? nullptr : &trampoline<M, Callback>;

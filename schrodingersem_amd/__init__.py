@@ -25,11 +25,12 @@ LIB_PATH = os.path.join(_HERE, "libsem_hip.so")
 
 OP_LAPLACE, OP_HELMHOLTZ, OP_MASS, OP_SCHRO_RES, OP_SCHRO_F, OP_SCHRO_L = range(6)
 PREC_FP64, PREC_MIXED = 0, 1
-GUESS_REFERENCE, GUESS_EXTRAPOLATED = 0, 1
+GUESS_REFERENCE, GUESS_EXTRAPOLATED, GUESS_LEARNED = 0, 1, 2
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
 _lp = C.POINTER(C.c_long)
+_fp = C.POINTER(C.c_float)
 
 
 class SemError(RuntimeError):
@@ -42,13 +43,14 @@ class _Results(C.Structure):
 
 class _Opts(C.Structure):
     _fields_ = [("max_iter", C.c_int), ("tol", C.c_double), ("dt", C.c_double), ("verbose", C.c_int),
-                ("precision", C.c_int), ("guess", C.c_int)]
+                ("precision", C.c_int), ("guess", C.c_int), ("learn_steps", C.c_int)]
 
 
 class _Stats(C.Structure):
     _fields_ = [("steps", C.c_int), ("stage_solves", C.c_int), ("total_iters", C.c_long),
                 ("refinements", C.c_long), ("last_error_estimate", C.c_double),
-                ("max_true_rel_residual", C.c_double)]
+                ("max_true_rel_residual", C.c_double), ("mlp_train_steps", C.c_long),
+                ("mlp_last_loss", C.c_double)]
 
 
 STEP_CB = C.CFUNCTYPE(C.c_int, _dp, _dp, C.c_void_p)
@@ -110,6 +112,15 @@ def lib():
             "sem_stepper_read": (C.c_int, [C.c_void_p, _dp, _dp, _dp]),
             "sem_stepper_stats": (C.c_int, [C.c_void_p, C.POINTER(_Stats)]),
             "sem_stepper_destroy": (None, [C.c_void_p]),
+            "sem_stepper_set_learning": (C.c_int, [C.c_void_p, C.c_int, C.c_float]),
+            "sem_mlp_create": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_ulonglong, C.POINTER(C.c_void_p)]),
+            "sem_mlp_destroy": (None, [C.c_void_p]),
+            "sem_mlp_info": (C.c_int, [C.c_void_p, _lp]),
+            "sem_mlp_set_params": (C.c_int, [C.c_void_p, _fp]),
+            "sem_mlp_get_params": (C.c_int, [C.c_void_p, _fp, C.c_int]),
+            "sem_mlp_forward": (C.c_int, [C.c_void_p, C.c_int, _fp, _fp]),
+            "sem_mlp_gradients": (C.c_int, [C.c_void_p, C.c_int, _fp, _fp, _dp]),
+            "sem_mlp_adam": (C.c_int, [C.c_void_p, C.c_float]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -151,13 +162,13 @@ class SolverOpts:
     """solver_opts (ode_base.hpp:6-13) plus the build-only precision / guess knobs."""
 
     def __init__(self, max_iter=1000, tol=1e-8, dt=1e-3, verbose=False, precision=PREC_FP64,
-                 guess=GUESS_REFERENCE):
+                 guess=GUESS_REFERENCE, learn_steps=0):
         self.max_iter, self.tol, self.dt, self.verbose = max_iter, tol, dt, verbose
-        self.precision, self.guess = precision, guess
+        self.precision, self.guess, self.learn_steps = precision, guess, learn_steps
 
     def _c(self):
         return _Opts(int(self.max_iter), float(self.tol), float(self.dt), int(bool(self.verbose)),
-                     int(self.precision), int(self.guess))
+                     int(self.precision), int(self.guess), int(self.learn_steps))
 
 
 def structured_mesh(n, box=(0.0, 1.0, 0.0, 1.0), jitter=0.0, seed=0):
@@ -400,6 +411,10 @@ class Stepper:
         _check(lib().sem_stepper_read(self._h, _ptr(re), _ptr(im), C.byref(t)))
         return re, im, t.value
 
+    def set_learning(self, epochs, lr):
+        """Learned guess: Adam steps per stage solve during training, and the learning rate."""
+        _check(lib().sem_stepper_set_learning(self._h, int(epochs), float(lr)))
+
     def stats(self):
         s = _Stats()
         _check(lib().sem_stepper_stats(self._h, C.byref(s)))
@@ -467,6 +482,54 @@ def time_element_op(mesh, kind, precision=PREC_FP64, reps=50):
     by = C.c_double()
     _check(lib().sem_time_element_op(mesh._h, int(kind), int(precision), int(reps), C.byref(us), C.byref(by)))
     return us.value, by.value
+
+
+class MlpNet:
+    """The guess network's kernels (forward, MSE backward, Adam) on host arrays."""
+
+    def __init__(self, n_in, h1, h2, n_out, seed=1):
+        h = C.c_void_p()
+        _check(lib().sem_mlp_create(int(n_in), int(h1), int(h2), int(n_out), int(seed), C.byref(h)))
+        self._h = h
+        info = (C.c_long * 9)()
+        _check(lib().sem_mlp_info(h, info))
+        self.in_pad, self.out_pad, self.nparams = int(info[0]), int(info[1]), int(info[2])
+        self.offsets = [int(v) for v in info[3:9]]
+        self.dims = (n_in, h1, h2, n_out)
+
+    def params(self, gradients=False):
+        p = np.zeros(self.nparams, dtype=np.float32)
+        _check(lib().sem_mlp_get_params(self._h, p.ctypes.data_as(_fp), int(gradients)))
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        _check(lib().sem_mlp_set_params(self._h, p.ctypes.data_as(_fp)))
+
+    def forward(self, X):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        Y = np.zeros((X.shape[0], self.out_pad), dtype=np.float32)
+        _check(lib().sem_mlp_forward(self._h, X.shape[0], X.ctypes.data_as(_fp), Y.ctypes.data_as(_fp)))
+        return Y
+
+    def gradients(self, X, T):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        T = np.ascontiguousarray(T, dtype=np.float32)
+        loss = C.c_double()
+        _check(lib().sem_mlp_gradients(self._h, X.shape[0], X.ctypes.data_as(_fp), T.ctypes.data_as(_fp),
+                                       C.byref(loss)))
+        return loss.value, self.params(gradients=True)
+
+    def adam(self, lr):
+        _check(lib().sem_mlp_adam(self._h, float(lr)))
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value and _lib is not None:
+            _lib.sem_mlp_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
 
 
 def device_info():

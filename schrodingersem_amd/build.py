@@ -32,6 +32,7 @@ UNITS = [
     ("kernels_vec.hip", "kernels_vec.o", []),
     ("engine.hip", "engine.o", []),
     ("mesh_host.cpp", "mesh_host.o", ["-x", "hip"]),
+    ("mlp.hip", "mlp.o", []),
 ]
 
 

@@ -12,6 +12,7 @@
 
 #include "sem_internal.hpp"
 #include "sem_kernels.hpp"
+#include "sem_mlp.hpp"
 
 namespace sem {
 
@@ -692,6 +693,16 @@ struct Stepper {
     int nps = 0;
     double alpha_bdf = 0;
     sem_run_stats stats{};
+    // learned guess (guess = 2)
+    std::unique_ptr<Mlp> mlp;
+    double* Vraw = nullptr;
+    double* ext = nullptr;
+    double* tmp = nullptr;
+    int learn_steps = 50;
+    int epochs = 2;
+    double s_feat[5][3] = {};  // per stage: scales of z, V, b features
+    double s_out[5] = {};      // per stage: scale of the correction
+    bool have_scale[5] = {}, have_out[5] = {};
 
     double* alloc()
     {
@@ -733,22 +744,110 @@ static Results solve_res(Stepper& st, double alpha, const double* b, double* x, 
     return r;
 }
 
+// root mean square of a device vector (all entries of ncomp components)
+static double rms_dev(sem_mesh* m, const double* a, int ncomp)
+{
+    const double s = wdot_host<double>(m, a, a, ncomp);
+    return std::sqrt(s / std::max(1.0, (double)ncomp * m->hm.dof));
+}
+
+static MlpScatterArgs scatter_args(Stepper& st, double* out, const double* base, double scale)
+{
+    sem_mesh* m = st.m;
+    MlpScatterArgs a{};
+    a.Y = st.mlp->Y;
+    a.ld = st.mlp->out_pad;
+    a.out = out;
+    a.base = base;
+    a.scale = scale;
+    a.len = m->len();
+    a.E = m->L.E;
+    a.N = m->L.N;
+    a.K = m->L.K;
+    a.ncomp = 2;
+    return a;
+}
+
+static void mlp_features(Stepper& st, int stage, const double* b)
+{
+    sem_mesh* m = st.m;
+    const size_t len = m->len();
+    const double* zp = st.zprev[stage];
+    const double* zq = st.zprev2[stage];
+    MlpFeatureArgs f{};
+    const double* src[7] = {zp, zp + len, zq, zq + len, st.Vraw, b, b + len};
+    const double sc[7] = {st.s_feat[stage][0], st.s_feat[stage][0], st.s_feat[stage][0], st.s_feat[stage][0],
+                          st.s_feat[stage][1], st.s_feat[stage][2], st.s_feat[stage][2]};
+    for (int g = 0; g < 7; ++g) {
+        f.src[g] = src[g];
+        f.scale[g] = sc[g];
+    }
+    f.ngroups = 7;
+    f.X = st.mlp->X;
+    f.ld = st.mlp->in_pad;
+    f.E = m->L.E;
+    f.N = m->L.N;
+    f.K = m->L.K;
+    launch_gather_features(f, m->stream);
+}
+
 static Results stage_solve(Stepper& st, double alpha, const double* b, double* z, int stage)
 {
+    sem_mesh* m = st.m;
     bool zero = true;
-    if (st.opts.guess == 1 && st.have_prev) {  // extrapolation from the previous steps' same stage
+    const bool learned = st.opts.guess == 2 && st.have_prev2;
+    if (learned) {
+        // extrapolation plus the network's correction, made consistent
+        lincomb<double>(m, st.ext, 1, {{2.0, st.zprev[stage]}, {-1.0, st.zprev2[stage]}}, st.n);
+        if (!st.have_scale[stage]) {
+            st.s_feat[stage][0] = 1.0 / std::max(rms_dev(m, st.zprev[stage], 2), 1e-300);
+            st.s_feat[stage][1] = 1.0 / std::max(rms_dev(m, st.Vraw, 1), 1e-300);
+            st.s_feat[stage][2] = 1.0 / std::max(rms_dev(m, b, 2), 1e-300);
+            st.have_scale[stage] = true;
+        }
+        if (st.have_out[stage]) {
+            st.mlp->reserve(m->L.E);
+            mlp_features(st, stage, b);
+            st.mlp->forward(m->L.E);
+            launch_scatter_output(scatter_args(st, z, st.ext, st.s_out[stage]), m->stream);
+            dss<double>(m, DSS_UNMASK, z, 2);
+            dss<double>(m, DSS_ZERO_BND, z, 2);
+        } else {
+            launch_cast<double, double>(z, st.ext, st.n, m->stream);
+        }
+        zero = false;
+    } else if (st.opts.guess >= 1 && st.have_prev) {  // extrapolation from the previous steps' same stage
         if (st.have_prev2)
-            lincomb<double>(st.m, z, 1, {{2.0, st.zprev[stage]}, {-1.0, st.zprev2[stage]}}, st.n);
+            lincomb<double>(m, z, 1, {{2.0, st.zprev[stage]}, {-1.0, st.zprev2[stage]}}, st.n);
         else
-            launch_cast<double, double>(z, st.zprev[stage], st.n, st.m->stream);
+            launch_cast<double, double>(z, st.zprev[stage], st.n, m->stream);
         zero = false;
     }
-    return solve_res(st, alpha, b, z, zero, !zero);
+    Results r = solve_res(st, alpha, b, z, zero, !zero);
+    if (learned && r.success && st.stats.steps < st.learn_steps) {
+        // train on this stage's solution: target = (z - extrapolation) / s_out
+        if (!st.have_out[stage]) {
+            lincomb<double>(m, st.tmp, 0, {{1.0, z}, {-1.0, st.ext}}, st.n);
+            st.s_out[stage] = std::max(rms_dev(m, st.tmp, 2), 1e-300);
+            st.have_out[stage] = true;
+        }
+        st.mlp->reserve(m->L.E);
+        mlp_features(st, stage, b);
+        launch_gather_target(scatter_args(st, nullptr, st.ext, 1.0 / st.s_out[stage]), st.mlp->T, z, m->stream);
+        for (int ep = 0; ep < st.epochs; ++ep) {
+            const bool last = ep + 1 == st.epochs;
+            const double loss = st.mlp->backward(m->L.E, last);
+            st.mlp->adam_step();
+            st.stats.mlp_train_steps++;
+            if (last) st.stats.mlp_last_loss = loss;
+        }
+    }
+    return r;
 }
 
 static void record_history(Stepper& st, int nstages)
 {
-    if (st.opts.guess != 1) return;
+    if (st.opts.guess < 1) return;
     for (int k = 0; k < nstages; ++k) {
         if (st.have_prev) std::swap(st.zprev2[k], st.zprev[k]);
         launch_cast<double, double>(st.zprev[k], st.z[k], st.n, st.m->stream);
@@ -792,7 +891,8 @@ static Stepper* stepper_create(sem_mesh* m, int scheme, const double* re, const 
                                const sem_solver_opts* o)
 {
     if (!(o->dt > 0)) throw std::invalid_argument("schrodinger: dt must be positive");
-    if (o->guess < 0 || o->guess > 1) throw std::invalid_argument("schrodinger: unsupported guess mode");
+    if (o->guess < 0 || o->guess > 2) throw std::invalid_argument("schrodinger: unsupported guess mode");
+    if (o->guess == 2 && scheme == 0) throw std::invalid_argument("schrodinger: the learned guess serves the SDIRK stages");
     if (o->precision < 0 || o->precision > 1) throw std::invalid_argument("schrodinger: unsupported precision");
     std::unique_ptr<Stepper> st(new Stepper);
     st->m = m;
@@ -815,11 +915,20 @@ static Stepper* stepper_create(sem_mesh* m, int scheme, const double* re, const 
     st->q = st->alloc();
     const int nz = scheme == 0 ? 0 : (scheme == 1 ? 3 : 5);
     for (int k = 0; k < nz; ++k) st->z.push_back(st->alloc());
-    if (o->guess == 1)
+    if (o->guess >= 1)
         for (int k = 0; k < nz; ++k) {
             st->zprev.push_back(st->alloc());
             st->zprev2.push_back(st->alloc());
         }
+    if (o->guess == 2) {
+        const int N = m->L.N, NN = N * N;
+        st->mlp.reset(new Mlp(7 * NN, 256, 256, 2 * NN, 0x5EED0001ull, m->stream));
+        st->learn_steps = o->learn_steps > 0 ? o->learn_steps : 50;
+        st->ext = st->alloc();
+        st->tmp = st->alloc();
+        st->Vraw = st->alloc();
+        m->upload(st->Vraw, V, 1);
+    }
     if (scheme == 0) {
         st->ps[0] = st->alloc();
         st->ps[1] = st->alloc();
@@ -1435,6 +1544,18 @@ int sem_stepper_read(sem_stepper* h, double* re, double* im, double* t)
     });
 }
 
+int sem_stepper_set_learning(sem_stepper* h, int epochs, float lr)
+{
+    if (!h || epochs < 1 || !(lr > 0)) return SEM_EINVAL;
+    if (!h->st->mlp) {
+        set_error("sem_stepper_set_learning: the stepper was not created with the learned guess");
+        return SEM_EINVAL;
+    }
+    h->st->epochs = epochs;
+    h->st->mlp->lr = lr;
+    return SEM_OK;
+}
+
 int sem_stepper_stats(const sem_stepper* h, sem_run_stats* out)
 {
     if (!h || !out) return SEM_EINVAL;
@@ -1638,6 +1759,100 @@ int sem_time_element_op(sem_mesh* m, int kind, int precision, int reps, double* 
         hipEventDestroy(e0);
         hipEventDestroy(e1);
         if (us_per_launch) *us_per_launch = 1000.0 * ms / reps;
+        return (int)SEM_OK;
+    });
+}
+
+
+struct sem_mlp {
+    hipStream_t stream = nullptr;
+    std::unique_ptr<Mlp> net;
+    ~sem_mlp()
+    {
+        net.reset();
+        if (stream) hipStreamDestroy(stream);
+    }
+};
+
+int sem_mlp_create(int in, int h1, int h2, int out, unsigned long long seed, sem_mlp** out_handle)
+{
+    return guarded([&] {
+        if (!out_handle || in < 1 || h1 < 1 || h2 < 1 || out < 1) throw std::invalid_argument("sem_mlp_create: bad argument");
+        *out_handle = nullptr;
+        if (!check_gpu()) return (int)SEM_EHIP;
+        std::unique_ptr<sem_mlp> h(new sem_mlp);
+        SEM_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+        h->net.reset(new Mlp(in, h1, h2, out, seed, h->stream));
+        *out_handle = h.release();
+        return (int)SEM_OK;
+    });
+}
+
+void sem_mlp_destroy(sem_mlp* m) { delete m; }
+
+int sem_mlp_info(const sem_mlp* m, long* out)
+{
+    if (!m || !out) return SEM_EINVAL;
+    out[0] = m->net->in_pad;
+    out[1] = m->net->out_pad;
+    out[2] = (long)m->net->nparams;
+    for (int k = 0; k < 6; ++k) out[3 + k] = (long)m->net->off[k];
+    return SEM_OK;
+}
+
+int sem_mlp_set_params(sem_mlp* m, const float* params)
+{
+    return guarded([&] {
+        if (!m || !params) throw std::invalid_argument("null argument");
+        m->net->set_params(params);
+        return (int)SEM_OK;
+    });
+}
+
+int sem_mlp_get_params(sem_mlp* m, float* params, int gradients)
+{
+    return guarded([&] {
+        if (!m || !params) throw std::invalid_argument("null argument");
+        m->net->get_params(params, gradients != 0);
+        return (int)SEM_OK;
+    });
+}
+
+int sem_mlp_forward(sem_mlp* m, int rows, const float* X, float* Y)
+{
+    return guarded([&] {
+        if (!m || !X || !Y || rows < 1) throw std::invalid_argument("sem_mlp_forward: bad argument");
+        Mlp& n = *m->net;
+        n.reserve(rows);
+        SEM_HIP(hipMemcpyAsync(n.X, X, (size_t)rows * n.in_pad * sizeof(float), hipMemcpyHostToDevice, m->stream));
+        n.forward(rows);
+        SEM_HIP(hipMemcpyAsync(Y, n.Y, (size_t)rows * n.out_pad * sizeof(float), hipMemcpyDeviceToHost, m->stream));
+        SEM_HIP(hipStreamSynchronize(m->stream));
+        return (int)SEM_OK;
+    });
+}
+
+int sem_mlp_gradients(sem_mlp* m, int rows, const float* X, const float* T, double* loss)
+{
+    return guarded([&] {
+        if (!m || !X || !T || rows < 1) throw std::invalid_argument("sem_mlp_gradients: bad argument");
+        Mlp& n = *m->net;
+        n.reserve(rows);
+        SEM_HIP(hipMemcpyAsync(n.X, X, (size_t)rows * n.in_pad * sizeof(float), hipMemcpyHostToDevice, m->stream));
+        SEM_HIP(hipMemcpyAsync(n.T, T, (size_t)rows * n.out_pad * sizeof(float), hipMemcpyHostToDevice, m->stream));
+        const double l = n.backward(rows, true);
+        if (loss) *loss = l;
+        return (int)SEM_OK;
+    });
+}
+
+int sem_mlp_adam(sem_mlp* m, float lr)
+{
+    return guarded([&] {
+        if (!m) throw std::invalid_argument("null argument");
+        m->net->lr = lr;
+        m->net->adam_step();
+        SEM_HIP(hipStreamSynchronize(m->stream));
         return (int)SEM_OK;
     });
 }
