@@ -154,7 +154,7 @@ int sem_stepper_create(sem_mesh* m, int scheme /* 0 bdf2, 1 dirk4s3, 2 dirk4s5 *
 int sem_stepper_advance(sem_stepper* s, double T, long max_steps, sem_step_cb cb, void* user, int* success);
 int sem_stepper_read(sem_stepper* s, double* re, double* im, double* t);
 int sem_stepper_stats(const sem_stepper* s, sem_run_stats* out);
-/* learned guess: Adam steps per stage solve while training (default 2) and learning rate (default 1e-3) */
+/* learned guess: Adam steps per stage solve while training (default 10) and learning rate (default 3e-3) */
 int sem_stepper_set_learning(sem_stepper* s, int epochs, float lr);
 void sem_stepper_destroy(sem_stepper* s);
 /* One Schrodinger stage solve Res(alpha, V) z = b (schrodinger.hpp:223 etc.), z in/out. */

@@ -699,7 +699,7 @@ struct Stepper {
     double* ext = nullptr;
     double* tmp = nullptr;
     int learn_steps = 50;
-    int epochs = 2;
+    int epochs = 10;
     double s_feat[5][3] = {};  // per stage: scales of z, V, b features
     double s_out[5] = {};      // per stage: scale of the correction
     bool have_scale[5] = {}, have_out[5] = {};
@@ -923,6 +923,7 @@ static Stepper* stepper_create(sem_mesh* m, int scheme, const double* re, const 
     if (o->guess == 2) {
         const int N = m->L.N, NN = N * N;
         st->mlp.reset(new Mlp(7 * NN, 256, 256, 2 * NN, 0x5EED0001ull, m->stream));
+        st->mlp->lr = 3e-3f;
         st->learn_steps = o->learn_steps > 0 ? o->learn_steps : 50;
         st->ext = st->alloc();
         st->tmp = st->alloc();

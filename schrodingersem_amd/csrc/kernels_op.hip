@@ -259,8 +259,11 @@ __device__ __forceinline__ bool owns(int i, int l, unsigned bits)
 // the assembled (A v)_owner - cr1*r1 on every copy. alpha = dot(v, y) is
 // therefore the sum of v*y over interior entries plus v*(copy value) over
 // every interface copy, computed here as one partial per workgroup.
+// The FP32 instantiation asks for 5 waves per SIMD (a 128 x 128, degree-8 mesh
+// then fits in one wave round on 256 CUs); the FP64 one keeps the default,
+// where forcing occupancy costs more in register spills than it gains.
 template <int N, typename T, int KIND>
-__global__ __launch_bounds__(128) void k_minres_op(MinresOpArgs<T> a, Dm<N, T> D)
+__global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 5 : 1, 8))) void k_minres_op(MinresOpArgs<T> a, Dm<N, T> D)
 {
     using S = Shape<N>;
     constexpr int NC = (KIND == K_RES) ? 2 : 1;
