@@ -179,10 +179,11 @@ int sem_time_element_op(sem_mesh* m, int kind, int precision, int reps, double* 
  * or 1 (FP32 kernel); returns launches, summed microseconds and the
  * algorithmic bytes of one launch (Res apply plus the fused w/x update). */
 int sem_set_kernel_timing(sem_mesh* m, int on);
-/* Device-to-device copy of `bytes` bytes (read `bytes`, write `bytes`) with the
- * library's grid-stride copy kernel, `reps` launches: measured copy bandwidth
- * and the calibration kernel for HBM byte counters. */
-int sem_time_copy(long bytes, int reps, double* us_per_launch);
+/* Device-to-device copy of `bytes` bytes (read `bytes`, write `bytes`), `reps`
+ * launches: measured copy bandwidth and the calibration kernel for HBM byte
+ * counters. variant 0: the 8-byte grid-stride copy; v > 0: 16-byte
+ * nontemporal copy with 8*v workgroups per CU. */
+int sem_time_copy(long bytes, int reps, int variant, double* us_per_launch);
 int sem_kernel_timing(const sem_mesh* m, int which, long* launches, double* total_us, double* bytes_per_launch);
 
 /* ----- per-element MLP of the learned stage guess (no counterpart in the

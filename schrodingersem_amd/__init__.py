@@ -104,7 +104,7 @@ def lib():
                                           C.c_double, C.POINTER(_Results)]),
             "sem_time_element_op": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, _dp, _dp]),
             "sem_set_kernel_timing": (C.c_int, [C.c_void_p, C.c_int]),
-            "sem_time_copy": (C.c_int, [C.c_long, C.c_int, _dp]),
+            "sem_time_copy": (C.c_int, [C.c_long, C.c_int, C.c_int, _dp]),
             "sem_kernel_timing": (C.c_int, [C.c_void_p, C.c_int, _lp, _dp, _dp]),
             "sem_stepper_create": (C.c_int, [C.c_void_p, C.c_int, _dp, _dp, _dp, C.POINTER(_Opts),
                                              C.POINTER(C.c_void_p)]),
@@ -429,10 +429,10 @@ class Stepper:
         self.close()
 
 
-def time_copy(nbytes, reps=20):
+def time_copy(nbytes, reps=20, variant=1):
     """Mean microseconds per launch of a device copy of nbytes (nbytes read + nbytes written)."""
     us = C.c_double()
-    _check(lib().sem_time_copy(int(nbytes), int(reps), C.byref(us)))
+    _check(lib().sem_time_copy(int(nbytes), int(reps), int(variant), C.byref(us)))
     return us.value
 
 

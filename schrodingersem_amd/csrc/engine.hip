@@ -1650,16 +1650,25 @@ int sem_pcg_tridiag(long n, double lo, double di, double up, const double* b, do
     return tridiag_solve(false, n, lo, di, up, b, x, maxit, tol, out);
 }
 
-int sem_time_copy(long bytes, int reps, double* us_per_launch)
+int sem_time_copy(long bytes, int reps, int variant, double* us_per_launch)
 {
     return guarded([&] {
-        if (bytes < 8 || reps < 1) throw std::invalid_argument("sem_time_copy: bad argument");
+        if (bytes < 16 || reps < 1) throw std::invalid_argument("sem_time_copy: bad argument");
         if (!check_gpu()) return (int)SEM_EHIP;
-        const size_t n = (size_t)bytes / sizeof(double);
+        const size_t n = (size_t)bytes / sizeof(double) / 2 * 2;
         double *a = nullptr, *b = nullptr;
         hipStream_t s = nullptr;
         hipEvent_t e0 = nullptr, e1 = nullptr;
         int rc = SEM_OK;
+        int ncu = 256;
+        {
+            hipDeviceProp_t p;
+            if (hipGetDeviceProperties(&p, 0) == hipSuccess) ncu = p.multiProcessorCount;
+        }
+        auto launch = [&]() {
+            if (variant == 0) launch_cast<double, double>(b, a, n, s);
+            else launch_copy16(b, a, n, ncu * 8 * variant, s);
+        };
         try {
             SEM_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
             SEM_HIP(hipMalloc(&a, n * sizeof(double)));
@@ -1667,9 +1676,9 @@ int sem_time_copy(long bytes, int reps, double* us_per_launch)
             SEM_HIP(hipEventCreate(&e0));
             SEM_HIP(hipEventCreate(&e1));
             launch_fill<double>(a, 1.0, n, s);
-            launch_cast<double, double>(b, a, n, s);
+            launch();
             SEM_HIP(hipEventRecord(e0, s));
-            for (int r = 0; r < reps; ++r) launch_cast<double, double>(b, a, n, s);
+            for (int r = 0; r < reps; ++r) launch();
             SEM_HIP(hipEventRecord(e1, s));
             SEM_HIP(hipEventSynchronize(e1));
             float ms = 0;

@@ -183,6 +183,30 @@ template void launch_cast<double, float>(double*, const float*, size_t, hipStrea
 template void launch_cast<double, double>(double*, const double*, size_t, hipStream_t);
 template void launch_cast<float, float>(float*, const float*, size_t, hipStream_t);
 
+// streaming copy: 16-byte accesses (native vector type), four in flight per thread
+typedef double dvec2 __attribute__((ext_vector_type(2)));
+__global__ __launch_bounds__(256) void k_copy16(dvec2* __restrict__ out, const dvec2* __restrict__ in, size_t n2)
+{
+    const size_t stride = (size_t)gridDim.x * 256;
+    size_t k = (size_t)blockIdx.x * 256 + threadIdx.x;
+    for (; k + 3 * stride < n2; k += 4 * stride) {
+        const dvec2 a = __builtin_nontemporal_load(in + k);
+        const dvec2 b = __builtin_nontemporal_load(in + k + stride);
+        const dvec2 c = __builtin_nontemporal_load(in + k + 2 * stride);
+        const dvec2 d = __builtin_nontemporal_load(in + k + 3 * stride);
+        __builtin_nontemporal_store(a, out + k);
+        __builtin_nontemporal_store(b, out + k + stride);
+        __builtin_nontemporal_store(c, out + k + 2 * stride);
+        __builtin_nontemporal_store(d, out + k + 3 * stride);
+    }
+    for (; k < n2; k += stride) __builtin_nontemporal_store(__builtin_nontemporal_load(in + k), out + k);
+}
+
+void launch_copy16(double* out, const double* in, size_t n, int blocks, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_copy16, dim3(blocks), dim3(256), 0, s, (dvec2*)out, (const dvec2*)in, n / 2);
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_fill(T* out, T v, size_t n)
 {

@@ -58,6 +58,8 @@ template <typename D, typename S>
 void launch_cast(D* out, const S* in, size_t n, hipStream_t s);
 template <typename T>
 void launch_fill(T* out, T v, size_t n, hipStream_t s);
+// n doubles (even), 16-byte nontemporal accesses
+void launch_copy16(double* out, const double* in, size_t n, int blocks, hipStream_t s);
 template <typename T>
 void launch_mul(T* out, const T* a, const T* b, size_t n, hipStream_t s);
 template <typename T>
