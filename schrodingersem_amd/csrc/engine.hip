@@ -36,6 +36,7 @@ struct KrylovOpts {
     double tol = 0;
     double atol = 0;      // absolute term of the stop test (tol in the reference)
     bool rel_to_b = false; // stop relative to ||b|| instead of ||b - A x0||
+    int hint_slot = -1;    // iteration-count history slot (default: by precision)
 };
 
 }  // namespace sem
@@ -69,7 +70,8 @@ struct sem_mesh {
     int n_partial = 0;
     std::unordered_map<std::string, DevBuf> pool;
     sem_run_stats stats{};
-    int batch = 8;
+    int batch = 8;          // Lanczos steps launched between host checks of the stop flag
+    int iter_hint[8] = {};  // iterations of the last MINRES solve per hint slot
     // live timing of the MINRES Lanczos-step kernel (index 0 FP64, 1 FP32)
     bool timing = false;
     std::vector<hipEvent_t> ev;
@@ -439,8 +441,13 @@ static Results minres_fused(sem_mesh* m, int kind, const T* cf, double alpha, co
     }
     const int nch = m->L.nchunks;
     const int ti = sizeof(T) == 8 ? 0 : 1;
+    // The first batch runs most of the expected iteration count without a
+    // host round trip (the previous solve of this precision predicts it);
+    // steps launched after the stop flag is set exit at once.
+    const int slot = o.hint_slot >= 0 ? std::min(o.hint_slot, 7) : ti;
+    const int first_batch = std::max(m->batch, (int)(0.85 * m->iter_hint[slot]));
     if (m->timing) {
-        while ((int)m->ev.size() < 2 * m->batch) {
+        while ((int)m->ev.size() < 2 * first_batch) {
             hipEvent_t e;
             SEM_HIP(hipEventCreate(&e));
             m->ev.push_back(e);
@@ -456,7 +463,8 @@ static Results minres_fused(sem_mesh* m, int kind, const T* cf, double alpha, co
     while (!done) {
         int launched = 0;
         const int i0 = i;
-        for (int b_ = 0; b_ < m->batch && i <= o.maxit + 1; ++b_, ++i) {
+        const int nbatch = (i == 1) ? first_batch : m->batch;
+        for (int b_ = 0; b_ < nbatch && i <= o.maxit + 1; ++b_, ++i) {
             MinresOpArgs<T> a{};
             a.r2 = R[(i - 1) % 3];
             a.r1 = R[(i + 1) % 3];
@@ -511,13 +519,13 @@ static Results minres_fused(sem_mesh* m, int kind, const T* cf, double alpha, co
     if (m->h_sc->p3_pending) {
         const int j = m->h_sc->iter;  // last Lanczos step
         launch_minres_flush<T>(R[(j + 2) % 3], W[(j + 1) % 3], W[(j + 2) % 3], W[j % 3], x, n, m->sc, s);
-        m->read_scalars();
     }
     Results r;
     r.success = m->h_sc->success;
     r.n_iter = r.success ? m->h_sc->iter : o.maxit + 1;
     if (o.maxit < 1) r.n_iter = 1;
     r.residual = m->h_sc->phibar;
+    m->iter_hint[slot] = m->h_sc->iter;
     return r;
 }
 
@@ -611,6 +619,7 @@ static Results solve_mixed(sem_mesh* m, int kind, const double* cf64, const floa
         // relative reduction of this sweep; never ask FP32 for more than it can give
         io.tol = std::max(inner_tol, 0.5 * target / std::max(rn, 1e-300));
         io.atol = 0;
+        io.hint_slot = 2 + std::min(sweeps, 5);
         Results ir = minres_fused<float>(m, kind, cf32, alpha, r32, d32, true, io);
         inner_total += ir.n_iter;
         launch_cast<double, float>(Ax, d32, n, s);
