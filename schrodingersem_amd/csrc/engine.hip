@@ -65,6 +65,7 @@ struct sem_mesh {
     std::vector<float> D32;
     // scalars and partials
     KrylovScalars* sc = nullptr;
+    KrylovScalars* sc_alt = nullptr;  // second state buffer of the Lanczos-step kernels
     KrylovScalars* h_sc = nullptr;  // pinned
     double* partial = nullptr;
     int n_partial = 0;
@@ -94,6 +95,7 @@ struct sem_mesh {
         hipFree(eside);
         hipFree(ecorner);
         hipFree(sc);
+        hipFree(sc_alt);
         hipFree(partial);
         if (h_sc) hipHostFree(h_sc);
         if (stream) hipStreamDestroy(stream);
@@ -317,6 +319,8 @@ static void setup_device(sem_mesh* m)
 
     SEM_HIP(hipMalloc(&m->sc, sizeof(KrylovScalars)));
     SEM_HIP(hipMemset(m->sc, 0, sizeof(KrylovScalars)));
+    SEM_HIP(hipMalloc(&m->sc_alt, sizeof(KrylovScalars)));
+    SEM_HIP(hipMemset(m->sc_alt, 0, sizeof(KrylovScalars)));
     SEM_HIP(hipHostMalloc(&m->h_sc, sizeof(KrylovScalars), hipHostMallocDefault));
     m->n_partial = 2 * L.nchunks + m->dss_nb + 1024;
     SEM_HIP(hipMalloc(&m->partial, (size_t)m->n_partial * sizeof(double)));
@@ -403,8 +407,8 @@ static double wdot_host(sem_mesh* m, const T* a, const T* b, int ncomp)
 }
 
 // ---------------------------------------------------------------- MINRES --
-// minres.hpp:19-107 with IdentityPreconditioner, three fused launches plus
-// two single-block scalar updates per Lanczos step. Vectors are T (FP64 or
+// minres.hpp:19-107 with IdentityPreconditioner, two fused launches per
+// Lanczos step (the scalar recurrence runs inside them). Vectors are T (FP64 or
 // FP32); every reduction and every recurrence scalar is FP64.
 template <typename T>
 static Results minres_fused(sem_mesh* m, int kind, const T* cf, double alpha, const T* b, T* x, bool x_zero,
@@ -458,11 +462,19 @@ static Results minres_fused(sem_mesh* m, int kind, const T* cf, double alpha, co
         const double op_vals = (kind == K_RES) ? 9 : (kind == K_HELMHOLTZ ? 7 : 5);
         m->kt_bytes[ti] = (op_vals + 1.0 * nc) * sizeof(T) * EN2;
     }
+    // Scalar state: the operator kernel of step i > 1 (not first in its batch)
+    // applies the beta recurrence of step i-1 in every workgroup and stores
+    // it in the other of the two buffers; the update kernel of step i reads
+    // that buffer and stores alpha there. A closing kernel ends each batch
+    // with the state back in m->sc.
+    KrylovScalars* buf[2] = {m->sc, m->sc_alt};
     int i = 1;
+    int nb2 = nch;
     bool done = o.maxit < 1;
     while (!done) {
         int launched = 0;
         const int i0 = i;
+        int cur = 0;
         const int nbatch = (i == 1) ? first_batch : m->batch;
         for (int b_ = 0; b_ < nbatch && i <= o.maxit + 1; ++b_, ++i) {
             MinresOpArgs<T> a{};
@@ -476,12 +488,18 @@ static Results minres_fused(sem_mesh* m, int kind, const T* cf, double alpha, co
             a.nchunks = nch;
             a.obits = m->obits;
             a.partial = m->partial;
-            a.sc = reinterpret_cast<const double*>(m->sc);
+            a.st_in = reinterpret_cast<const double*>(buf[cur]);
+            a.first = (b_ == 0);
+            if (b_ > 0) {
+                cur ^= 1;
+                a.st_out = reinterpret_cast<double*>(buf[cur]);
+                a.prev_partial = m->partial + nch;
+                a.n_prev = nb2;
+            }
             if (m->timing) SEM_HIP(hipEventRecord(m->ev[2 * b_], s));
             launch_minres_op_d<T>(m->L.N, kind, a, m->dmat<T>(), s);
             if (m->timing) SEM_HIP(hipEventRecord(m->ev[2 * b_ + 1], s));
             ++launched;
-            launch_minres_fin_alpha(m->sc, m->partial, nch, s);
             MinresUpdArgs<T> u{};
             u.yraw = Yraw;
             u.y = R[i % 3];
@@ -500,10 +518,12 @@ static Results minres_fused(sem_mesh* m, int kind, const T* cf, double alpha, co
             u.c_off = m->c_off;
             u.c_pos = m->c_pos;
             u.partial = m->partial + nch;
-            u.sc = reinterpret_cast<const double*>(m->sc);
-            const int nb2 = launch_minres_upd_n<T>(m->L.N, u, s);
-            launch_minres_fin_beta(m->sc, m->partial + nch, nb2, s);
+            u.st = reinterpret_cast<double*>(buf[cur]);
+            u.op_partial = m->partial;
+            u.n_op = nch;
+            nb2 = launch_minres_upd_n<T>(m->L.N, u, s);
         }
+        launch_minres_close(kind, buf[cur], m->sc, m->partial + nch, nb2, s);
         m->read_scalars();
         if (m->timing) {
             // launches that ran a full Lanczos step (later ones in the batch exit early)

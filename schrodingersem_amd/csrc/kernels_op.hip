@@ -11,6 +11,7 @@
 //   K u (i,l) = sum_k D(k,i) G1(k,l) + sum_k D(k,l) G2(i,k),
 //   G1 = gA u_xi - gB u_eta, G2 = gC u_eta - gB u_xi,
 // which is F W + W G of glaplace.hpp:22-25 with the weights moved inside.
+#include "krylov_device.hpp"
 #include "sem_internal.hpp"
 
 namespace sem {
@@ -235,6 +236,30 @@ __device__ __forceinline__ double wave_sum(double v)
     return v;
 }
 
+// Sum of p[0..n) by a whole workgroup of nw waves in a fixed order; every
+// thread returns the same value, and so does every workgroup. Each thread
+// adds p[tid], p[tid + 64 nw], ... in index order; the loads are issued 16 at
+// a time so that a thread waits for memory once per 16 values.
+__device__ __forceinline__ double group_sum(const double* __restrict__ p, int n, double* sh, int nw)
+{
+    const int stride = nw * 64;
+    double s = 0;
+    for (int k = threadIdx.x; k < n; k += 16 * stride) {
+        double v[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) v[u] = (k + u * stride < n) ? p[k + u * stride] : 0.0;
+#pragma unroll
+        for (int u = 0; u < 16; ++u) s += v[u];
+    }
+    s = wave_sum(s);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+    __syncthreads();
+    double t = sh[0];
+    for (int w = 1; w < nw; ++w) t += sh[w];
+    __syncthreads();
+    return t;
+}
+
 // owner test of entry (i, l) of an element with ownership bits (bit s-1:
 // side s owned; bit 4+k: corner k owned)
 template <int N>
@@ -269,8 +294,15 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(sizeof(T) =
     constexpr int NC = (KIND == K_RES) ? 2 : 1;
     __shared__ T lds[NC][2][S::LDS];
     __shared__ double wpart[NC];
-    const KrylovScalars* sc = reinterpret_cast<const KrylovScalars*>(a.sc);
-    const int done = sc->done;
+    MinresState sc = *reinterpret_cast<const MinresState*>(a.st_in);
+    if (!a.first) {
+        // the previous step's beta recurrence, evaluated by every workgroup
+        const double s2 = group_sum(a.prev_partial, a.n_prev, wpart, NC);
+        if (!sc.done) minres_beta_update(sc, s2);
+        else sc.p3_pending = 0;  // the update kernel of the stopped step applied it
+        if (blockIdx.x == 0 && threadIdx.x == 0) *reinterpret_cast<MinresState*>(a.st_out) = sc;
+    }
+    const int done = sc.done;
     const int w = threadIdx.x >> 6;  // output component of this wave
     const int lane = threadIdx.x & 63;
     const bool act = lane < S::KN;
@@ -280,8 +312,8 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(sizeof(T) =
     const size_t len = a.len;
     double part = 0;
     if (!done) {
-        const T ib = (T)sc->ib, cr1 = (T)sc->cr1;
-        const int has_r1 = sc->has_r1;
+        const T ib = (T)sc.ib, cr1 = (T)sc.cr1;
+        const int has_r1 = sc.has_r1;
         const unsigned bits = act ? a.obits[blockIdx.x * S::K + el] : 0u;
         T ga[N], gb[N], gc[N], qs[N], y[N];
         // stiffness input: the other component for Res, the only one otherwise
@@ -335,117 +367,145 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(sizeof(T) =
     }
 }
 
-// Second kernel of a Lanczos step, four chunks per 256-thread workgroup,
-// lane = column. Every entry of y is finished here: interface copies gather
-// the unassembled values of all copies of their node (edge: owner + other,
+// Closes a batch of Lanczos steps: the beta recurrence of the last step with
+// the same reduction (NC waves, same order) as the operator kernel of the
+// step that would follow, so the scalars do not depend on where batches end.
+// Reads *in, writes *out (single workgroup; in may equal out).
+template <int NC>
+__global__ __launch_bounds__(64 * NC) void k_minres_close(const KrylovScalars* in, KrylovScalars* out,
+                                                           const double* partial, int n)
+{
+    __shared__ double wpart[NC];
+    const double s2 = group_sum(partial, n, wpart, NC);
+    if (threadIdx.x == 0) {
+        MinresState sc = *static_cast<const MinresState*>(in);
+        if (!sc.done) minres_beta_update(sc, s2);
+        else sc.p3_pending = 0;  // the update kernel of the stopped step applied it
+        *static_cast<MinresState*>(out) = sc;
+    }
+}
+
+#ifndef SEM_OP_FP32  // one definition across the FP64 and FP32 translation units
+void launch_minres_close(int kind, const KrylovScalars* in, KrylovScalars* out, const double* partial, int n,
+                         hipStream_t s)
+{
+    if (kind == K_RES) hipLaunchKernelGGL(k_minres_close<2>, dim3(1), dim3(128), 0, s, in, out, partial, n);
+    else hipLaunchKernelGGL(k_minres_close<1>, dim3(1), dim3(64), 0, s, in, out, partial, n);
+}
+#endif
+
+
+// Second kernel of a Lanczos step: one workgroup per chunk, 2*NC waves.
+// Waves 0..NC-1 finish component w of y: interface copies gather the
+// unassembled values of all copies of their node (edge: owner + other,
 // global_sum.hpp:28-34; corner: left fold from +0.0 in connected_elements
 // order, global_sum.hpp:54-63), so every copy computes the same sum bit for
 // bit; Dirichlet entries become zero; then y -= c2*r2 (minres.hpp:55) and the
-// beta^2 partial with unique-DOF weights (minres.hpp:63). The pending w/x
-// update of the previous step runs here too (it reads r1 = R[i-2]).
+// beta^2 partial with unique-DOF weights (minres.hpp:63). Waves NC..2NC-1 run
+// the pending w/x update of the previous step on component w - NC
+// (w_new = ig*((ibp*r1 - eo*wa) - dl*wb), x += phi*w_new; minres.hpp:82-86).
 template <int N, typename T>
 __global__ __launch_bounds__(256) void k_minres_upd(MinresUpdArgs<T> a)
 {
     using S = Shape<N>;
     constexpr int K = S::K, KN = S::KN;
     __shared__ double wpart[4];
-    const KrylovScalars* sc = reinterpret_cast<const KrylovScalars*>(a.sc);
+    const KrylovScalars* sc = reinterpret_cast<const KrylovScalars*>(a.st);
     const int done = sc->done, p3 = sc->p3_pending;
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int chunk = blockIdx.x * 4 + wv;
-    const bool act = lane < KN && chunk < a.nchunks;
+    const int nc = a.ncomp;
+    // alpha = dot(v, y) from the operator kernel's partials (minres.hpp:54),
+    // c2 = alpha / beta (minres.hpp:55); workgroup 0 keeps alpha for the recurrence
+    double c2d = 0;
+    if (!done) {
+        const double alpha = group_sum(a.op_partial, a.n_op, wpart, 2 * nc);
+        c2d = alpha / sc->beta;
+        if (blockIdx.x == 0 && threadIdx.x == 0) reinterpret_cast<KrylovScalars*>(a.st)->alpha = alpha;
+    }
+    const int chunk = blockIdx.x;
+    const bool act = lane < KN;
+    const size_t base = (size_t)chunk * K * N * N + lane;
+    const size_t len = a.len;
     double part = 0;
-    if (act) {
+    if (act && wv >= nc && p3) {
+        const size_t o = (size_t)(wv - nc) * len + base;
+        const T ibp = (T)sc->p3_ibp, ig = (T)sc->p3_ig, eo = (T)sc->p3_eo, dl = (T)sc->p3_dl, phi = (T)sc->p3_phi;
+        T r[N], wa[N], wb[N], x[N];
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+            r[i] = a.r1[o + (size_t)i * KN];
+            wa[i] = a.wa[o + (size_t)i * KN];
+            wb[i] = a.wb[o + (size_t)i * KN];
+            x[i] = a.x[o + (size_t)i * KN];
+        }
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+            const T v = ibp * r[i];
+            const T wn = ig * ((v - eo * wa[i]) - dl * wb[i]);
+            a.wn[o + (size_t)i * KN] = wn;
+            a.x[o + (size_t)i * KN] = x[i] + phi * wn;
+        }
+    }
+    if (act && wv < nc && !done) {
         const int el = lane / N, j = lane % N;
         const int e = chunk * K + el;
-        const size_t base = (size_t)chunk * K * N * N + lane;
-        const size_t len = a.len;
-        if (p3) {
-            const T ibp = (T)sc->p3_ibp, ig = (T)sc->p3_ig, eo = (T)sc->p3_eo, dl = (T)sc->p3_dl, phi = (T)sc->p3_phi;
-            for (int c = 0; c < a.ncomp; ++c) {
-                T r[N], wa[N], wb[N], x[N];
-                const size_t o = (size_t)c * len + base;
+        const T c2 = (T)c2d;
+        const unsigned bits = a.obits[e];
+        const T* yr = a.yraw + (size_t)wv * len;
+        auto side_pos = [&](int d, int k) -> long {
+            const int e2 = d >> 3, s2 = ((d >> 1) & 3) + 1;
+            const int k2 = (d & 1) ? N - 1 - k : k;
+            int i2, j2;
+            if (s2 == 1) { i2 = k2; j2 = 0; }
+            else if (s2 == 2) { i2 = N - 1; j2 = k2; }
+            else if (s2 == 3) { i2 = k2; j2 = N - 1; }
+            else { i2 = 0; j2 = k2; }
+            return (long)(e2 / K) * K * N * N + (long)i2 * KN + (long)(e2 % K) * N + j2;
+        };
+        T v[N], r[N];
 #pragma unroll
-                for (int i = 0; i < N; ++i) {
-                    r[i] = a.r1[o + (size_t)i * KN];
-                    wa[i] = a.wa[o + (size_t)i * KN];
-                    wb[i] = a.wb[o + (size_t)i * KN];
-                    x[i] = a.x[o + (size_t)i * KN];
-                }
-#pragma unroll
-                for (int i = 0; i < N; ++i) {
-                    const T v = ibp * r[i];
-                    const T wn = ig * ((v - eo * wa[i]) - dl * wb[i]);
-                    a.wn[o + (size_t)i * KN] = wn;
-                    a.x[o + (size_t)i * KN] = x[i] + phi * wn;
-                }
-            }
+        for (int i = 0; i < N; ++i) {
+            v[i] = yr[base + (size_t)i * KN];
+            r[i] = a.r2[(size_t)wv * len + base + (size_t)i * KN];
         }
-        if (!done) {
-            const T c2 = (T)sc->c2;
-            const unsigned bits = a.obits[e];
-            // partner position of every interface entry of this column (or -1: Dirichlet)
-            auto side_pos = [&](int d, int k) -> long {
-                const int e2 = d >> 3, s2 = ((d >> 1) & 3) + 1;
-                const int k2 = (d & 1) ? N - 1 - k : k;
-                int i2, j2;
-                if (s2 == 1) { i2 = k2; j2 = 0; }
-                else if (s2 == 2) { i2 = N - 1; j2 = k2; }
-                else if (s2 == 3) { i2 = k2; j2 = N - 1; }
-                else { i2 = 0; j2 = k2; }
-                return (long)(e2 / K) * K * N * N + (long)i2 * KN + (long)(e2 % K) * N + j2;
-            };
-            const int dcol = (j == 0) ? a.eside[4 * e + 0] : ((j == N - 1) ? a.eside[4 * e + 2] : 0);
-            const int d4 = a.eside[4 * e + 3], d2 = a.eside[4 * e + 1];
-            const int c_lo = (j == 0) ? a.ecorner[4 * e + 0] : ((j == N - 1) ? a.ecorner[4 * e + 3] : -1);
-            const int c_hi = (j == 0) ? a.ecorner[4 * e + 1] : ((j == N - 1) ? a.ecorner[4 * e + 2] : -1);
-            for (int c = 0; c < a.ncomp; ++c) {
-                const T* yr = a.yraw + (size_t)c * len;
-                T v[N], r[N], pv[N];
-                const size_t o = base;
+        if (j == 0 || j == N - 1) {
+            // column on side 1 (j = 0) or side 3 (j = N-1): planes 1..N-2; corners at planes 0, N-1
+            const int dcol = a.eside[4 * e + (j == 0 ? 0 : 2)];
+            const int c_lo = a.ecorner[4 * e + (j == 0 ? 0 : 3)];
+            const int c_hi = a.ecorner[4 * e + (j == 0 ? 1 : 2)];
+            T pv[N];
+            if (dcol >= 0) {
 #pragma unroll
-                for (int i = 0; i < N; ++i) {
-                    v[i] = yr[o + (size_t)i * KN];
-                    r[i] = a.r2[(size_t)c * len + o + (size_t)i * KN];
-                    pv[i] = 0;
-                }
-                const bool rj = (j == 0 || j == N - 1);
-                if (rj) {
-                    // column on side 1 (j = 0) or side 3 (j = N-1): planes 1..N-2
-                    if (dcol >= 0) {
-#pragma unroll
-                        for (int i = 1; i < N - 1; ++i) pv[i] = yr[side_pos(dcol, i)];
-                    }
-                    // corners at planes 0 and N-1
-                    T s0 = 0, s1 = 0;
-                    if (c_lo >= 0)
-                        for (int q = a.c_off[c_lo]; q < a.c_off[c_lo + 1]; ++q) s0 += yr[a.c_pos[q]];
-                    if (c_hi >= 0)
-                        for (int q = a.c_off[c_hi]; q < a.c_off[c_hi + 1]; ++q) s1 += yr[a.c_pos[q]];
-#pragma unroll
-                    for (int i = 1; i < N - 1; ++i) v[i] = (dcol >= 0) ? v[i] + pv[i] : T(0);
-                    v[0] = s0;
-                    v[N - 1] = s1;
-                } else {
-                    // plane 0 is side 4, plane N-1 is side 2
-                    const T p0 = (d4 >= 0) ? yr[side_pos(d4, j)] : T(0);
-                    const T p1 = (d2 >= 0) ? yr[side_pos(d2, j)] : T(0);
-                    v[0] = (d4 >= 0) ? v[0] + p0 : T(0);
-                    v[N - 1] = (d2 >= 0) ? v[N - 1] + p1 : T(0);
-                }
-#pragma unroll
-                for (int i = 0; i < N; ++i) {
-                    const T yv = v[i] - c2 * r[i];
-                    a.y[(size_t)c * len + o + (size_t)i * KN] = yv;
-                    if (owns<N>(i, j, bits)) part += (double)yv * (double)yv;
-                }
+                for (int i = 1; i < N - 1; ++i) pv[i] = yr[side_pos(dcol, i)];
             }
+            T s0 = 0, s1 = 0;
+            if (c_lo >= 0)
+                for (int q = a.c_off[c_lo]; q < a.c_off[c_lo + 1]; ++q) s0 += yr[a.c_pos[q]];
+            if (c_hi >= 0)
+                for (int q = a.c_off[c_hi]; q < a.c_off[c_hi + 1]; ++q) s1 += yr[a.c_pos[q]];
+#pragma unroll
+            for (int i = 1; i < N - 1; ++i) v[i] = (dcol >= 0) ? v[i] + pv[i] : T(0);
+            v[0] = s0;
+            v[N - 1] = s1;
+        } else {
+            // plane 0 is side 4, plane N-1 is side 2
+            const int d4 = a.eside[4 * e + 3], d2 = a.eside[4 * e + 1];
+            const T p0 = (d4 >= 0) ? yr[side_pos(d4, j)] : T(0);
+            const T p1 = (d2 >= 0) ? yr[side_pos(d2, j)] : T(0);
+            v[0] = (d4 >= 0) ? v[0] + p0 : T(0);
+            v[N - 1] = (d2 >= 0) ? v[N - 1] + p1 : T(0);
+        }
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+            const T yv = v[i] - c2 * r[i];
+            a.y[(size_t)wv * len + base + (size_t)i * KN] = yv;
+            if (owns<N>(i, j, bits)) part += (double)yv * (double)yv;
         }
     }
     part = wave_sum(part);
     if (lane == 0) wpart[wv] = part;
     __syncthreads();
-    if (threadIdx.x == 0) a.partial[blockIdx.x] = ((wpart[0] + wpart[1]) + wpart[2]) + wpart[3];
+    if (threadIdx.x == 0) a.partial[blockIdx.x] = (nc == 2) ? wpart[0] + wpart[1] : wpart[0];
 }
 
 // PCG step (pcg.hpp:70-80, identity preconditioner): p = r (first step) or
@@ -535,9 +595,8 @@ static void pcg_op_N(const PcgOpArgs<T>& a, int nchunks, const T* d, hipStream_t
 template <int N, typename T>
 static int minres_upd_N(const MinresUpdArgs<T>& a, hipStream_t s)
 {
-    const int nb = (a.nchunks + 3) / 4;
-    hipLaunchKernelGGL((k_minres_upd<N, T>), dim3(nb), dim3(256), 0, s, a);
-    return nb;
+    hipLaunchKernelGGL((k_minres_upd<N, T>), dim3(a.nchunks), dim3(128 * a.ncomp), 0, s, a);
+    return a.nchunks;
 }
 
 #define SEM_DISPATCH_N(N, CALL)                                   \

@@ -333,7 +333,6 @@ __global__ void k_minres_init(KrylovScalars* sc, int slot, int maxit, double tol
     sc->alpha = 0;
     sc->ib = 1.0 / beta1;
     sc->cr1 = 0;
-    sc->c2 = 0;
     sc->p3_pending = 0;
     sc->iter = 0;
     sc->maxit = maxit;
@@ -345,82 +344,6 @@ __global__ void k_minres_init(KrylovScalars* sc, int slot, int maxit, double tol
 void launch_minres_init(KrylovScalars* sc, int slot, int maxit, double tol, hipStream_t s)
 {
     hipLaunchKernelGGL(k_minres_init, dim3(1), dim3(64), 0, s, sc, slot, maxit, tol);
-}
-
-// beta = sqrt(dot(y,y)) and the Givens update of minres.hpp:62-99, given
-// s = dot(y, y) of the step just finished.
-__device__ void minres_beta_update(KrylovScalars* sc, double s)
-{
-    const double alpha = sc->alpha;
-    const double beta_prev = sc->beta;
-    const double ib_prev = sc->ib;
-    sc->beta_old = beta_prev;
-    const double beta = sqrt(s);
-    sc->beta = beta;
-    sc->tnorm += alpha * alpha + beta * beta + beta_prev * beta_prev;
-    const double eps_old = sc->eps;
-    sc->eps_old = eps_old;
-    const double cs = sc->cs, sn = sc->sn, dbar = sc->dbar;
-    const double delta = cs * dbar + sn * alpha;
-    const double gbar = sn * dbar - cs * alpha;
-    sc->eps = sn * beta;
-    sc->dbar = -cs * beta;
-    const double gamma = fmax(sqrt(gbar * gbar + beta * beta), DBL_EPSILON);
-    sc->cs = gbar / gamma;
-    sc->sn = beta / gamma;
-    const double phi = sc->cs * sc->phibar;
-    sc->phibar *= sc->sn;
-    // pending w/x update of this step (minres.hpp:82-86)
-    sc->p3_ibp = ib_prev;
-    sc->p3_ig = 1.0 / gamma;
-    sc->p3_eo = eps_old;
-    sc->p3_dl = delta;
-    sc->p3_phi = phi;
-    sc->p3_pending = 1;
-    // coefficients of the next Lanczos step (minres.hpp:48-52)
-    sc->ib = 1.0 / beta;
-    sc->cr1 = beta / beta_prev;
-    sc->has_r1 = 1;
-    sc->iter += 1;
-    if (sc->phibar < sc->tol * sc->b_norm + sc->atol) {  // minres.hpp:92 (atol = tol there)
-        sc->success = 1;
-        sc->done = 1;
-    } else if (sc->iter >= sc->maxit) {
-        sc->done = 1;
-    }
-}
-
-// alpha = dot(v, y) from the operator kernel's partials (minres.hpp:54),
-// c2 = alpha / beta (minres.hpp:55).
-__global__ __launch_bounds__(256) void k_minres_fin_alpha(KrylovScalars* sc, const double* partial, int n)
-{
-    if (sc->done) return;
-    const double s = sum_partials(partial, n);
-    if (threadIdx.x == 0) {
-        sc->alpha = s;
-        sc->c2 = s / sc->beta;
-    }
-}
-
-void launch_minres_fin_alpha(KrylovScalars* sc, const double* partial, int n, hipStream_t s)
-{
-    hipLaunchKernelGGL(k_minres_fin_alpha, dim3(1), dim3(256), 0, s, sc, partial, n);
-}
-
-__global__ __launch_bounds__(256) void k_minres_fin_beta(KrylovScalars* sc, const double* partial, int n)
-{
-    if (sc->done) {
-        // the update kernel of this step applied the pending w/x update
-        if (threadIdx.x == 0) sc->p3_pending = 0;
-        return;
-    }
-    const double s = sum_partials(partial, n);
-    if (threadIdx.x == 0) minres_beta_update(sc, s);
-}
-
-void launch_minres_fin_beta(KrylovScalars* sc, const double* partial, int n, hipStream_t s)
-{
-    hipLaunchKernelGGL(k_minres_fin_beta, dim3(1), dim3(256), 0, s, sc, partial, n);
 }
 
 // final pending w/x update after the loop ends (minres.hpp:82-86)

@@ -153,8 +153,15 @@ struct MinresOpArgs {
     size_t len;
     int nchunks;
     const uint8_t* obits;  // ownership bits per element
-    double* partial;       // per-workgroup alpha partial sums
-    const double* sc;      // device scalars (KrylovScalars)
+    double* partial;       // per-workgroup alpha partial sums (out)
+    // Krylov scalars: step 1 reads st_in as is; later steps apply the beta
+    // recurrence of the previous step to st_in with the update kernel's
+    // partials (prev_partial, n_prev) and workgroup 0 stores it in st_out
+    const double* st_in;
+    double* st_out;
+    const double* prev_partial;
+    int n_prev;
+    int first;
 };
 
 // Arguments of the MINRES update kernel: interface assembly by gather,
@@ -177,17 +184,23 @@ struct MinresUpdArgs {
     const int* ecorner;  // per element corner: interior corner index, or -1 on the Dirichlet boundary
     const int* c_off;
     const int* c_pos;
-    double* partial;
-    const double* sc;
+    double* partial;           // per-workgroup beta^2 partial sums (out)
+    double* st;                // scalars after the previous step (workgroup 0 stores alpha here)
+    const double* op_partial;  // the operator kernel's alpha partials
+    int n_op;
 };
 
 // Device-resident Krylov scalars (FP64), written by single-block finalize kernels.
-struct KrylovScalars {
-    // MINRES (minres.hpp:28-40 names)
+// MINRES recurrence state (minres.hpp:28-40 names); the Lanczos-step kernels
+// copy and store only this part of KrylovScalars.
+struct MinresState {
     double beta, beta_old, tnorm, dbar, eps, eps_old, phibar, cs, sn, b_norm, tol, atol;
-    double alpha, ib, cr1, c2;
+    double alpha, ib, cr1;
     double p3_ibp, p3_ig, p3_eo, p3_dl, p3_phi;
     int p3_pending, iter, done, success, maxit, has_r1;
+};
+
+struct KrylovScalars : MinresState {
     // PCG (pcg.hpp:62-89)
     double rho, rho_prev, pcg_beta, pcg_alpha, bnorm, resid;
     int first;
@@ -216,6 +229,9 @@ template <typename T>
 void launch_minres_op_d(int N, int kind, const MinresOpArgs<T>& a, const T* d, hipStream_t s);
 template <typename T>
 void launch_pcg_op_d(int N, const PcgOpArgs<T>& a, int nchunks, const T* d, hipStream_t s);
+// beta recurrence closing a batch of Lanczos steps (reads *in, writes *out)
+void launch_minres_close(int kind, const KrylovScalars* in, KrylovScalars* out, const double* partial, int n,
+                         hipStream_t s);
 // update kernel of a Lanczos step; returns the number of partials written
 template <typename T>
 int launch_minres_upd_n(int N, const MinresUpdArgs<T>& a, hipStream_t s);

@@ -69,8 +69,6 @@ int launch_edot(const double* a, const double* b, size_t n, double* partial, hip
 void launch_fin_sum(const double* partial, int n, double* dst, hipStream_t s);
 
 void launch_minres_init(KrylovScalars* sc, int slot, int maxit, double tol, hipStream_t s);
-void launch_minres_fin_alpha(KrylovScalars* sc, const double* partial, int n, hipStream_t s);
-void launch_minres_fin_beta(KrylovScalars* sc, const double* partial, int n, hipStream_t s);
 template <typename T>
 void launch_minres_flush(const T* r1, const T* wa, const T* wb, T* wn, T* x, size_t n, KrylovScalars* sc,
                          hipStream_t s);
