@@ -24,8 +24,11 @@ A third, untimed run at tolerance 1e-14 (FP64) is the accuracy reference:
 value = stage solves/s of the fastest mode whose final psi agrees with it to
 1e-9 relative L2.
 
+Before timing, the autotuner picks the work shape of the two Lanczos-step
+kernels on this mesh for each precision (results do not depend on it).
 Also measured: live HIP-event timing of the dominant kernel (the fused
-MINRES Lanczos-step kernel) over the timed region -> roofline; glaplace
+MINRES Lanczos-step operator kernel, sampled every fourth Lanczos step) over
+the timed region -> roofline, and of the update kernel beside it; glaplace
 GB/s on a 512 x 512, degree 7 mesh (671 MB of algorithmic traffic, larger
 than the 256 MB Infinity Cache); the CPU oracle timed on one stage solve of
 the same workload (bounded sample) -> cpu_baseline.
@@ -79,6 +82,7 @@ def run_mode(sem, m, Vref, re, im, dt, tol, steps, warmup, precision, guess):
     t1 = time.perf_counter()
     which = 0 if precision == sem.PREC_FP64 else 1
     kt = sem.kernel_timing(m, which)
+    kt_upd = sem.kernel_timing(m, which + 2)
     sem.set_kernel_timing(m, False)
     s1 = st.stats()
     a, b, t = st.read()
@@ -86,7 +90,7 @@ def run_mode(sem, m, Vref, re, im, dt, tol, steps, warmup, precision, guess):
     solves = s1["stage_solves"] - s0["stage_solves"]
     iters = s1["total_iters"] - s0["total_iters"]
     return dict(ok=ok, seconds=t1 - t0, stage_solves=solves, iters=iters, psi=np.concatenate([a, b]),
-                kernel=kt, refinements=s1["refinements"] - s0["refinements"], warmup_seconds=tw,
+                kernel=kt, kernel_upd=kt_upd, refinements=s1["refinements"] - s0["refinements"], warmup_seconds=tw,
                 mlp_loss=s1["mlp_last_loss"])
 
 
@@ -114,6 +118,12 @@ def main():
     m, xy, q, Vref, re, im = config3(sem, args.n, args.p)
     dt_ref = 1e-3 / 2.0
     log("mesh: E=%d N=%d dof=%d" % (m.n_elements, m.N, m.dof()))
+    # per-precision work shape of the Lanczos-step kernels, picked by the autotuner on this mesh
+    tuned = {}
+    for name, prec in (("fp64", sem.PREC_FP64), ("fp32", sem.PREC_MIXED)):
+        shape, rows = sem.autotune(m, sem.OP_SCHRO_RES, prec, 24)
+        tuned[name] = {"shape": list(shape), "candidates": [[r[0], r[1], round(r[2], 2), round(r[3], 2)] for r in rows]}
+        log("autotune %s: chosen %s; (chunks/wg, grid, op us, update us): %s" % (name, shape, tuned[name]["candidates"]))
 
     log("accuracy reference: fp64, tol 1e-14")
     truth = run_mode(sem, m, Vref, re, im, dt_ref, 1e-14, args.steps, args.warmup, sem.PREC_FP64,
@@ -139,10 +149,17 @@ def main():
     launches, total_us, bpl = res[best]["kernel"]
     avg_us = total_us / max(launches, 1)
     achieved = bpl / (avg_us * 1e-6) / 1e9 if launches else 0.0
+    shape = tuned["fp64" if best == "fp64" else "fp32"]["shape"]
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": "k_minres_op<9,%s,RES>" % ("double" if best == "fp64" else "float"),
+                "kernel": "k_minres_op<%d,%s,RES,%d,%s>" % (args.p + 1, "double" if best == "fp64" else "float",
+                                                          shape[0], "true" if shape[1] else "false"),
                 "avg_us": round(avg_us, 2), "launches": launches, "bytes_per_launch": bpl}
+    ul, uus, ubpl = res[best]["kernel_upd"]
+    upd = {"kernel": "k_minres_upd<%d,%s,%d,%s>" % (args.p + 1, "double" if best == "fp64" else "float", shape[2],
+                                                    "true" if shape[3] else "false"),
+           "avg_us": round(uus / max(ul, 1), 2), "launches": ul, "bytes_per_launch": ubpl,
+           "GB/s": round(ubpl / (uus / max(ul, 1)) / 1e3, 1) if ul else None}
 
     copy = {}
     for v in (0, 1, 2, 4):
@@ -208,6 +225,8 @@ def main():
         "rel_l2_vs_tol1e-14": {k: res[k]["err"] for k in res},
         "psi_norm_drift": norm_drift,
         "roofline": roofline,
+        "update_kernel": upd,
+        "step_shape_autotuned": tuned,
         "glaplace": glap,
         "cpu_baseline": cpu,
         "copy_GBs_measured": round(copy_gbs, 1),

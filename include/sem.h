@@ -174,10 +174,13 @@ int sem_pcg_tridiag(long n, double lo, double di, double up, const double* b, do
  * random data (kind, precision 0 = FP64, 1 = FP32); returns mean microseconds
  * per launch and the algorithmic bytes per launch. */
 int sem_time_element_op(sem_mesh* m, int kind, int precision, int reps, double* us_per_launch, double* bytes_per_launch);
-/* Live timing of the MINRES Lanczos-step kernel: when enabled, every launch of
- * it is bracketed by HIP events on the handle's stream. which = 0 (FP64 kernel)
- * or 1 (FP32 kernel); returns launches, summed microseconds and the
- * algorithmic bytes of one launch (Res apply plus the fused w/x update). */
+/* Live timing of the two MINRES Lanczos-step kernels: on = 1 brackets the
+ * launches of every fourth Lanczos step with HIP events on the handle's
+ * stream, on = 2 every step, 0 turns it off. which = 0 / 1:
+ * operator kernel (operator apply, y -= cr1*r1, alpha partial) FP64 / FP32;
+ * 2 / 3: update kernel (interface gather, y -= c2*r2, beta^2 partial, w/x
+ * update) FP64 / FP32. Returns launches, summed microseconds and the
+ * algorithmic bytes of one launch. */
 int sem_set_kernel_timing(sem_mesh* m, int on);
 /* Device-to-device copy of `bytes` bytes (read `bytes`, write `bytes`), `reps`
  * launches: measured copy bandwidth and the calibration kernel for HBM byte
@@ -185,6 +188,21 @@ int sem_set_kernel_timing(sem_mesh* m, int on);
  * nontemporal copy with 8*v workgroups per CU. */
 int sem_time_copy(long bytes, int reps, int variant, double* us_per_launch);
 int sem_kernel_timing(const sem_mesh* m, int which, long* launches, double* total_us, double* bytes_per_launch);
+
+/* ----- work shape of the Lanczos-step kernels (no counterpart in the reference) ----- */
+/* shape[4] = {operator chunks per workgroup (1 or 2), operator grid (0: one
+ * workgroup per work item, > 0: persistent grid of that many workgroups),
+ * update chunks per workgroup, update grid}, per precision (SEM_PREC_FP64 or
+ * SEM_PREC_MIXED for the FP32 inner solver). Results do not depend on it. */
+int sem_set_step_shape(sem_mesh* m, int precision, const int* shape);
+int sem_get_step_shape(const sem_mesh* m, int precision, int* shape);
+/* Autotuner: times each candidate shape on this mesh over `steps` Lanczos
+ * steps of a synthetic solve of operator `kind` (SEM_OP_LAPLACE, _HELMHOLTZ or
+ * _RES) and keeps the fastest shape per kernel. Outputs (optional):
+ * *n_candidates; cand[2c], cand[2c+1] = chunks per workgroup and grid of
+ * candidate c; us[2c], us[2c+1] = operator / update kernel microseconds per
+ * launch. At most 16 candidates. */
+int sem_autotune(sem_mesh* m, int kind, int precision, int steps, int* n_candidates, int* cand, double* us);
 
 /* ----- per-element MLP of the learned stage guess (no counterpart in the
  * reference: its stage guesses are zero, openmp_impl/schrodinger.hpp:222 etc.).

@@ -106,6 +106,9 @@ def lib():
             "sem_set_kernel_timing": (C.c_int, [C.c_void_p, C.c_int]),
             "sem_time_copy": (C.c_int, [C.c_long, C.c_int, C.c_int, _dp]),
             "sem_kernel_timing": (C.c_int, [C.c_void_p, C.c_int, _lp, _dp, _dp]),
+            "sem_set_step_shape": (C.c_int, [C.c_void_p, C.c_int, _ip]),
+            "sem_get_step_shape": (C.c_int, [C.c_void_p, C.c_int, _ip]),
+            "sem_autotune": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, _ip, _ip, _dp]),
             "sem_stepper_create": (C.c_int, [C.c_void_p, C.c_int, _dp, _dp, _dp, C.POINTER(_Opts),
                                              C.POINTER(C.c_void_p)]),
             "sem_stepper_advance": (C.c_int, [C.c_void_p, C.c_double, C.c_long, STEP_CB, C.c_void_p, _ip]),
@@ -437,16 +440,43 @@ def time_copy(nbytes, reps=20, variant=1):
 
 
 def set_kernel_timing(mesh, on):
-    _check(lib().sem_set_kernel_timing(mesh._h, int(bool(on))))
+    """0 off, 1 (or True) every fourth Lanczos step, 2 every step."""
+    _check(lib().sem_set_kernel_timing(mesh._h, int(on)))
 
 
 def kernel_timing(mesh, which=0):
-    """(launches, total microseconds, algorithmic bytes per launch) of the MINRES step kernel."""
+    """(launches, total microseconds, algorithmic bytes per launch) of a Lanczos-step kernel:
+    which = 0/1 operator kernel FP64/FP32, 2/3 update kernel FP64/FP32."""
     n = C.c_long()
     us = C.c_double()
     by = C.c_double()
     _check(lib().sem_kernel_timing(mesh._h, int(which), C.byref(n), C.byref(us), C.byref(by)))
     return n.value, us.value, by.value
+
+
+def set_step_shape(mesh, shape, precision=PREC_FP64):
+    """Work shape (op chunks/workgroup, op grid, update chunks/workgroup, update grid) of the Lanczos step."""
+    arr = (C.c_int * 4)(*[int(v) for v in shape])
+    _check(lib().sem_set_step_shape(mesh._h, int(precision), arr))
+
+
+def get_step_shape(mesh, precision=PREC_FP64):
+    arr = (C.c_int * 4)()
+    _check(lib().sem_get_step_shape(mesh._h, int(precision), arr))
+    return tuple(arr)
+
+
+def autotune(mesh, kind=None, precision=PREC_FP64, steps=24):
+    """Time the candidate Lanczos-step shapes on this mesh and keep the fastest per kernel.
+
+    Returns (chosen shape, [(chunks per workgroup, grid, op us, update us), ...])."""
+    kind = OP_SCHRO_RES if kind is None else kind
+    n = C.c_int()
+    cand = (C.c_int * 32)()
+    us = (C.c_double * 32)()
+    _check(lib().sem_autotune(mesh._h, int(kind), int(precision), int(steps), C.byref(n), cand, us))
+    rows = [(cand[2 * k], cand[2 * k + 1], us[2 * k], us[2 * k + 1]) for k in range(n.value)]
+    return get_step_shape(mesh, precision), rows
 
 
 def stage_solve(mesh, V, alpha, b, z=None, max_iter=10000, tol=1e-10, precision=PREC_FP64):

@@ -64,21 +64,34 @@ struct sem_mesh {
     std::vector<double> D64;
     std::vector<float> D32;
     // scalars and partials
+    // instances (a batched mesh holds several independent meshes of one
+    // degree; a plain mesh is one instance): element and chunk ranges
+    int n_inst = 1;
+    std::vector<int> inst_el_off{0, 0};  // first element of each instance (+ end)
+    std::vector<int> inst_el_count{0};   // real elements (the rest up to the next offset are padding)
+    std::vector<int> h_inst_off;         // chunk ranges
+    int* d_inst_off = nullptr;
+    int4* d_chunk_info = nullptr;
+    std::vector<Results> inst_results;   // per-instance results of the last Krylov solve
+    // scalars: one record per instance
     KrylovScalars* sc = nullptr;
     KrylovScalars* sc_alt = nullptr;  // second state buffer of the Lanczos-step kernels
-    KrylovScalars* h_sc = nullptr;  // pinned
+    KrylovScalars* h_sc = nullptr;    // pinned
+    StepShape shape[2];               // Lanczos-step work shape (FP64, FP32)
     double* partial = nullptr;
     int n_partial = 0;
     std::unordered_map<std::string, DevBuf> pool;
     sem_run_stats stats{};
     int batch = 8;          // Lanczos steps launched between host checks of the stop flag
     int iter_hint[8] = {};  // iterations of the last MINRES solve per hint slot
-    // live timing of the MINRES Lanczos-step kernel (index 0 FP64, 1 FP32)
-    bool timing = false;
+    // live timing of the two Lanczos-step kernels: 0 off, 1 every fourth
+    // Lanczos step, 2 every step
+    int timing = 0;
     std::vector<hipEvent_t> ev;
-    long kt_launches[2] = {0, 0};
-    double kt_us[2] = {0, 0};
-    double kt_bytes[2] = {0, 0};
+    // slots: 0/1 operator kernel FP64/FP32, 2/3 update kernel FP64/FP32
+    long kt_launches[4] = {0, 0, 0, 0};
+    double kt_us[4] = {0, 0, 0, 0};
+    double kt_bytes[4] = {0, 0, 0, 0};
 
     ~sem_mesh()
     {
@@ -96,6 +109,8 @@ struct sem_mesh {
         hipFree(ecorner);
         hipFree(sc);
         hipFree(sc_alt);
+        hipFree(d_inst_off);
+        hipFree(d_chunk_info);
         hipFree(partial);
         if (h_sc) hipHostFree(h_sc);
         if (stream) hipStreamDestroy(stream);
@@ -148,9 +163,10 @@ struct sem_mesh {
     }
     void read_scalars()
     {
-        SEM_HIP(hipMemcpyAsync(h_sc, sc, sizeof(KrylovScalars), hipMemcpyDeviceToHost, stream));
+        SEM_HIP(hipMemcpyAsync(h_sc, sc, n_inst * sizeof(KrylovScalars), hipMemcpyDeviceToHost, stream));
         sync();
     }
+    InstTable it() const { return InstTable{d_chunk_info, d_inst_off, n_inst, L.nchunks}; }
 };
 
 template <>
@@ -317,11 +333,35 @@ static void setup_device(sem_mesh* m)
     const long total = (long)m->n_pairs + m->n_corners + m->n_bnd;
     m->dss_nb = (int)std::max<long>(1, (total + 255) / 256);
 
-    SEM_HIP(hipMalloc(&m->sc, sizeof(KrylovScalars)));
-    SEM_HIP(hipMemset(m->sc, 0, sizeof(KrylovScalars)));
-    SEM_HIP(hipMalloc(&m->sc_alt, sizeof(KrylovScalars)));
-    SEM_HIP(hipMemset(m->sc_alt, 0, sizeof(KrylovScalars)));
-    SEM_HIP(hipHostMalloc(&m->h_sc, sizeof(KrylovScalars), hipHostMallocDefault));
+    // instance table: chunk ranges from the element ranges (each instance
+    // starts on a kChunkAlign-chunk boundary)
+    if (m->n_inst == 1) {
+        m->inst_el_off = {0, hm.n_el};
+        m->inst_el_count = {hm.n_el};
+    }
+    m->h_inst_off.assign(m->n_inst + 1, 0);
+    std::vector<int4> chunk_info(L.nchunks);
+    for (int k = 0; k < m->n_inst; ++k) {
+        const int e0 = m->inst_el_off[k];
+        if (e0 % (L.K * kChunkAlign) != 0) throw std::runtime_error("internal error: instance not chunk aligned");
+        m->h_inst_off[k] = e0 / L.K;
+    }
+    m->h_inst_off[m->n_inst] = L.nchunks;
+    for (int k = 0; k < m->n_inst; ++k) {
+        if (m->h_inst_off[k + 1] <= m->h_inst_off[k]) throw std::runtime_error("internal error: empty instance");
+        for (int c = m->h_inst_off[k]; c < m->h_inst_off[k + 1]; ++c)
+            chunk_info[c] = make_int4(k, m->h_inst_off[k], m->h_inst_off[k + 1], 0);
+    }
+    up(m->d_inst_off, m->h_inst_off);
+    up(m->d_chunk_info, chunk_info);
+    m->inst_results.assign(m->n_inst, Results{});
+
+    const size_t scb = (size_t)m->n_inst * sizeof(KrylovScalars);
+    SEM_HIP(hipMalloc(&m->sc, scb));
+    SEM_HIP(hipMemset(m->sc, 0, scb));
+    SEM_HIP(hipMalloc(&m->sc_alt, scb));
+    SEM_HIP(hipMemset(m->sc_alt, 0, scb));
+    SEM_HIP(hipHostMalloc(&m->h_sc, scb, hipHostMallocDefault));
     m->n_partial = 2 * L.nchunks + m->dss_nb + 1024;
     SEM_HIP(hipMalloc(&m->partial, (size_t)m->n_partial * sizeof(double)));
 }
@@ -390,12 +430,12 @@ static void lincomb(sem_mesh* m, T* out, int mode, std::initializer_list<std::pa
     launch_lincomb<T>(a, m->stream);
 }
 
-// owner-weighted dot into sc->red[slot] (device)
+// owner-weighted dot of every instance into sc[k].red[slot] (device)
 template <typename T>
 static void wdot_dev(sem_mesh* m, const T* a, const T* b, int ncomp, int slot)
 {
     launch_wdot<T>(a, b, m->len(), ncomp, m->L, m->obits, m->partial, m->stream);
-    launch_fin_sum(m->partial, m->L.nchunks, m->sc->red + slot, m->stream);
+    launch_fin_sum_inst(m->partial, m->it(), m->sc, slot, m->stream);
 }
 
 template <typename T>
@@ -409,7 +449,10 @@ static double wdot_host(sem_mesh* m, const T* a, const T* b, int ncomp)
 // ---------------------------------------------------------------- MINRES --
 // minres.hpp:19-107 with IdentityPreconditioner, two fused launches per
 // Lanczos step (the scalar recurrence runs inside them). Vectors are T (FP64 or
-// FP32); every reduction and every recurrence scalar is FP64.
+// FP32); every reduction and every recurrence scalar is FP64. Each instance of
+// a batched mesh runs its own recurrence and stop test; steps continue until
+// every instance has stopped, and the kernels skip stopped instances. Returns
+// instance 0's result; m->inst_results holds all of them.
 template <typename T>
 static Results minres_fused(sem_mesh* m, int kind, const T* cf, double alpha, const T* b, T* x, bool x_zero,
                             const KrylovOpts& o)
@@ -421,6 +464,7 @@ static Results minres_fused(sem_mesh* m, int kind, const T* cf, double alpha, co
     T* W[3] = {m->buf<T>("mr_W0" + tp, n), m->buf<T>("mr_W1" + tp, n), m->buf<T>("mr_W2" + tp, n)};
     T* Yraw = m->buf<T>("mr_Yraw" + tp, n);
     hipStream_t s = m->stream;
+    const int ni = m->n_inst;
     if (x_zero) {
         launch_fill<T>(x, T(0), n, s);
         launch_cast<T, T>(R[0], b, n, s);
@@ -432,26 +476,29 @@ static Results minres_fused(sem_mesh* m, int kind, const T* cf, double alpha, co
     launch_fill<T>(W[0], T(0), n, s);
     launch_fill<T>(W[2], T(0), n, s);
     wdot_dev<T>(m, R[0], R[0], nc, 0);
-    launch_minres_init(m->sc, 0, o.maxit, o.tol, s);
+    launch_minres_init(m->sc, ni, 0, o.maxit, o.tol, s);
     if (o.rel_to_b) {  // b_norm = ||b|| (fair stop test for nonzero guesses)
         wdot_dev<T>(m, b, b, nc, 1);
     }
     if (o.rel_to_b || o.atol != o.tol) {
         // stop test phibar < tol * b_ref + atol (minres.hpp:92 has b_ref = ||r0||, atol = tol)
         m->read_scalars();
-        if (o.rel_to_b) m->h_sc->b_norm = std::sqrt(m->h_sc->red[1]);
-        m->h_sc->atol = o.atol;
-        SEM_HIP(hipMemcpyAsync(m->sc, m->h_sc, sizeof(KrylovScalars), hipMemcpyHostToDevice, s));
+        for (int k = 0; k < ni; ++k) {
+            if (o.rel_to_b) m->h_sc[k].b_norm = std::sqrt(m->h_sc[k].red[1]);
+            m->h_sc[k].atol = o.atol;
+        }
+        SEM_HIP(hipMemcpyAsync(m->sc, m->h_sc, ni * sizeof(KrylovScalars), hipMemcpyHostToDevice, s));
     }
     const int nch = m->L.nchunks;
     const int ti = sizeof(T) == 8 ? 0 : 1;
+    const StepShape shp = m->shape[ti];
     // The first batch runs most of the expected iteration count without a
     // host round trip (the previous solve of this precision predicts it);
     // steps launched after the stop flag is set exit at once.
     const int slot = o.hint_slot >= 0 ? std::min(o.hint_slot, 7) : ti;
     const int first_batch = std::max(m->batch, (int)(0.85 * m->iter_hint[slot]));
     if (m->timing) {
-        while ((int)m->ev.size() < 2 * first_batch) {
+        while ((int)m->ev.size() < 4 * first_batch) {
             hipEvent_t e;
             SEM_HIP(hipEventCreate(&e));
             m->ev.push_back(e);
@@ -461,6 +508,8 @@ static Results minres_fused(sem_mesh* m, int kind, const T* cf, double alpha, co
         const double EN2 = (double)m->L.E * m->L.N * m->L.N;
         const double op_vals = (kind == K_RES) ? 9 : (kind == K_HELMHOLTZ ? 7 : 5);
         m->kt_bytes[ti] = (op_vals + 1.0 * nc) * sizeof(T) * EN2;
+        // update kernel: yraw, r2 in and y out, then r1, wa, wb, x in and wn, x out, per component
+        m->kt_bytes[2 + ti] = 9.0 * nc * sizeof(T) * EN2;
     }
     // Scalar state: the operator kernel of step i > 1 (not first in its batch)
     // applies the beta recurrence of step i-1 in every workgroup and stores
@@ -468,8 +517,18 @@ static Results minres_fused(sem_mesh* m, int kind, const T* cf, double alpha, co
     // that buffer and stores alpha there. A closing kernel ends each batch
     // with the state back in m->sc.
     KrylovScalars* buf[2] = {m->sc, m->sc_alt};
+    const InstTable itab = m->it();
+    auto max_iter = [&]() {
+        int v = 0;
+        for (int k = 0; k < ni; ++k) v = std::max(v, m->h_sc[k].iter);
+        return v;
+    };
+    auto all_done = [&]() {
+        for (int k = 0; k < ni; ++k)
+            if (!m->h_sc[k].done) return false;
+        return true;
+    };
     int i = 1;
-    int nb2 = nch;
     bool done = o.maxit < 1;
     while (!done) {
         int launched = 0;
@@ -488,17 +547,18 @@ static Results minres_fused(sem_mesh* m, int kind, const T* cf, double alpha, co
             a.nchunks = nch;
             a.obits = m->obits;
             a.partial = m->partial;
-            a.st_in = reinterpret_cast<const double*>(buf[cur]);
+            a.it = itab;
+            a.st_in = buf[cur];
             a.first = (b_ == 0);
             if (b_ > 0) {
                 cur ^= 1;
-                a.st_out = reinterpret_cast<double*>(buf[cur]);
+                a.st_out = buf[cur];
                 a.prev_partial = m->partial + nch;
-                a.n_prev = nb2;
             }
-            if (m->timing) SEM_HIP(hipEventRecord(m->ev[2 * b_], s));
-            launch_minres_op_d<T>(m->L.N, kind, a, m->dmat<T>(), s);
-            if (m->timing) SEM_HIP(hipEventRecord(m->ev[2 * b_ + 1], s));
+            const bool tim = m->timing == 2 || (m->timing == 1 && i % 4 == 1);
+            if (tim) SEM_HIP(hipEventRecord(m->ev[4 * b_], s));
+            launch_minres_op_d<T>(m->L.N, kind, a, m->dmat<T>(), shp.op_cpb, shp.op_grid, s);
+            if (tim) SEM_HIP(hipEventRecord(m->ev[4 * b_ + 1], s));
             ++launched;
             MinresUpdArgs<T> u{};
             u.yraw = Yraw;
@@ -518,35 +578,138 @@ static Results minres_fused(sem_mesh* m, int kind, const T* cf, double alpha, co
             u.c_off = m->c_off;
             u.c_pos = m->c_pos;
             u.partial = m->partial + nch;
-            u.st = reinterpret_cast<double*>(buf[cur]);
+            u.it = itab;
+            u.st = buf[cur];
             u.op_partial = m->partial;
-            u.n_op = nch;
-            nb2 = launch_minres_upd_n<T>(m->L.N, u, s);
+            if (tim) SEM_HIP(hipEventRecord(m->ev[4 * b_ + 2], s));
+            launch_minres_upd<T>(m->L.N, u, shp.upd_cpb, shp.upd_grid, s);
+            if (tim) SEM_HIP(hipEventRecord(m->ev[4 * b_ + 3], s));
         }
-        launch_minres_close(kind, buf[cur], m->sc, m->partial + nch, nb2, s);
+        launch_minres_close(kind, itab, buf[cur], m->sc, m->partial + nch, s);
         m->read_scalars();
         if (m->timing) {
             // launches that ran a full Lanczos step (later ones in the batch exit early)
-            for (int b_ = 0; b_ < launched && i0 + b_ <= m->h_sc->iter; ++b_) {
-                float ms = 0;
-                SEM_HIP(hipEventElapsedTime(&ms, m->ev[2 * b_], m->ev[2 * b_ + 1]));
-                m->kt_us[ti] += 1000.0 * ms;
-                m->kt_launches[ti] += 1;
+            const int last = max_iter();
+            for (int b_ = 0; b_ < launched && i0 + b_ <= last; ++b_) {
+                if (m->timing == 1 && (i0 + b_) % 4 != 1) continue;
+                for (int q = 0; q < 2; ++q) {
+                    float ms = 0;
+                    SEM_HIP(hipEventElapsedTime(&ms, m->ev[4 * b_ + 2 * q], m->ev[4 * b_ + 2 * q + 1]));
+                    m->kt_us[2 * q + ti] += 1000.0 * ms;
+                    m->kt_launches[2 * q + ti] += 1;
+                }
             }
         }
-        done = m->h_sc->done != 0 || i > o.maxit + 1;
+        done = all_done() || i > o.maxit + 1;
     }
-    if (m->h_sc->p3_pending) {
-        const int j = m->h_sc->iter;  // last Lanczos step
-        launch_minres_flush<T>(R[(j + 2) % 3], W[(j + 1) % 3], W[(j + 2) % 3], W[j % 3], x, n, m->sc, s);
+    bool pending = false;
+    for (int k = 0; k < ni; ++k) pending |= m->h_sc[k].p3_pending != 0;
+    if (pending) {
+        MinresFlushArgs<T> fa{};
+        for (int q = 0; q < 3; ++q) {
+            fa.R[q] = R[q];
+            fa.W[q] = W[q];
+        }
+        fa.x = x;
+        fa.len = m->len();
+        fa.chunk_len = m->L.K * m->L.N * m->L.N;
+        fa.ncomp = nc;
+        launch_minres_flush<T>(fa, itab, nch, m->sc, s);
     }
-    Results r;
-    r.success = m->h_sc->success;
-    r.n_iter = r.success ? m->h_sc->iter : o.maxit + 1;
-    if (o.maxit < 1) r.n_iter = 1;
-    r.residual = m->h_sc->phibar;
-    m->iter_hint[slot] = m->h_sc->iter;
-    return r;
+    for (int k = 0; k < ni; ++k) {
+        Results& r = m->inst_results[k];
+        const KrylovScalars& h = m->h_sc[k];
+        r.success = h.success;
+        r.n_iter = r.success ? h.iter : o.maxit + 1;
+        if (o.maxit < 1) r.n_iter = 1;
+        r.residual = h.phibar;
+    }
+    m->iter_hint[slot] = max_iter();
+    return m->inst_results[0];
+}
+
+// ------------------------------------------------------------- autotuner --
+// Work shapes tried for the two Lanczos-step kernels: chunks per workgroup
+// (1 or 2) times one workgroup per work item or a persistent grid of 2 or 4
+// workgroups per compute unit. Results do not depend on the shape (partials
+// are per chunk and every reduction has a fixed order), so the choice only
+// changes speed.
+static std::vector<std::pair<int, int>> step_candidates()
+{
+    int dev = 0, ncu = 256;
+    SEM_HIP(hipGetDevice(&dev));
+    SEM_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    return {{1, 0}, {2, 0}, {1, 2 * ncu}, {2, 2 * ncu}, {1, 4 * ncu}, {2, 4 * ncu}};
+}
+
+// Times every candidate on this mesh with `steps` Lanczos steps of a
+// synthetic solve (b = 1 in consistent form, zero guess, no stop) and keeps
+// the fastest shape for each kernel. us[2c], us[2c+1]: operator and update
+// kernel microseconds per launch of candidate c.
+template <typename T>
+static void autotune_steps(sem_mesh* m, int kind, int steps, std::vector<double>& us)
+{
+    const int nc = op_ncomp(kind);
+    const size_t n = (size_t)nc * m->len();
+    const int ti = sizeof(T) == 8 ? 0 : 1;
+    const std::string tp = sizeof(T) == 8 ? "d" : "f";
+    T* b = m->buf<T>("tune_b" + tp, n);
+    T* x = m->buf<T>("tune_x" + tp, n);
+    T* cf = m->buf<T>("tune_cf" + tp, m->len());
+    launch_fill<T>(b, T(1), n, m->stream);
+    dss<T>(m, DSS_ZERO_BND, b, nc);
+    launch_fill<T>(cf, T(0), m->len(), m->stream);
+    const auto cand = step_candidates();
+    us.assign(2 * cand.size(), 0.0);
+    const int timing = m->timing;
+    const StepShape keep = m->shape[ti];
+    const int hint = m->iter_hint[7];
+    long saved_l[4];
+    double saved_u[4];
+    for (int q = 0; q < 4; ++q) {
+        saved_l[q] = m->kt_launches[q];
+        saved_u[q] = m->kt_us[q];
+    }
+    KrylovOpts o;
+    o.maxit = steps;
+    o.tol = 0;
+    o.atol = 0;
+    o.hint_slot = 7;
+    m->timing = 2;
+    for (int rep = 0; rep < 2; ++rep)  // first pass warms up and allocates
+        for (size_t c = 0; c < cand.size(); ++c) {
+            m->shape[ti] = StepShape{cand[c].first, cand[c].second, cand[c].first, cand[c].second};
+            for (int q = 0; q < 4; ++q) {
+                m->kt_launches[q] = 0;
+                m->kt_us[q] = 0;
+            }
+            m->iter_hint[7] = steps;
+            minres_fused<T>(m, kind, cf, 0.5, b, x, true, o);
+            if (rep == 1)
+                for (int q = 0; q < 2; ++q)
+                    us[2 * c + q] = m->kt_us[2 * q + ti] / std::max<long>(1, m->kt_launches[2 * q + ti]);
+        }
+    StepShape best = keep;
+    double b_op = 1e300, b_up = 1e300;
+    for (size_t c = 0; c < cand.size(); ++c) {
+        if (us[2 * c] < b_op) {
+            b_op = us[2 * c];
+            best.op_cpb = cand[c].first;
+            best.op_grid = cand[c].second;
+        }
+        if (us[2 * c + 1] < b_up) {
+            b_up = us[2 * c + 1];
+            best.upd_cpb = cand[c].first;
+            best.upd_grid = cand[c].second;
+        }
+    }
+    m->shape[ti] = best;
+    m->timing = timing;
+    m->iter_hint[7] = hint;
+    for (int q = 0; q < 4; ++q) {
+        m->kt_launches[q] = saved_l[q];
+        m->kt_us[q] = saved_u[q];
+    }
 }
 
 // ------------------------------------------------------------------- PCG --
@@ -1728,9 +1891,9 @@ int sem_time_copy(long bytes, int reps, int variant, double* us_per_launch)
 
 int sem_set_kernel_timing(sem_mesh* m, int on)
 {
-    if (!m) return SEM_EINVAL;
-    m->timing = on != 0;
-    for (int k = 0; k < 2; ++k) {
+    if (!m || on < 0 || on > 2) return SEM_EINVAL;
+    m->timing = on;
+    for (int k = 0; k < 4; ++k) {
         m->kt_launches[k] = 0;
         m->kt_us[k] = 0;
     }
@@ -1739,11 +1902,55 @@ int sem_set_kernel_timing(sem_mesh* m, int on)
 
 int sem_kernel_timing(const sem_mesh* m, int which, long* launches, double* total_us, double* bytes_per_launch)
 {
-    if (!m || which < 0 || which > 1) return SEM_EINVAL;
+    if (!m || which < 0 || which > 3) return SEM_EINVAL;
     if (launches) *launches = m->kt_launches[which];
     if (total_us) *total_us = m->kt_us[which];
     if (bytes_per_launch) *bytes_per_launch = m->kt_bytes[which];
     return SEM_OK;
+}
+
+int sem_set_step_shape(sem_mesh* m, int precision, const int* shape)
+{
+    if (!m || !shape || (precision != SEM_PREC_FP64 && precision != SEM_PREC_MIXED)) return SEM_EINVAL;
+    for (int q = 0; q < 4; q += 2)
+        if ((shape[q] != 1 && shape[q] != 2) || shape[q + 1] < 0) return SEM_EINVAL;
+    m->shape[precision == SEM_PREC_FP64 ? 0 : 1] = StepShape{shape[0], shape[1], shape[2], shape[3]};
+    return SEM_OK;
+}
+
+int sem_get_step_shape(const sem_mesh* m, int precision, int* shape)
+{
+    if (!m || !shape || (precision != SEM_PREC_FP64 && precision != SEM_PREC_MIXED)) return SEM_EINVAL;
+    const StepShape& s = m->shape[precision == SEM_PREC_FP64 ? 0 : 1];
+    shape[0] = s.op_cpb;
+    shape[1] = s.op_grid;
+    shape[2] = s.upd_cpb;
+    shape[3] = s.upd_grid;
+    return SEM_OK;
+}
+
+int sem_autotune(sem_mesh* m, int kind, int precision, int steps, int* n_candidates, int* cand, double* us)
+{
+    if (!m || steps < 1 || (kind != K_LAPLACE && kind != K_HELMHOLTZ && kind != K_RES)) return SEM_EINVAL;
+    if (precision != SEM_PREC_FP64 && precision != SEM_PREC_MIXED) return SEM_EINVAL;
+    return guarded([&] {
+        std::vector<double> t;
+        if (precision == SEM_PREC_FP64) autotune_steps<double>(m, kind, steps, t);
+        else autotune_steps<float>(m, kind, steps, t);
+        const auto c = step_candidates();
+        if (n_candidates) *n_candidates = (int)c.size();
+        for (size_t k = 0; k < c.size(); ++k) {
+            if (cand) {
+                cand[2 * k] = c[k].first;
+                cand[2 * k + 1] = c[k].second;
+            }
+            if (us) {
+                us[2 * k] = t[2 * k];
+                us[2 * k + 1] = t[2 * k + 1];
+            }
+        }
+        return SEM_OK;
+    });
 }
 
 int sem_time_element_op(sem_mesh* m, int kind, int precision, int reps, double* us_per_launch, double* bytes_per_launch)

@@ -228,37 +228,6 @@ __global__ __launch_bounds__(64) void k_element_op(OpArgs<T> a, Dm<N, T> D)
     }
 }
 
-// Deterministic block sum: butterfly inside the wave, waves in index order.
-__device__ __forceinline__ double wave_sum(double v)
-{
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
-}
-
-// Sum of p[0..n) by a whole workgroup of nw waves in a fixed order; every
-// thread returns the same value, and so does every workgroup. Each thread
-// adds p[tid], p[tid + 64 nw], ... in index order; the loads are issued 16 at
-// a time so that a thread waits for memory once per 16 values.
-__device__ __forceinline__ double group_sum(const double* __restrict__ p, int n, double* sh, int nw)
-{
-    const int stride = nw * 64;
-    double s = 0;
-    for (int k = threadIdx.x; k < n; k += 16 * stride) {
-        double v[16];
-#pragma unroll
-        for (int u = 0; u < 16; ++u) v[u] = (k + u * stride < n) ? p[k + u * stride] : 0.0;
-#pragma unroll
-        for (int u = 0; u < 16; ++u) s += v[u];
-    }
-    s = wave_sum(s);
-    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
-    __syncthreads();
-    double t = sh[0];
-    for (int w = 1; w < nw; ++w) t += sh[w];
-    __syncthreads();
-    return t;
-}
 
 // owner test of entry (i, l) of an element with ownership bits (bit s-1:
 // side s owned; bit 4+k: corner k owned)
@@ -287,225 +256,276 @@ __device__ __forceinline__ bool owns(int i, int l, unsigned bits)
 // The FP32 instantiation asks for 5 waves per SIMD (a 128 x 128, degree-8 mesh
 // then fits in one wave round on 256 CUs); the FP64 one keeps the default,
 // where forcing occupancy costs more in register spills than it gains.
-template <int N, typename T, int KIND>
-__global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 5 : 1, 8))) void k_minres_op(MinresOpArgs<T> a, Dm<N, T> D)
+template <int N, typename T, int KIND, int CPB, bool PERSIST>
+__global__ __launch_bounds__(64 * CPB * ((KIND == K_RES) ? 2 : 1))
+__attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 5 : 1, 8))) void k_minres_op(MinresOpArgs<T> a, Dm<N, T> D)
 {
     using S = Shape<N>;
     constexpr int NC = (KIND == K_RES) ? 2 : 1;
-    __shared__ T lds[NC][2][S::LDS];
-    __shared__ double wpart[NC];
-    MinresState sc = *reinterpret_cast<const MinresState*>(a.st_in);
-    if (!a.first) {
-        // the previous step's beta recurrence, evaluated by every workgroup
-        const double s2 = group_sum(a.prev_partial, a.n_prev, wpart, NC);
-        if (!sc.done) minres_beta_update(sc, s2);
-        else sc.p3_pending = 0;  // the update kernel of the stopped step applied it
-        if (blockIdx.x == 0 && threadIdx.x == 0) *reinterpret_cast<MinresState*>(a.st_out) = sc;
-    }
-    const int done = sc.done;
-    const int w = threadIdx.x >> 6;  // output component of this wave
+    constexpr int NW = NC * CPB;
+    __shared__ T lds[NW][2][S::LDS];
+    __shared__ double wpart[NW];
+    __shared__ double red[4];
+    const int wv = threadIdx.x >> 6;
+    const int w = wv % NC;    // output component of this wave
+    const int sub = wv / NC;  // chunk of the work item
     const int lane = threadIdx.x & 63;
     const bool act = lane < S::KN;
     const int el = act ? lane / N : 0;
     const int l = act ? lane % N : 0;
-    const Col<N, T> c{(size_t)blockIdx.x * S::K * N * N + lane, act};
     const size_t len = a.len;
-    double part = 0;
-    if (!done) {
-        const T ib = (T)sc.ib, cr1 = (T)sc.cr1;
-        const int has_r1 = sc.has_r1;
-        const unsigned bits = act ? a.obits[blockIdx.x * S::K + el] : 0u;
-        T ga[N], gb[N], gc[N], qs[N], y[N];
-        // stiffness input: the other component for Res, the only one otherwise
-        c.ld(a.r2 + (NC == 2 ? (1 - w) : 0) * len, qs);
-        c.ld(a.geo, ga);
-        c.ld(a.geo + len, gb);
-        c.ld(a.geo + 2 * len, gc);
+    const int n_items = a.nchunks / CPB;
+    int cur_inst = -1;
+    MinresState sc;
+    auto body = [&](int item) {
+        const int c0 = item * CPB;
+        const int4 ci = chunk_instance(a.it, c0);
+        const int inst = ci.x;
+        if (inst != cur_inst) {
+            cur_inst = inst;
+            sc = *static_cast<const MinresState*>(a.st_in + inst);
+            if (!a.first) {
+                // the previous step's beta recurrence, evaluated by every workgroup
+                const int p0 = ci.y;
+                const double s2 = group_sum(a.prev_partial + p0, ci.z - p0, red, NW);
+                if (!sc.done) minres_beta_update(sc, s2);
+                else sc.p3_pending = 0;  // the update kernel of the stopped step applied it
+                if (c0 == p0 && threadIdx.x == 0) *static_cast<MinresState*>(a.st_out + inst) = sc;
+            }
+        }
+        const int chunk = c0 + sub;
+        const Col<N, T> c{(size_t)chunk * S::K * N * N + lane, act};
+        double part = 0;
+        if (!sc.done) {
+            const T ib = (T)sc.ib, cr1 = (T)sc.cr1;
+            const int has_r1 = sc.has_r1;
+            const unsigned bits = act ? a.obits[chunk * S::K + el] : 0u;
+            T ga[N], gb[N], gc[N], qs[N], y[N];
+            // stiffness input: the other component for Res, the only one otherwise
+            c.ld(a.r2 + (NC == 2 ? (1 - w) : 0) * len, qs);
+            c.ld(a.geo, ga);
+            c.ld(a.geo + len, gb);
+            c.ld(a.geo + 2 * len, gc);
 #pragma unroll
-        for (int i = 0; i < N; ++i) qs[i] = ib * qs[i];
-        stiffness<N>(D, lds[w][0], lds[w][1], el, l, act, qs, ga, gb, gc, y);
-        T qm[N];  // v of the output component (alpha partial, mass term)
-        if constexpr (KIND == K_RES) {
-            T cf[N], gj[N];
-            c.ld(a.r2 + w * len, qm);
-            c.ld(a.cf, cf);
-            c.ld(a.geo + 3 * len, gj);
+            for (int i = 0; i < N; ++i) qs[i] = ib * qs[i];
+            stiffness<N>(D, lds[wv][0], lds[wv][1], el, l, act, qs, ga, gb, gc, y);
+            T qm[N];  // v of the output component (alpha partial, mass term)
+            if constexpr (KIND == K_RES) {
+                T cf[N], gj[N];
+                c.ld(a.r2 + w * len, qm);
+                c.ld(a.cf, cf);
+                c.ld(a.geo + 3 * len, gj);
+#pragma unroll
+                for (int i = 0; i < N; ++i) {
+                    qm[i] = ib * qm[i];
+                    const T t = a.alpha * (y[i] + cf[i] * qs[i]);
+                    y[i] = (w == 0) ? gj[i] * qm[i] + t : t - gj[i] * qm[i];
+                }
+            } else if constexpr (KIND == K_HELMHOLTZ) {
+                T cf[N];
+                c.ld(a.cf, cf);
+#pragma unroll
+                for (int i = 0; i < N; ++i) {
+                    qm[i] = qs[i];
+                    y[i] = cf[i] * qs[i] + y[i];
+                }
+            } else {
+#pragma unroll
+                for (int i = 0; i < N; ++i) qm[i] = qs[i];
+            }
+            T r[N];
+            if (has_r1) c.ld(a.r1 + w * len, r);
 #pragma unroll
             for (int i = 0; i < N; ++i) {
-                qm[i] = ib * qm[i];
-                const T t = a.alpha * (y[i] + cf[i] * qs[i]);
-                y[i] = (w == 0) ? gj[i] * qm[i] + t : t - gj[i] * qm[i];
+                if (has_r1 && owns<N>(i, l, bits)) y[i] = y[i] - cr1 * r[i];
+                part += (double)qm[i] * (double)y[i];
             }
-        } else if constexpr (KIND == K_HELMHOLTZ) {
-            T cf[N];
-            c.ld(a.cf, cf);
-#pragma unroll
-            for (int i = 0; i < N; ++i) {
-                qm[i] = qs[i];
-                y[i] = cf[i] * qs[i] + y[i];
-            }
-        } else {
-#pragma unroll
-            for (int i = 0; i < N; ++i) qm[i] = qs[i];
+            c.st(a.y + w * len, y);
         }
-        T r[N];
-        if (has_r1) c.ld(a.r1 + w * len, r);
-#pragma unroll
-        for (int i = 0; i < N; ++i) {
-            if (has_r1 && owns<N>(i, l, bits)) y[i] = y[i] - cr1 * r[i];
-            part += (double)qm[i] * (double)y[i];
+        part = wave_sum(part);
+        if (lane == 0) wpart[wv] = part;
+        __syncthreads();
+        if (threadIdx.x < CPB) {
+            double t = wpart[threadIdx.x * NC];
+            if constexpr (NC == 2) t += wpart[threadIdx.x * NC + 1];
+            a.partial[c0 + threadIdx.x] = t;
         }
-        c.st(a.y + w * len, y);
-    }
-    part = wave_sum(part);
-    if (lane == 0) wpart[w] = part;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double s = wpart[0];
-        if constexpr (NC == 2) s += wpart[1];
-        a.partial[blockIdx.x] = s;
+        __syncthreads();
+    };
+    if constexpr (PERSIST) {
+        for (int item = blockIdx.x; item < n_items; item += gridDim.x) body(item);
+    } else if ((int)blockIdx.x < n_items) {
+        body(blockIdx.x);
     }
 }
 
 // Closes a batch of Lanczos steps: the beta recurrence of the last step with
-// the same reduction (NC waves, same order) as the operator kernel of the
-// step that would follow, so the scalars do not depend on where batches end.
-// Reads *in, writes *out (single workgroup; in may equal out).
-template <int NC>
-__global__ __launch_bounds__(64 * NC) void k_minres_close(const KrylovScalars* in, KrylovScalars* out,
-                                                           const double* partial, int n)
+// the same reduction as the operator kernel of the step that would follow,
+// so the scalars do not depend on where batches end. One workgroup per
+// instance reads in[k] and writes out[k] (in may equal out). Defined in the
+// FP64 translation unit only.
+#ifndef SEM_OP_FP32
+__global__ __launch_bounds__(256) void k_minres_close(InstTable it, const KrylovScalars* in, KrylovScalars* out,
+                                                      const double* partial)
 {
-    __shared__ double wpart[NC];
-    const double s2 = group_sum(partial, n, wpart, NC);
+    __shared__ double red[4];
+    const int k = blockIdx.x;
+    const int p0 = it.inst_off[k];
+    const double s2 = group_sum(partial + p0, it.inst_off[k + 1] - p0, red, 4);
     if (threadIdx.x == 0) {
-        MinresState sc = *static_cast<const MinresState*>(in);
+        MinresState sc = *static_cast<const MinresState*>(in + k);
         if (!sc.done) minres_beta_update(sc, s2);
         else sc.p3_pending = 0;  // the update kernel of the stopped step applied it
-        *static_cast<MinresState*>(out) = sc;
+        *static_cast<MinresState*>(out + k) = sc;
     }
 }
 
-#ifndef SEM_OP_FP32  // one definition across the FP64 and FP32 translation units
-void launch_minres_close(int kind, const KrylovScalars* in, KrylovScalars* out, const double* partial, int n,
-                         hipStream_t s)
+void launch_minres_close(int kind, const InstTable& it, const KrylovScalars* in, KrylovScalars* out,
+                         const double* partial, hipStream_t s)
 {
-    if (kind == K_RES) hipLaunchKernelGGL(k_minres_close<2>, dim3(1), dim3(128), 0, s, in, out, partial, n);
-    else hipLaunchKernelGGL(k_minres_close<1>, dim3(1), dim3(64), 0, s, in, out, partial, n);
+    (void)kind;
+    hipLaunchKernelGGL(k_minres_close, dim3(it.n_inst), dim3(256), 0, s, it, in, out, partial);
 }
 #endif
 
 
-// Second kernel of a Lanczos step: one workgroup per chunk, 2*NC waves.
-// Waves 0..NC-1 finish component w of y: interface copies gather the
-// unassembled values of all copies of their node (edge: owner + other,
-// global_sum.hpp:28-34; corner: left fold from +0.0 in connected_elements
-// order, global_sum.hpp:54-63), so every copy computes the same sum bit for
-// bit; Dirichlet entries become zero; then y -= c2*r2 (minres.hpp:55) and the
-// beta^2 partial with unique-DOF weights (minres.hpp:63). Waves NC..2NC-1 run
-// the pending w/x update of the previous step on component w - NC
-// (w_new = ig*((ibp*r1 - eo*wa) - dl*wb), x += phi*w_new; minres.hpp:82-86).
-template <int N, typename T>
-__global__ __launch_bounds__(256) void k_minres_upd(MinresUpdArgs<T> a)
+// Second kernel of a Lanczos step: work items of CPB chunks, 2*NC waves per
+// chunk. Waves 0..NC-1 of a chunk finish component w of y: interface copies
+// gather the unassembled values of all copies of their node (edge: owner +
+// other, global_sum.hpp:28-34; corner: left fold from +0.0 in
+// connected_elements order, global_sum.hpp:54-63), so every copy computes the
+// same sum bit for bit; Dirichlet entries become zero; then y -= c2*r2
+// (minres.hpp:55) and the beta^2 partial with unique-DOF weights
+// (minres.hpp:63). Waves NC..2NC-1 run the pending w/x update of the previous
+// step on component w - NC (w_new = ig*((ibp*r1 - eo*wa) - dl*wb),
+// x += phi*w_new; minres.hpp:82-86).
+template <int N, typename T, int CPB, bool PERSIST>
+__global__ __launch_bounds__(256 * CPB) void k_minres_upd(MinresUpdArgs<T> a)
 {
     using S = Shape<N>;
     constexpr int K = S::K, KN = S::KN;
-    __shared__ double wpart[4];
-    const KrylovScalars* sc = reinterpret_cast<const KrylovScalars*>(a.st);
-    const int done = sc->done, p3 = sc->p3_pending;
-    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    __shared__ double wpart[4 * CPB];
+    __shared__ double red[4];
     const int nc = a.ncomp;
-    // alpha = dot(v, y) from the operator kernel's partials (minres.hpp:54),
-    // c2 = alpha / beta (minres.hpp:55); workgroup 0 keeps alpha for the recurrence
-    double c2d = 0;
-    if (!done) {
-        const double alpha = group_sum(a.op_partial, a.n_op, wpart, 2 * nc);
-        c2d = alpha / sc->beta;
-        if (blockIdx.x == 0 && threadIdx.x == 0) reinterpret_cast<KrylovScalars*>(a.st)->alpha = alpha;
-    }
-    const int chunk = blockIdx.x;
+    const int wpc = 2 * nc;  // waves per chunk
+    const int wall = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int sub = wall / wpc, wv = wall % wpc;
     const bool act = lane < KN;
-    const size_t base = (size_t)chunk * K * N * N + lane;
     const size_t len = a.len;
-    double part = 0;
-    if (act && wv >= nc && p3) {
-        const size_t o = (size_t)(wv - nc) * len + base;
-        const T ibp = (T)sc->p3_ibp, ig = (T)sc->p3_ig, eo = (T)sc->p3_eo, dl = (T)sc->p3_dl, phi = (T)sc->p3_phi;
-        T r[N], wa[N], wb[N], x[N];
-#pragma unroll
-        for (int i = 0; i < N; ++i) {
-            r[i] = a.r1[o + (size_t)i * KN];
-            wa[i] = a.wa[o + (size_t)i * KN];
-            wb[i] = a.wb[o + (size_t)i * KN];
-            x[i] = a.x[o + (size_t)i * KN];
-        }
-#pragma unroll
-        for (int i = 0; i < N; ++i) {
-            const T v = ibp * r[i];
-            const T wn = ig * ((v - eo * wa[i]) - dl * wb[i]);
-            a.wn[o + (size_t)i * KN] = wn;
-            a.x[o + (size_t)i * KN] = x[i] + phi * wn;
-        }
-    }
-    if (act && wv < nc && !done) {
-        const int el = lane / N, j = lane % N;
-        const int e = chunk * K + el;
-        const T c2 = (T)c2d;
-        const unsigned bits = a.obits[e];
-        const T* yr = a.yraw + (size_t)wv * len;
-        auto side_pos = [&](int d, int k) -> long {
-            const int e2 = d >> 3, s2 = ((d >> 1) & 3) + 1;
-            const int k2 = (d & 1) ? N - 1 - k : k;
-            int i2, j2;
-            if (s2 == 1) { i2 = k2; j2 = 0; }
-            else if (s2 == 2) { i2 = N - 1; j2 = k2; }
-            else if (s2 == 3) { i2 = k2; j2 = N - 1; }
-            else { i2 = 0; j2 = k2; }
-            return (long)(e2 / K) * K * N * N + (long)i2 * KN + (long)(e2 % K) * N + j2;
-        };
-        T v[N], r[N];
-#pragma unroll
-        for (int i = 0; i < N; ++i) {
-            v[i] = yr[base + (size_t)i * KN];
-            r[i] = a.r2[(size_t)wv * len + base + (size_t)i * KN];
-        }
-        if (j == 0 || j == N - 1) {
-            // column on side 1 (j = 0) or side 3 (j = N-1): planes 1..N-2; corners at planes 0, N-1
-            const int dcol = a.eside[4 * e + (j == 0 ? 0 : 2)];
-            const int c_lo = a.ecorner[4 * e + (j == 0 ? 0 : 3)];
-            const int c_hi = a.ecorner[4 * e + (j == 0 ? 1 : 2)];
-            T pv[N];
-            if (dcol >= 0) {
-#pragma unroll
-                for (int i = 1; i < N - 1; ++i) pv[i] = yr[side_pos(dcol, i)];
+    const int n_items = a.nchunks / CPB;
+    int cur_inst = -1;
+    double c2d = 0;
+    auto body = [&](int item) {
+        const int c0 = item * CPB;
+        const int4 ci = chunk_instance(a.it, c0);
+        const int inst = ci.x;
+        const KrylovScalars* sc = a.st + inst;
+        const int done = sc->done, p3 = sc->p3_pending;
+        if (inst != cur_inst) {
+            cur_inst = inst;
+            // alpha = dot(v, y) from the operator kernel's partials (minres.hpp:54),
+            // c2 = alpha / beta (minres.hpp:55); the instance's first workgroup keeps alpha
+            if (!done) {
+                const int p0 = ci.y;
+                const double alpha = group_sum(a.op_partial + p0, ci.z - p0, red, wpc * CPB);
+                c2d = alpha / sc->beta;
+                if (c0 == p0 && threadIdx.x == 0) a.st[inst].alpha = alpha;
             }
-            T s0 = 0, s1 = 0;
-            if (c_lo >= 0)
-                for (int q = a.c_off[c_lo]; q < a.c_off[c_lo + 1]; ++q) s0 += yr[a.c_pos[q]];
-            if (c_hi >= 0)
-                for (int q = a.c_off[c_hi]; q < a.c_off[c_hi + 1]; ++q) s1 += yr[a.c_pos[q]];
-#pragma unroll
-            for (int i = 1; i < N - 1; ++i) v[i] = (dcol >= 0) ? v[i] + pv[i] : T(0);
-            v[0] = s0;
-            v[N - 1] = s1;
-        } else {
-            // plane 0 is side 4, plane N-1 is side 2
-            const int d4 = a.eside[4 * e + 3], d2 = a.eside[4 * e + 1];
-            const T p0 = (d4 >= 0) ? yr[side_pos(d4, j)] : T(0);
-            const T p1 = (d2 >= 0) ? yr[side_pos(d2, j)] : T(0);
-            v[0] = (d4 >= 0) ? v[0] + p0 : T(0);
-            v[N - 1] = (d2 >= 0) ? v[N - 1] + p1 : T(0);
         }
+        const int chunk = c0 + sub;
+        const size_t base = (size_t)chunk * K * N * N + lane;
+        double part = 0;
+        if (act && wv >= nc && p3) {
+            const size_t o = (size_t)(wv - nc) * len + base;
+            const T ibp = (T)sc->p3_ibp, ig = (T)sc->p3_ig, eo = (T)sc->p3_eo, dl = (T)sc->p3_dl,
+                    phi = (T)sc->p3_phi;
+            T r[N], wa[N], wb[N], x[N];
 #pragma unroll
-        for (int i = 0; i < N; ++i) {
-            const T yv = v[i] - c2 * r[i];
-            a.y[(size_t)wv * len + base + (size_t)i * KN] = yv;
-            if (owns<N>(i, j, bits)) part += (double)yv * (double)yv;
+            for (int i = 0; i < N; ++i) {
+                r[i] = a.r1[o + (size_t)i * KN];
+                wa[i] = a.wa[o + (size_t)i * KN];
+                wb[i] = a.wb[o + (size_t)i * KN];
+                x[i] = a.x[o + (size_t)i * KN];
+            }
+#pragma unroll
+            for (int i = 0; i < N; ++i) {
+                const T v = ibp * r[i];
+                const T wn = ig * ((v - eo * wa[i]) - dl * wb[i]);
+                a.wn[o + (size_t)i * KN] = wn;
+                a.x[o + (size_t)i * KN] = x[i] + phi * wn;
+            }
         }
+        if (act && wv < nc && !done) {
+            const int el = lane / N, j = lane % N;
+            const int e = chunk * K + el;
+            const T c2 = (T)c2d;
+            const unsigned bits = a.obits[e];
+            const T* yr = a.yraw + (size_t)wv * len;
+            auto side_pos = [&](int d, int k) -> long {
+                const int e2 = d >> 3, s2 = ((d >> 1) & 3) + 1;
+                const int k2 = (d & 1) ? N - 1 - k : k;
+                int i2, j2;
+                if (s2 == 1) { i2 = k2; j2 = 0; }
+                else if (s2 == 2) { i2 = N - 1; j2 = k2; }
+                else if (s2 == 3) { i2 = k2; j2 = N - 1; }
+                else { i2 = 0; j2 = k2; }
+                return (long)(e2 / K) * K * N * N + (long)i2 * KN + (long)(e2 % K) * N + j2;
+            };
+            T v[N], r[N];
+#pragma unroll
+            for (int i = 0; i < N; ++i) {
+                v[i] = yr[base + (size_t)i * KN];
+                r[i] = a.r2[(size_t)wv * len + base + (size_t)i * KN];
+            }
+            if (j == 0 || j == N - 1) {
+                // column on side 1 (j = 0) or side 3 (j = N-1): planes 1..N-2; corners at planes 0, N-1
+                const int dcol = a.eside[4 * e + (j == 0 ? 0 : 2)];
+                const int c_lo = a.ecorner[4 * e + (j == 0 ? 0 : 3)];
+                const int c_hi = a.ecorner[4 * e + (j == 0 ? 1 : 2)];
+                T pv[N];
+                if (dcol >= 0) {
+#pragma unroll
+                    for (int i = 1; i < N - 1; ++i) pv[i] = yr[side_pos(dcol, i)];
+                }
+                T s0 = 0, s1 = 0;
+                if (c_lo >= 0)
+                    for (int q = a.c_off[c_lo]; q < a.c_off[c_lo + 1]; ++q) s0 += yr[a.c_pos[q]];
+                if (c_hi >= 0)
+                    for (int q = a.c_off[c_hi]; q < a.c_off[c_hi + 1]; ++q) s1 += yr[a.c_pos[q]];
+#pragma unroll
+                for (int i = 1; i < N - 1; ++i) v[i] = (dcol >= 0) ? v[i] + pv[i] : T(0);
+                v[0] = s0;
+                v[N - 1] = s1;
+            } else {
+                // plane 0 is side 4, plane N-1 is side 2
+                const int d4 = a.eside[4 * e + 3], d2 = a.eside[4 * e + 1];
+                const T p0 = (d4 >= 0) ? yr[side_pos(d4, j)] : T(0);
+                const T p1 = (d2 >= 0) ? yr[side_pos(d2, j)] : T(0);
+                v[0] = (d4 >= 0) ? v[0] + p0 : T(0);
+                v[N - 1] = (d2 >= 0) ? v[N - 1] + p1 : T(0);
+            }
+#pragma unroll
+            for (int i = 0; i < N; ++i) {
+                const T yv = v[i] - c2 * r[i];
+                a.y[(size_t)wv * len + base + (size_t)i * KN] = yv;
+                if (owns<N>(i, j, bits)) part += (double)yv * (double)yv;
+            }
+        }
+        part = wave_sum(part);
+        if (lane == 0) wpart[wall] = part;
+        __syncthreads();
+        if (threadIdx.x < CPB) {
+            double t = wpart[threadIdx.x * wpc];
+            if (nc == 2) t += wpart[threadIdx.x * wpc + 1];
+            a.partial[c0 + threadIdx.x] = t;
+        }
+        __syncthreads();
+    };
+    if constexpr (PERSIST) {
+        for (int item = blockIdx.x; item < n_items; item += gridDim.x) body(item);
+    } else if ((int)blockIdx.x < n_items) {
+        body(blockIdx.x);
     }
-    part = wave_sum(part);
-    if (lane == 0) wpart[wv] = part;
-    __syncthreads();
-    if (threadIdx.x == 0) a.partial[blockIdx.x] = (nc == 2) ? wpart[0] + wpart[1] : wpart[0];
 }
 
 // PCG step (pcg.hpp:70-80, identity preconditioner): p = r (first step) or
@@ -572,16 +592,31 @@ static void element_op_N(int kind, const OpArgs<T>& a, const T* d, hipStream_t s
     }
 }
 
+template <int N, typename T, int CPB, bool P>
+static void minres_op_NC(int kind, const MinresOpArgs<T>& a, const Dm<N, T>& D, int grid, hipStream_t s)
+{
+    const int items = a.nchunks / CPB;
+    dim3 g(P ? std::min(grid, items) : items);
+    dim3 b1(64 * CPB), b2(128 * CPB);
+    switch (kind) {
+    case K_LAPLACE: hipLaunchKernelGGL((k_minres_op<N, T, K_LAPLACE, CPB, P>), g, b1, 0, s, a, D); break;
+    case K_HELMHOLTZ: hipLaunchKernelGGL((k_minres_op<N, T, K_HELMHOLTZ, CPB, P>), g, b1, 0, s, a, D); break;
+    case K_RES: hipLaunchKernelGGL((k_minres_op<N, T, K_RES, CPB, P>), g, b2, 0, s, a, D); break;
+    default: throw std::runtime_error("minres: unsupported operator kind");
+    }
+}
+
+// grid > 0 selects the persistent form (grid-stride over work items)
 template <int N, typename T>
-static void minres_op_N(int kind, const MinresOpArgs<T>& a, const T* d, hipStream_t s)
+static void minres_op_N(int kind, const MinresOpArgs<T>& a, const T* d, int cpb, int grid, hipStream_t s)
 {
     auto D = make_dm<N, T>(d);
-    dim3 g(a.nchunks), b1(64), b2(128);
-    switch (kind) {
-    case K_LAPLACE: hipLaunchKernelGGL((k_minres_op<N, T, K_LAPLACE>), g, b1, 0, s, a, D); break;
-    case K_HELMHOLTZ: hipLaunchKernelGGL((k_minres_op<N, T, K_HELMHOLTZ>), g, b1, 0, s, a, D); break;
-    case K_RES: hipLaunchKernelGGL((k_minres_op<N, T, K_RES>), g, b2, 0, s, a, D); break;
-    default: throw std::runtime_error("minres: unsupported operator kind");
+    if (cpb == 2) {
+        if (grid > 0) minres_op_NC<N, T, 2, true>(kind, a, D, grid, s);
+        else minres_op_NC<N, T, 2, false>(kind, a, D, grid, s);
+    } else {
+        if (grid > 0) minres_op_NC<N, T, 1, true>(kind, a, D, grid, s);
+        else minres_op_NC<N, T, 1, false>(kind, a, D, grid, s);
     }
 }
 
@@ -592,11 +627,24 @@ static void pcg_op_N(const PcgOpArgs<T>& a, int nchunks, const T* d, hipStream_t
     hipLaunchKernelGGL((k_pcg_op<N, T>), dim3(nchunks), dim3(64), 0, s, a, D);
 }
 
-template <int N, typename T>
-static int minres_upd_N(const MinresUpdArgs<T>& a, hipStream_t s)
+template <int N, typename T, int CPB, bool P>
+static void minres_upd_NC(const MinresUpdArgs<T>& a, int grid, hipStream_t s)
 {
-    hipLaunchKernelGGL((k_minres_upd<N, T>), dim3(a.nchunks), dim3(128 * a.ncomp), 0, s, a);
-    return a.nchunks;
+    const int items = a.nchunks / CPB;
+    dim3 g(P ? std::min(grid, items) : items), b(128 * a.ncomp * CPB);
+    hipLaunchKernelGGL((k_minres_upd<N, T, CPB, P>), g, b, 0, s, a);
+}
+
+template <int N, typename T>
+static void minres_upd_N(const MinresUpdArgs<T>& a, int cpb, int grid, hipStream_t s)
+{
+    if (cpb == 2) {
+        if (grid > 0) minres_upd_NC<N, T, 2, true>(a, grid, s);
+        else minres_upd_NC<N, T, 2, false>(a, grid, s);
+    } else {
+        if (grid > 0) minres_upd_NC<N, T, 1, true>(a, grid, s);
+        else minres_upd_NC<N, T, 1, false>(a, grid, s);
+    }
 }
 
 #define SEM_DISPATCH_N(N, CALL)                                   \
@@ -628,21 +676,21 @@ void launch_element_op_d(int N, int kind, const OpArgs<T>& a, const T* d, hipStr
 }
 
 template <typename T>
-void launch_minres_op_d(int N, int kind, const MinresOpArgs<T>& a, const T* d, hipStream_t s)
+void launch_minres_op_d(int N, int kind, const MinresOpArgs<T>& a, const T* d, int cpb, int grid, hipStream_t s)
 {
-#define CALL_MO(n) minres_op_N<n, T>(kind, a, d, s)
+    if (a.nchunks % kChunkAlign != 0) throw std::runtime_error("internal error: chunk count not aligned");
+#define CALL_MO(n) minres_op_N<n, T>(kind, a, d, cpb, grid, s)
     SEM_DISPATCH_N(N, CALL_MO)
 #undef CALL_MO
 }
 
 template <typename T>
-int launch_minres_upd_n(int N, const MinresUpdArgs<T>& a, hipStream_t s)
+void launch_minres_upd(int N, const MinresUpdArgs<T>& a, int cpb, int grid, hipStream_t s)
 {
-    int nb = 0;
-#define CALL_MU(n) nb = minres_upd_N<n, T>(a, s)
+    if (a.nchunks % kChunkAlign != 0) throw std::runtime_error("internal error: chunk count not aligned");
+#define CALL_MU(n) minres_upd_N<n, T>(a, cpb, grid, s)
     SEM_DISPATCH_N(N, CALL_MU)
 #undef CALL_MU
-    return nb;
 }
 
 template <typename T>
@@ -657,13 +705,13 @@ void launch_pcg_op_d(int N, const PcgOpArgs<T>& a, int nchunks, const T* d, hipS
 // not) so the two halves of the kernel set build in parallel.
 #ifndef SEM_OP_FP32
 template void launch_element_op_d<double>(int, int, const OpArgs<double>&, const double*, hipStream_t);
-template void launch_minres_op_d<double>(int, int, const MinresOpArgs<double>&, const double*, hipStream_t);
-template int launch_minres_upd_n<double>(int, const MinresUpdArgs<double>&, hipStream_t);
+template void launch_minres_op_d<double>(int, int, const MinresOpArgs<double>&, const double*, int, int, hipStream_t);
+template void launch_minres_upd<double>(int, const MinresUpdArgs<double>&, int, int, hipStream_t);
 template void launch_pcg_op_d<double>(int, const PcgOpArgs<double>&, int, const double*, hipStream_t);
 #else
 template void launch_element_op_d<float>(int, int, const OpArgs<float>&, const float*, hipStream_t);
-template void launch_minres_op_d<float>(int, int, const MinresOpArgs<float>&, const float*, hipStream_t);
-template int launch_minres_upd_n<float>(int, const MinresUpdArgs<float>&, hipStream_t);
+template void launch_minres_op_d<float>(int, int, const MinresOpArgs<float>&, const float*, int, int, hipStream_t);
+template void launch_minres_upd<float>(int, const MinresUpdArgs<float>&, int, int, hipStream_t);
 #endif
 
 }  // namespace sem

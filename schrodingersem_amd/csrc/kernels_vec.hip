@@ -13,6 +13,7 @@
 // partials in a fixed order. The result depends only on the launch shape.
 #include <cfloat>
 
+#include "krylov_device.hpp"
 #include "sem_internal.hpp"
 #include "sem_kernels.hpp"
 
@@ -292,12 +293,11 @@ int launch_edot(const double* a, const double* b, size_t n, double* partial, hip
     return nb;
 }
 
-// single-block fixed-order sum of n partials into *dst
+// single-block fixed-order sum of n partials (blockDim = 256)
 __device__ __forceinline__ double sum_partials(const double* p, int n)
 {
-    double s = 0;
-    for (int k = threadIdx.x; k < n; k += 256) s += p[k];
-    return block_sum256(s);
+    __shared__ double sh[4];
+    return group_sum(p, n, sh, 4);
 }
 
 __global__ __launch_bounds__(256) void k_fin_sum(const double* partial, int n, double* dst)
@@ -311,11 +311,25 @@ void launch_fin_sum(const double* partial, int n, double* dst, hipStream_t s)
     hipLaunchKernelGGL(k_fin_sum, dim3(1), dim3(256), 0, s, partial, n, dst);
 }
 
+// per-instance sums of per-chunk partials into sc[k].red[slot]
+__global__ __launch_bounds__(256) void k_fin_sum_inst(const double* partial, InstTable it, KrylovScalars* sc, int slot)
+{
+    const int k = blockIdx.x, p0 = it.inst_off[k];
+    const double s = sum_partials(partial + p0, it.inst_off[k + 1] - p0);
+    if (threadIdx.x == 0) sc[k].red[slot] = s;
+}
+
+void launch_fin_sum_inst(const double* partial, const InstTable& it, KrylovScalars* sc, int slot, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_fin_sum_inst, dim3(it.n_inst), dim3(256), 0, s, partial, it, sc, slot);
+}
+
 // --------------------------------------------------------- MINRES scalars --
-// minres.hpp:24-44 given red[slot] = dot(r1, r1)
-__global__ void k_minres_init(KrylovScalars* sc, int slot, int maxit, double tol)
+// minres.hpp:24-44 given red[slot] = dot(r1, r1); one workgroup per instance
+__global__ void k_minres_init(KrylovScalars* scs, int slot, int maxit, double tol)
 {
     if (threadIdx.x != 0) return;
+    KrylovScalars* sc = scs + blockIdx.x;
     const double r = sc->red[slot];
     sc->b_norm = sqrt(r);
     const double beta1 = sqrt(r);
@@ -341,45 +355,52 @@ __global__ void k_minres_init(KrylovScalars* sc, int slot, int maxit, double tol
     sc->done = (maxit < 1) ? 1 : 0;
 }
 
-void launch_minres_init(KrylovScalars* sc, int slot, int maxit, double tol, hipStream_t s)
+void launch_minres_init(KrylovScalars* sc, int n_inst, int slot, int maxit, double tol, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_minres_init, dim3(1), dim3(64), 0, s, sc, slot, maxit, tol);
+    hipLaunchKernelGGL(k_minres_init, dim3(n_inst), dim3(64), 0, s, sc, slot, maxit, tol);
 }
 
-// final pending w/x update after the loop ends (minres.hpp:82-86)
+// Final pending w/x update of every instance whose last step left one
+// (minres.hpp:82-86). The buffers rotate with the step index j = iter of the
+// instance: r1 = R[(j+2)%3], wa = W[(j+1)%3], wb = W[(j+2)%3], wn = W[j%3].
 template <typename T>
-__global__ __launch_bounds__(256) void k_minres_flush(const T* r1, const T* wa, const T* wb, T* wn, T* x, size_t n,
-                                                     const double* scp)
+__global__ __launch_bounds__(256) void k_minres_flush(MinresFlushArgs<T> a, InstTable it, const KrylovScalars* scs)
 {
-    const KrylovScalars* sc = reinterpret_cast<const KrylovScalars*>(scp);
+    const int chunk = blockIdx.x;
+    const KrylovScalars* sc = scs + chunk_instance(it, chunk).x;
     if (!sc->p3_pending) return;
+    const int j = sc->iter;
+    const T* r1 = a.R[(j + 2) % 3];
+    const T* wa = a.W[(j + 1) % 3];
+    const T* wb = a.W[(j + 2) % 3];
+    T* wn = a.W[j % 3];
     const T ibp = (T)sc->p3_ibp, ig = (T)sc->p3_ig, eo = (T)sc->p3_eo, dl = (T)sc->p3_dl, phi = (T)sc->p3_phi;
-    for (size_t k = (size_t)blockIdx.x * 256 + threadIdx.x; k < n; k += (size_t)gridDim.x * 256) {
-        const T v = ibp * r1[k];
-        const T w = ig * ((v - eo * wa[k]) - dl * wb[k]);
-        wn[k] = w;
-        x[k] += phi * w;
-    }
+    for (int c = 0; c < a.ncomp; ++c)
+        for (int q = threadIdx.x; q < a.chunk_len; q += 256) {
+            const size_t k = (size_t)c * a.len + (size_t)chunk * a.chunk_len + q;
+            const T v = ibp * r1[k];
+            const T w = ig * ((v - eo * wa[k]) - dl * wb[k]);
+            wn[k] = w;
+            a.x[k] += phi * w;
+        }
 }
 
-__global__ void k_clear_pending(KrylovScalars* sc)
+__global__ void k_clear_pending(KrylovScalars* scs)
 {
-    if (threadIdx.x == 0) sc->p3_pending = 0;
+    if (threadIdx.x == 0) scs[blockIdx.x].p3_pending = 0;
 }
 
 template <typename T>
-void launch_minres_flush(const T* r1, const T* wa, const T* wb, T* wn, T* x, size_t n, KrylovScalars* sc,
+void launch_minres_flush(const MinresFlushArgs<T>& a, const InstTable& it, int nchunks, KrylovScalars* sc,
                          hipStream_t s)
 {
-    const int nb = (int)std::min<size_t>((n + 255) / 256, 8192);
-    hipLaunchKernelGGL((k_minres_flush<T>), dim3(nb), dim3(256), 0, s, r1, wa, wb, wn, x, n,
-                       reinterpret_cast<const double*>(sc));
-    hipLaunchKernelGGL(k_clear_pending, dim3(1), dim3(64), 0, s, sc);
+    hipLaunchKernelGGL((k_minres_flush<T>), dim3(nchunks), dim3(256), 0, s, a, it, sc);
+    hipLaunchKernelGGL(k_clear_pending, dim3(it.n_inst), dim3(64), 0, s, sc);
 }
-template void launch_minres_flush<double>(const double*, const double*, const double*, double*, double*, size_t,
-                                          KrylovScalars*, hipStream_t);
-template void launch_minres_flush<float>(const float*, const float*, const float*, float*, float*, size_t,
-                                         KrylovScalars*, hipStream_t);
+template void launch_minres_flush<double>(const MinresFlushArgs<double>&, const InstTable&, int, KrylovScalars*,
+                                          hipStream_t);
+template void launch_minres_flush<float>(const MinresFlushArgs<float>&, const InstTable&, int, KrylovScalars*,
+                                         hipStream_t);
 
 // ------------------------------------------------------------ PCG scalars --
 // red[0] = dot(b,b), red[1] = dot(r,r) (pcg.hpp:62-63, :24)

@@ -1,6 +1,6 @@
-// krylov_device.hpp -- the MINRES recurrence on device scalars, shared by the
-// Lanczos-step kernels (every workgroup evaluates it on the same inputs, so
-// every workgroup holds bit-identical scalars).
+// krylov_device.hpp -- device-side pieces shared by the Krylov kernels: the
+// fixed-order sum of per-chunk partials and the MINRES recurrence. Every
+// workgroup that evaluates them on the same inputs gets bit-identical scalars.
 #pragma once
 
 #include <cfloat>
@@ -8,6 +8,65 @@
 #include "sem_internal.hpp"
 
 namespace sem {
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// Group sums of a fixed 256-lane decomposition of p[0..n): virtual lane
+// v = 64 g + l (g = 0..3) adds p[v], p[v + 256], ... in index order, and group
+// g's sum is a butterfly over l. M groups (g0, g0 + gstep, ...) are handled by
+// the calling wave; loads are issued 16 at a time.
+template <int M>
+__device__ __forceinline__ void group_sums(const double* __restrict__ p, int n, int g0, int gstep, double* sh)
+{
+    const int lane = threadIdx.x & 63;
+    constexpr int KB = 16 / M;
+    double s[M];
+#pragma unroll
+    for (int j = 0; j < M; ++j) s[j] = 0;
+    for (int kb = 0; kb * 256 < n; kb += KB) {
+        double v[M][KB];
+#pragma unroll
+        for (int j = 0; j < M; ++j)
+#pragma unroll
+            for (int u = 0; u < KB; ++u) {
+                const int k = (kb + u) * 256 + (g0 + j * gstep) * 64 + lane;
+                v[j][u] = k < n ? p[k] : 0.0;
+            }
+#pragma unroll
+        for (int j = 0; j < M; ++j)
+#pragma unroll
+            for (int u = 0; u < KB; ++u) {
+                const int k = (kb + u) * 256 + (g0 + j * gstep) * 64 + lane;
+                if (k < n) s[j] += v[j][u];
+            }
+    }
+#pragma unroll
+    for (int j = 0; j < M; ++j) {
+        const double t = wave_sum(s[j]);
+        if (lane == 0) sh[g0 + j * gstep] = t;
+    }
+}
+
+// Sum of p[0..n) by a workgroup of nw waves, ((G0 + G1) + G2) + G3 over the
+// group sums above; every thread returns it, and the value does not depend
+// on nw. sh: 4 doubles of LDS. Contains workgroup barriers.
+__device__ __forceinline__ double group_sum(const double* __restrict__ p, int n, double* sh, int nw)
+{
+    const int wv = threadIdx.x >> 6;
+    if (nw == 1) group_sums<4>(p, n, 0, 1, sh);
+    else if (nw == 2) group_sums<2>(p, n, wv, 2, sh);
+    else if (nw == 3) { if (wv == 0) group_sums<2>(p, n, 0, 3, sh); else group_sums<1>(p, n, wv, 1, sh); }
+    else if (wv < 4) group_sums<1>(p, n, wv, 1, sh);
+    __syncthreads();
+    const double t = ((sh[0] + sh[1]) + sh[2]) + sh[3];
+    __syncthreads();
+    return t;
+}
 
 // beta = sqrt(s), s = dot(y, y) of the step just finished, and the Givens
 // update of minres.hpp:62-99 (stop test minres.hpp:92 with atol = tol there).

@@ -26,6 +26,10 @@ namespace sem {
 
 constexpr int kMinN = 2;
 constexpr int kMaxN = 16;  // p <= 15
+// Chunk counts are padded to a multiple of this many chunks, so that a
+// workgroup of up to kChunkAlign chunks never straddles two instances of a
+// batched mesh.
+constexpr int kChunkAlign = 4;
 
 void set_error(const std::string& s);
 
@@ -79,6 +83,7 @@ struct Layout {
         E = E_;
         K = 64 / N;
         nchunks = (E + K - 1) / K;
+        nchunks = (nchunks + kChunkAlign - 1) / kChunkAlign * kChunkAlign;
         len = (size_t)nchunks * K * N * N;
     }
     size_t pos(int e, int i, int j) const
@@ -91,37 +96,6 @@ struct Layout {
 // host-side permutations between the reference layout and the device layout
 void to_device_layout(const Layout& L, const double* ref, double* dev);
 void from_device_layout(const Layout& L, const double* dev, double* ref);
-
-// ---------------------------------------------------------- device mesh --
-struct DssLists {
-    // interior edges: pairs (owner copy, other copy), (N-2) per interior edge
-    int n_pairs = 0;
-    int2* pairs = nullptr;
-    // interior corners: CSR over copies in connected_elements order
-    int n_corners = 0;
-    int* corner_off = nullptr;
-    int* corner_pos = nullptr;
-    // Dirichlet entries: boundary-edge owner sides and every copy of a boundary corner
-    int n_bnd = 0;
-    int* bnd = nullptr;
-    // non-owner copies (for converting to the reference's masked form)
-    int n_nonowner = 0;
-    int* nonowner = nullptr;
-};
-
-struct Workspace;
-
-struct DeviceMesh {
-    Layout L;
-    hipStream_t stream = nullptr;
-    // geometry, premultiplied by w_i w_j: gA, gB, gC, gJ (FP64 and FP32 copies)
-    double* geo = nullptr;  // 4 * L.len
-    float* geo32 = nullptr;
-    uint8_t* obits = nullptr;  // per element: bit s-1 side s owned, bit 4+k corner k owned
-    DssLists dss;
-    std::vector<uint8_t> h_obits;
-    Workspace* ws = nullptr;
-};
 
 // ------------------------------------------------------------ op params --
 enum OpKind { K_LAPLACE = 0, K_HELMHOLTZ = 1, K_MASS = 2, K_RES = 3, K_F = 4, K_L = 5 };
@@ -141,7 +115,43 @@ struct OpArgs {
     int nchunks;
 };
 
+// Device-resident Krylov scalars (FP64), one record per instance.
+// MINRES recurrence state (minres.hpp:28-40 names); the Lanczos-step kernels
+// copy and store only this part of KrylovScalars.
+struct MinresState {
+    double beta, beta_old, tnorm, dbar, eps, eps_old, phibar, cs, sn, b_norm, tol, atol;
+    double alpha, ib, cr1;
+    double p3_ibp, p3_ig, p3_eo, p3_dl, p3_phi;
+    int p3_pending, iter, done, success, maxit, has_r1;
+};
+
+struct KrylovScalars : MinresState {
+    // PCG (pcg.hpp:62-89)
+    double rho, rho_prev, pcg_beta, pcg_alpha, bnorm, resid;
+    int first;
+    int pad_;
+    // generic reduction results
+    double red[8];
+};
+
+// Instances of a batched mesh: instance k owns chunks [inst_off[k], inst_off[k+1])
+// (multiples of kChunkAlign); chunk_info[c] = (instance, first chunk, end chunk, 0)
+// of chunk c's instance. A plain mesh is one instance and skips the lookup.
+struct InstTable {
+    const int4* chunk_info;
+    const int* inst_off;
+    int n_inst;
+    int nchunks;
+};
+
+__device__ __forceinline__ int4 chunk_instance(const InstTable& it, int chunk)
+{
+    return (it.n_inst == 1) ? make_int4(0, 0, it.nchunks, 0) : it.chunk_info[chunk];
+}
+
 // Arguments of the MINRES operator kernel (one Lanczos step, minres.hpp:48-55).
+// Work items are groups of CPB chunks (template parameter of the kernel),
+// visited grid-stride, so any grid size covers the mesh.
 template <typename T>
 struct MinresOpArgs {
     const T* r2;  // R[i-1]: v_i = ib * r2
@@ -153,14 +163,15 @@ struct MinresOpArgs {
     size_t len;
     int nchunks;
     const uint8_t* obits;  // ownership bits per element
-    double* partial;       // per-workgroup alpha partial sums (out)
-    // Krylov scalars: step 1 reads st_in as is; later steps apply the beta
-    // recurrence of the previous step to st_in with the update kernel's
-    // partials (prev_partial, n_prev) and workgroup 0 stores it in st_out
-    const double* st_in;
-    double* st_out;
+    double* partial;       // per-chunk alpha partial sums (out)
+    InstTable it;
+    // Krylov scalars: step 1 of a batch reads st_in as is; later steps apply
+    // the beta recurrence of the previous step to st_in with the update
+    // kernel's per-chunk partials, and the workgroup holding an instance's
+    // first chunk stores the result in st_out
+    const KrylovScalars* st_in;
+    KrylovScalars* st_out;
     const double* prev_partial;
-    int n_prev;
     int first;
 };
 
@@ -184,29 +195,17 @@ struct MinresUpdArgs {
     const int* ecorner;  // per element corner: interior corner index, or -1 on the Dirichlet boundary
     const int* c_off;
     const int* c_pos;
-    double* partial;           // per-workgroup beta^2 partial sums (out)
-    double* st;                // scalars after the previous step (workgroup 0 stores alpha here)
-    const double* op_partial;  // the operator kernel's alpha partials
-    int n_op;
+    double* partial;           // per-chunk beta^2 partial sums (out)
+    InstTable it;
+    KrylovScalars* st;         // scalars after the previous step (alpha is stored here)
+    const double* op_partial;  // the operator kernel's per-chunk alpha partials
 };
 
-// Device-resident Krylov scalars (FP64), written by single-block finalize kernels.
-// MINRES recurrence state (minres.hpp:28-40 names); the Lanczos-step kernels
-// copy and store only this part of KrylovScalars.
-struct MinresState {
-    double beta, beta_old, tnorm, dbar, eps, eps_old, phibar, cs, sn, b_norm, tol, atol;
-    double alpha, ib, cr1;
-    double p3_ibp, p3_ig, p3_eo, p3_dl, p3_phi;
-    int p3_pending, iter, done, success, maxit, has_r1;
-};
-
-struct KrylovScalars : MinresState {
-    // PCG (pcg.hpp:62-89)
-    double rho, rho_prev, pcg_beta, pcg_alpha, bnorm, resid;
-    int first;
-    int pad_;
-    // generic reduction results
-    double red[8];
+// Work-item shape of the two Lanczos-step kernels: CPB chunks per workgroup
+// and the grid size (0: one workgroup per item).
+struct StepShape {
+    int op_cpb = 1, op_grid = 0;
+    int upd_cpb = 1, upd_grid = 0;
 };
 
 // Arguments of the PCG-fused operator kernel (pcg.hpp:70-80).
@@ -226,14 +225,15 @@ struct PcgOpArgs {
 template <typename T>
 void launch_element_op_d(int N, int kind, const OpArgs<T>& a, const T* d, hipStream_t s);
 template <typename T>
-void launch_minres_op_d(int N, int kind, const MinresOpArgs<T>& a, const T* d, hipStream_t s);
+void launch_minres_op_d(int N, int kind, const MinresOpArgs<T>& a, const T* d, int cpb, int grid, hipStream_t s);
 template <typename T>
 void launch_pcg_op_d(int N, const PcgOpArgs<T>& a, int nchunks, const T* d, hipStream_t s);
-// beta recurrence closing a batch of Lanczos steps (reads *in, writes *out)
-void launch_minres_close(int kind, const KrylovScalars* in, KrylovScalars* out, const double* partial, int n,
-                         hipStream_t s);
-// update kernel of a Lanczos step; returns the number of partials written
+// beta recurrence closing a batch of Lanczos steps, one workgroup per instance
+// (reads in[k], writes out[k])
+void launch_minres_close(int kind, const InstTable& it, const KrylovScalars* in, KrylovScalars* out,
+                         const double* partial, hipStream_t s);
+// update kernel of a Lanczos step
 template <typename T>
-int launch_minres_upd_n(int N, const MinresUpdArgs<T>& a, hipStream_t s);
+void launch_minres_upd(int N, const MinresUpdArgs<T>& a, int cpb, int grid, hipStream_t s);
 
 }  // namespace sem

@@ -68,9 +68,22 @@ void launch_wdot(const T* a, const T* b, size_t len, int ncomp, const Layout& L,
 int launch_edot(const double* a, const double* b, size_t n, double* partial, hipStream_t s);
 void launch_fin_sum(const double* partial, int n, double* dst, hipStream_t s);
 
-void launch_minres_init(KrylovScalars* sc, int slot, int maxit, double tol, hipStream_t s);
+void launch_fin_sum_inst(const double* partial, const InstTable& it, KrylovScalars* sc, int slot, hipStream_t s);
+
+// MINRES set-up of every instance from sc[k].red[slot] = dot(r1, r1)
+void launch_minres_init(KrylovScalars* sc, int n_inst, int slot, int maxit, double tol, hipStream_t s);
+
 template <typename T>
-void launch_minres_flush(const T* r1, const T* wa, const T* wb, T* wn, T* x, size_t n, KrylovScalars* sc,
+struct MinresFlushArgs {
+    const T* R[3];
+    T* W[3];
+    T* x;
+    size_t len;     // entries per component
+    int chunk_len;  // entries per chunk and component (K*N*N)
+    int ncomp;
+};
+template <typename T>
+void launch_minres_flush(const MinresFlushArgs<T>& a, const InstTable& it, int nchunks, KrylovScalars* sc,
                          hipStream_t s);
 
 void launch_pcg_init(KrylovScalars* sc, int maxit, double tol, hipStream_t s);

@@ -128,3 +128,30 @@ def test_solver_repeatable(sem):
     r2 = sem.helmholtz(u2, c, F, m, 5000, 1e-12)
     assert r1.n_iter == r2.n_iter and r1.residual == r2.residual
     assert np.array_equal(u1, u2)
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_step_shapes_bit_identical(sem, precision):
+    # every work shape of the Lanczos-step kernels (chunks per workgroup, one
+    # workgroup per item or a persistent grid) gives the same bits
+    xy, q = structured(20, (-2, 2, -2, 2), 0.1, 3)
+    m = sem.Mesh.from_arrays(6, xy, q)
+    rng = np.random.default_rng(11)
+    V = m.field(lambda x, y: x * x + y * y)
+    b = np.concatenate([sem.gproj(rng.standard_normal(m.size), m), sem.gproj(rng.standard_normal(m.size), m)])
+    ref = None
+    for shape in [(1, 0, 1, 0), (2, 0, 2, 0), (1, 3, 2, 5), (2, 64, 1, 7), (1, 1, 1, 1)]:
+        sem.set_step_shape(m, shape, precision)
+        assert sem.get_step_shape(m, precision) == shape
+        z, r = sem.stage_solve(m, V, 2e-3, b, None, 3000, 1e-11, precision)
+        assert r.success
+        if ref is None:
+            ref = (z, r.n_iter, r.residual)
+        else:
+            assert np.array_equal(z, ref[0]), shape
+            assert (r.n_iter, r.residual) == ref[1:], shape
+    chosen, rows = sem.autotune(m, sem.OP_SCHRO_RES, precision, 12)
+    assert len(rows) >= 2 and all(r[2] > 0 and r[3] > 0 for r in rows)
+    assert (chosen[0], chosen[1]) in [(r[0], r[1]) for r in rows]
+    z, r = sem.stage_solve(m, V, 2e-3, b, None, 3000, 1e-11, precision)
+    assert np.array_equal(z, ref[0])
