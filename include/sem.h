@@ -157,6 +157,33 @@ int sem_stepper_stats(const sem_stepper* s, sem_run_stats* out);
 /* learned guess: Adam steps per stage solve while training (default 10) and learning rate (default 3e-3) */
 int sem_stepper_set_learning(sem_stepper* s, int epochs, float lr);
 void sem_stepper_destroy(sem_stepper* s);
+/* Batched instances (no counterpart in the reference, which runs one problem
+ * per call): independent meshes advanced together. Instances of one degree
+ * share one device mesh and one launch per kernel; each keeps its own Krylov
+ * scalars and stop test and computes bit for bit what it computes alone.
+ * Groups of different degree run concurrently on their own streams. The
+ * batch copies what it needs: the instance meshes may be destroyed after
+ * sem_batch_create. */
+typedef struct sem_batch sem_batch;
+int sem_batch_create(int n, sem_mesh* const* meshes, sem_batch** out);
+void sem_batch_destroy(sem_batch* b);
+/* per degree group (arrays sized by the caller, at most n entries):
+ * degree, instances, device elements (ghost padding included) */
+int sem_batch_info(const sem_batch* b, int* n_groups, int* group_p, int* group_size, long* group_elements);
+/* sem_autotune on every group's mesh (operator SEM_OP_RES); shapes: 4 ints per group (optional) */
+int sem_batch_autotune(sem_batch* b, int precision, int steps, int* shapes);
+int sem_batch_set_step_shape(sem_batch* b, int group, int precision, const int* shape);
+/* schrodinger_bdf2 (schrodinger.hpp:22-128) on every instance to time T with
+ * one set of options (FP64; reference or extrapolated guess). re[k], im[k]
+ * (in/out) and V[k] are instance k's element-local arrays. ok[k] (optional):
+ * every solve of instance k converged. A group stops stepping when any of its
+ * instances fails to converge, as the reference stops that instance. */
+int sem_batch_schrodinger_bdf2(sem_batch* b, double* const* re, double* const* im, const double* const* V, double T,
+                               const sem_solver_opts* opts, int* ok);
+/* statistics of the last batch run (steps, stage solves and Lanczos steps summed
+ * over groups) and its wall time in seconds */
+int sem_batch_last_run_stats(const sem_batch* b, sem_run_stats* out, double* seconds);
+
 /* One Schrodinger stage solve Res(alpha, V) z = b (schrodinger.hpp:223 etc.), z in/out. */
 int sem_stage_solve(sem_mesh* m, const double* V, double alpha, const double* b, double* z, int max_iter, double tol,
                     int precision, sem_solver_results* out);

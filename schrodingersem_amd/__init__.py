@@ -116,6 +116,14 @@ def lib():
             "sem_stepper_stats": (C.c_int, [C.c_void_p, C.POINTER(_Stats)]),
             "sem_stepper_destroy": (None, [C.c_void_p]),
             "sem_stepper_set_learning": (C.c_int, [C.c_void_p, C.c_int, C.c_float]),
+            "sem_batch_create": (C.c_int, [C.c_int, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)]),
+            "sem_batch_destroy": (None, [C.c_void_p]),
+            "sem_batch_info": (C.c_int, [C.c_void_p, _ip, _ip, _ip, _lp]),
+            "sem_batch_autotune": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _ip]),
+            "sem_batch_set_step_shape": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _ip]),
+            "sem_batch_schrodinger_bdf2": (C.c_int, [C.c_void_p, C.POINTER(_dp), C.POINTER(_dp), C.POINTER(_dp),
+                                                     C.c_double, C.POINTER(_Opts), _ip]),
+            "sem_batch_last_run_stats": (C.c_int, [C.c_void_p, C.POINTER(_Stats), _dp]),
             "sem_mlp_create": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_ulonglong, C.POINTER(C.c_void_p)]),
             "sem_mlp_destroy": (None, [C.c_void_p]),
             "sem_mlp_info": (C.c_int, [C.c_void_p, _lp]),
@@ -376,6 +384,72 @@ def schrodinger_dirk4s3(psi_real, psi_imag, potential, mesh, T, opts, callback=N
 def schrodinger_dirk4s5(psi_real, psi_imag, potential, mesh, T, opts, callback=None):
     """schrodinger_dirk4s5 (openmp_impl/schrodinger.hpp:138-285)."""
     return _stepper("sem_schrodinger_dirk4s5", psi_real, psi_imag, potential, mesh, T, opts, callback)
+
+
+class Batch:
+    """Independent meshes advanced together (one device mesh and one launch per
+    kernel for each degree; each instance computes what it computes alone)."""
+
+    def __init__(self, meshes):
+        self.meshes = list(meshes)
+        arr = (C.c_void_p * len(self.meshes))(*[m._h.value for m in self.meshes])
+        h = C.c_void_p()
+        _check(lib().sem_batch_create(len(self.meshes), arr, C.byref(h)))
+        self._h = h
+
+    def info(self):
+        """[(degree, instances, device elements)] per degree group."""
+        n = len(self.meshes)
+        ng = C.c_int()
+        gp = (C.c_int * n)()
+        gs = (C.c_int * n)()
+        ge = (C.c_long * n)()
+        _check(lib().sem_batch_info(self._h, C.byref(ng), gp, gs, ge))
+        return [(gp[g], gs[g], ge[g]) for g in range(ng.value)]
+
+    def autotune(self, precision=PREC_FP64, steps=24):
+        """Autotune every group's Lanczos-step shape; returns {degree: shape}."""
+        groups = self.info()
+        sh = (C.c_int * (4 * len(groups)))()
+        _check(lib().sem_batch_autotune(self._h, int(precision), int(steps), sh))
+        return {groups[g][0]: tuple(sh[4 * g:4 * g + 4]) for g in range(len(groups))}
+
+    def set_step_shape(self, group, shape, precision=PREC_FP64):
+        arr = (C.c_int * 4)(*[int(v) for v in shape])
+        _check(lib().sem_batch_set_step_shape(self._h, int(group), int(precision), arr))
+
+    def schrodinger_bdf2(self, psi_real, psi_imag, potentials, T, opts):
+        """schrodinger_bdf2 on every instance; psi arrays updated in place; returns per-instance success."""
+        n = len(self.meshes)
+        for k in range(n):
+            for a in (psi_real[k], psi_imag[k]):
+                if not (isinstance(a, np.ndarray) and a.dtype == np.float64 and a.flags.c_contiguous
+                        and a.size == self.meshes[k].size):
+                    raise ValueError("psi arrays must be contiguous float64 arrays of mesh.size values")
+        V = [_f64(potentials[k], self.meshes[k].size) for k in range(n)]
+        re = (_dp * n)(*[_ptr(a) for a in psi_real])
+        im = (_dp * n)(*[_ptr(a) for a in psi_imag])
+        vv = (_dp * n)(*[_ptr(a) for a in V])
+        ok = (C.c_int * n)()
+        o = opts._c()
+        _check(lib().sem_batch_schrodinger_bdf2(self._h, re, im, vv, float(T), C.byref(o), ok))
+        return [bool(v) for v in ok]
+
+    def last_run_stats(self):
+        s = _Stats()
+        sec = C.c_double()
+        _check(lib().sem_batch_last_run_stats(self._h, C.byref(s), C.byref(sec)))
+        d = {k: getattr(s, k) for k, _ in _Stats._fields_}
+        d["seconds"] = sec.value
+        return d
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value and _lib is not None:
+            _lib.sem_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
 
 
 class Stepper:

@@ -6,8 +6,11 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <exception>
+#include <map>
 #include <memory>
 #include <stdexcept>
+#include <thread>
 #include <unordered_map>
 
 #include "sem_internal.hpp"
@@ -67,12 +70,13 @@ struct sem_mesh {
     // instances (a batched mesh holds several independent meshes of one
     // degree; a plain mesh is one instance): element and chunk ranges
     int n_inst = 1;
-    std::vector<int> inst_el_off{0, 0};  // first element of each instance (+ end)
-    std::vector<int> inst_el_count{0};   // real elements (the rest up to the next offset are padding)
+    std::vector<int> inst_el_off;        // first element of each instance (+ end); empty: one instance
+    std::vector<int> inst_el_count;      // real elements (the rest up to the next offset are padding)
     std::vector<int> h_inst_off;         // chunk ranges
     int* d_inst_off = nullptr;
     int4* d_chunk_info = nullptr;
     std::vector<Results> inst_results;   // per-instance results of the last Krylov solve
+    std::vector<uint8_t> ghost;          // per element: padding element of a batched mesh
     // scalars: one record per instance
     KrylovScalars* sc = nullptr;
     KrylovScalars* sc_alt = nullptr;  // second state buffer of the Lanczos-step kernels
@@ -201,6 +205,7 @@ static void setup_device(sem_mesh* m)
                 for (int i = 0; i < N; ++i) {
                     size_t k = (size_t)e * N * N + i + (size_t)j * N;
                     tmp[k] = (hm.w[i] * hm.w[j]) * (*src[f])[k];
+                    if (!m->ghost.empty() && m->ghost[e]) tmp[k] = 0.0;  // ghosts carry no operator
                 }
         to_device_layout(L, tmp.data(), g.data() + f * L.len);
     }
@@ -335,10 +340,14 @@ static void setup_device(sem_mesh* m)
 
     // instance table: chunk ranges from the element ranges (each instance
     // starts on a kChunkAlign-chunk boundary)
-    if (m->n_inst == 1) {
+    if (m->inst_el_off.empty()) {  // a plain mesh is one instance (a batch sets its own tables)
+        m->n_inst = 1;
         m->inst_el_off = {0, hm.n_el};
         m->inst_el_count = {hm.n_el};
     }
+    if ((int)m->inst_el_off.size() != m->n_inst + 1 || (int)m->inst_el_count.size() != m->n_inst ||
+        m->inst_el_off.back() != hm.n_el)
+        throw std::runtime_error("internal error: instance table does not cover the mesh");
     m->h_inst_off.assign(m->n_inst + 1, 0);
     std::vector<int4> chunk_info(L.nchunks);
     for (int k = 0; k < m->n_inst; ++k) {
@@ -452,7 +461,8 @@ static double wdot_host(sem_mesh* m, const T* a, const T* b, int ncomp)
 // FP32); every reduction and every recurrence scalar is FP64. Each instance of
 // a batched mesh runs its own recurrence and stop test; steps continue until
 // every instance has stopped, and the kernels skip stopped instances. Returns
-// instance 0's result; m->inst_results holds all of them.
+// the aggregate over instances (all succeeded, longest iteration count, largest
+// residual); m->inst_results holds each instance's result.
 template <typename T>
 static Results minres_fused(sem_mesh* m, int kind, const T* cf, double alpha, const T* b, T* x, bool x_zero,
                             const KrylovOpts& o)
@@ -625,7 +635,14 @@ static Results minres_fused(sem_mesh* m, int kind, const T* cf, double alpha, co
         r.residual = h.phibar;
     }
     m->iter_hint[slot] = max_iter();
-    return m->inst_results[0];
+    // aggregate: success of every instance, the longest iteration count, the largest residual
+    Results all = m->inst_results[0];
+    for (int k = 1; k < ni; ++k) {
+        all.success = all.success && m->inst_results[k].success;
+        all.n_iter = std::max(all.n_iter, m->inst_results[k].n_iter);
+        all.residual = std::max(all.residual, m->inst_results[k].residual);
+    }
+    return all;
 }
 
 // ------------------------------------------------------------- autotuner --
@@ -885,6 +902,7 @@ struct Stepper {
     int nps = 0;
     double alpha_bdf = 0;
     sem_run_stats stats{};
+    std::vector<int> inst_ok;  // per instance: every solve so far succeeded
     // learned guess (guess = 2)
     std::unique_ptr<Mlp> mlp;
     double* Vraw = nullptr;
@@ -930,6 +948,8 @@ static Results solve_res(Stepper& st, double alpha, const double* b, double* x, 
         r = solve_mixed(m, K_RES, st.V.c64, st.V.c32, alpha, b, x, zero, ko, 2e-5);
     else
         r = minres_fused<double>(m, K_RES, st.V.c64, alpha, b, x, zero, ko);
+    if (st.inst_ok.size() != (size_t)m->n_inst) st.inst_ok.assign(m->n_inst, 1);
+    for (int k = 0; k < m->n_inst; ++k) st.inst_ok[k] = st.inst_ok[k] && m->inst_results[k].success;
     st.stats.refinements += m->stats.refinements - ref0;
     st.stats.stage_solves++;
     st.stats.total_iters += r.n_iter;
@@ -1086,6 +1106,8 @@ static Stepper* stepper_create(sem_mesh* m, int scheme, const double* re, const 
     if (o->guess < 0 || o->guess > 2) throw std::invalid_argument("schrodinger: unsupported guess mode");
     if (o->guess == 2 && scheme == 0) throw std::invalid_argument("schrodinger: the learned guess serves the SDIRK stages");
     if (o->precision < 0 || o->precision > 1) throw std::invalid_argument("schrodinger: unsupported precision");
+    if (m->n_inst > 1 && (o->precision != SEM_PREC_FP64 || o->guess == 2))
+        throw std::invalid_argument("schrodinger: batched meshes step in FP64 with the reference or extrapolated guess");
     std::unique_ptr<Stepper> st(new Stepper);
     st->m = m;
     st->scheme = scheme;
@@ -1255,6 +1277,118 @@ static void stepper_read(Stepper& st, double* re, double* im)
     m->download(h.data(), st.p, 2);
     if (re) std::copy(h.begin(), h.begin() + rl, re);
     if (im) std::copy(h.begin() + rl, h.end(), im);
+}
+
+// -------------------------------------------------------------- batches --
+// Independent meshes advanced together. Instances of one degree share a union
+// mesh: the instance meshes side by side (element and node ids offset, so
+// edge owners, orientations and corner orders are those of each instance),
+// each padded with ghost elements (a far-away unit square with zero geometry,
+// every side on the Dirichlet boundary) to a whole number of kChunkAlign-chunk
+// work items. Every workgroup of the Lanczos-step kernels then serves one
+// instance, every instance keeps its own Krylov scalars and reductions over
+// its own chunks, and each instance computes bit for bit what it computes
+// alone. Groups of different degree run on their own streams from their own
+// host threads.
+static sem_mesh* make_union_mesh(const std::vector<const sem_mesh*>& parts)
+{
+    const int p = parts[0]->hm.p, N = p + 1, K = 64 / N, align = K * kChunkAlign;
+    double far = 0;
+    for (const sem_mesh* s : parts)
+        for (double v : s->hm.node_xy) far = std::max(far, std::fabs(v));
+    const double X0 = 4.0 * far + 16.0;
+    std::vector<double> xy;
+    std::vector<int> quads, el_off{0}, el_count;
+    std::vector<uint8_t> ghost;
+    int n_ghost = 0;
+    for (const sem_mesh* s : parts) {
+        const HostMesh& h = s->hm;
+        if (h.p != p) throw std::invalid_argument("internal error: mixed degrees in a union mesh");
+        const int base = (int)(xy.size() / 2);
+        xy.insert(xy.end(), h.node_xy.begin(), h.node_xy.end());
+        for (int v : h.quads) quads.push_back(v + base);
+        ghost.insert(ghost.end(), h.n_el, 0);
+        const int E = h.n_el, Ep = (E + align - 1) / align * align;
+        for (int g = E; g < Ep; ++g, ++n_ghost) {
+            const int b = (int)(xy.size() / 2);
+            const double x0 = X0 + 2.0 * n_ghost, y0 = X0;
+            const double c[8] = {x0, y0, x0 + 1, y0, x0 + 1, y0 + 1, x0, y0 + 1};
+            xy.insert(xy.end(), c, c + 8);
+            for (int q = 0; q < 4; ++q) quads.push_back(b + q);
+            ghost.push_back(1);
+        }
+        el_count.push_back(E);
+        el_off.push_back(el_off.back() + Ep);
+    }
+    std::unique_ptr<sem_mesh> m(new sem_mesh);
+    build_host_mesh(m->hm, p, xy.data(), (int)(xy.size() / 2), quads.data(), (int)(quads.size() / 4));
+    m->n_inst = (int)parts.size();
+    m->inst_el_off = el_off;
+    m->inst_el_count = el_count;
+    m->ghost = ghost;
+    setup_device(m.get());
+    return m.release();
+}
+
+}  // namespace sem
+
+struct sem_batch {
+    struct Group {
+        int p = 0;
+        std::vector<int> members;  // batch instance ids, in union order
+        std::unique_ptr<sem_mesh> m;
+    };
+    std::vector<Group> groups;
+    std::vector<int> inst_group, inst_slot;
+    std::vector<long> inst_ref_len;  // E * N * N of each instance
+    sem_run_stats stats{};
+    double seconds = 0;
+};
+
+namespace sem {
+
+// instance k's reference-layout values (ncomp components of E_k*N*N) into the
+// union's reference layout (ghost entries zero), and back
+static void pack_union(const sem_mesh* um, int slot, const double* src, std::vector<double>& dst, int ncomp)
+{
+    const size_t NN = (size_t)um->L.N * um->L.N, rl = (size_t)um->L.E * NN;
+    const size_t e0 = (size_t)um->inst_el_off[slot], ek = (size_t)um->inst_el_count[slot];
+    for (int c = 0; c < ncomp; ++c) std::copy(src + c * ek * NN, src + (c + 1) * ek * NN, dst.begin() + c * rl + e0 * NN);
+}
+
+static void unpack_union(const sem_mesh* um, int slot, const std::vector<double>& src, double* dst, int ncomp)
+{
+    const size_t NN = (size_t)um->L.N * um->L.N, rl = (size_t)um->L.E * NN;
+    const size_t e0 = (size_t)um->inst_el_off[slot], ek = (size_t)um->inst_el_count[slot];
+    for (int c = 0; c < ncomp; ++c)
+        std::copy(src.begin() + c * rl + e0 * NN, src.begin() + c * rl + (e0 + ek) * NN, dst + c * ek * NN);
+}
+
+// schrodinger_bdf2 (schrodinger.hpp:22-128) on every instance of one group
+static void batch_group_bdf2(sem_batch::Group& g, double* const* re, double* const* im, const double* const* V,
+                             double T, const sem_solver_opts* o, int* ok, sem_run_stats& stats)
+{
+    sem_mesh* um = g.m.get();
+    const size_t rl = (size_t)um->L.E * um->L.N * um->L.N;
+    std::vector<double> ur(rl, 0.0), ui(rl, 0.0), uv(rl, 0.0);
+    for (size_t s = 0; s < g.members.size(); ++s) {
+        const int k = g.members[s];
+        pack_union(um, (int)s, re[k], ur, 1);
+        pack_union(um, (int)s, im[k], ui, 1);
+        pack_union(um, (int)s, V[k], uv, 1);
+    }
+    std::unique_ptr<Stepper> st(stepper_create(um, 0, ur.data(), ui.data(), uv.data(), o));
+    st->inst_ok.assign(um->n_inst, 1);
+    const bool all = stepper_advance(*st, T, 0, nullptr, nullptr);
+    (void)all;
+    stepper_read(*st, ur.data(), ui.data());
+    for (size_t s = 0; s < g.members.size(); ++s) {
+        const int k = g.members[s];
+        unpack_union(um, (int)s, ur, re[k], 1);
+        unpack_union(um, (int)s, ui, im[k], 1);
+        if (ok) ok[k] = st->inst_ok[s];
+    }
+    stats = st->stats;
 }
 
 // ------------------------------------------------------- generic solvers --
@@ -1757,6 +1891,118 @@ int sem_stepper_stats(const sem_stepper* h, sem_run_stats* out)
 }
 
 void sem_stepper_destroy(sem_stepper* h) { delete h; }
+
+int sem_batch_create(int n, sem_mesh* const* meshes, sem_batch** out)
+{
+    return guarded([&] {
+        if (!out || !meshes || n < 1) throw std::invalid_argument("sem_batch_create: bad arguments");
+        *out = nullptr;
+        std::unique_ptr<sem_batch> b(new sem_batch);
+        b->inst_group.assign(n, -1);
+        b->inst_slot.assign(n, -1);
+        b->inst_ref_len.assign(n, 0);
+        std::map<int, std::vector<int>> by_p;
+        for (int k = 0; k < n; ++k) {
+            if (!meshes[k]) throw std::invalid_argument("sem_batch_create: null mesh");
+            if (meshes[k]->n_inst != 1) throw std::invalid_argument("sem_batch_create: instances must be plain meshes");
+            by_p[meshes[k]->hm.p].push_back(k);
+            b->inst_ref_len[k] = (long)meshes[k]->L.E * meshes[k]->L.N * meshes[k]->L.N;
+        }
+        for (auto& kv : by_p) {
+            sem_batch::Group g;
+            g.p = kv.first;
+            g.members = kv.second;
+            std::vector<const sem_mesh*> parts;
+            for (int k : g.members) parts.push_back(meshes[k]);
+            g.m.reset(make_union_mesh(parts));
+            for (size_t s = 0; s < g.members.size(); ++s) {
+                b->inst_group[g.members[s]] = (int)b->groups.size();
+                b->inst_slot[g.members[s]] = (int)s;
+            }
+            b->groups.push_back(std::move(g));
+        }
+        *out = b.release();
+        return (int)SEM_OK;
+    });
+}
+
+void sem_batch_destroy(sem_batch* b) { delete b; }
+
+int sem_batch_info(const sem_batch* b, int* n_groups, int* group_p, int* group_size, long* group_elements)
+{
+    if (!b) return SEM_EINVAL;
+    if (n_groups) *n_groups = (int)b->groups.size();
+    for (size_t g = 0; g < b->groups.size(); ++g) {
+        if (group_p) group_p[g] = b->groups[g].p;
+        if (group_size) group_size[g] = (int)b->groups[g].members.size();
+        if (group_elements) group_elements[g] = b->groups[g].m->L.E;
+    }
+    return SEM_OK;
+}
+
+int sem_batch_autotune(sem_batch* b, int precision, int steps, int* shapes)
+{
+    if (!b || steps < 1 || (precision != SEM_PREC_FP64 && precision != SEM_PREC_MIXED)) return SEM_EINVAL;
+    return guarded([&] {
+        for (size_t g = 0; g < b->groups.size(); ++g) {
+            std::vector<double> t;
+            sem_mesh* m = b->groups[g].m.get();
+            if (precision == SEM_PREC_FP64) autotune_steps<double>(m, K_RES, steps, t);
+            else autotune_steps<float>(m, K_RES, steps, t);
+            if (shapes) sem_get_step_shape(m, precision, shapes + 4 * g);
+        }
+        return (int)SEM_OK;
+    });
+}
+
+int sem_batch_set_step_shape(sem_batch* b, int group, int precision, const int* shape)
+{
+    if (!b || group < 0 || group >= (int)b->groups.size()) return SEM_EINVAL;
+    return sem_set_step_shape(b->groups[group].m.get(), precision, shape);
+}
+
+int sem_batch_schrodinger_bdf2(sem_batch* b, double* const* re, double* const* im, const double* const* V, double T,
+                               const sem_solver_opts* opts, int* ok)
+{
+    return guarded([&] {
+        if (!b || !re || !im || !V || !opts) throw std::invalid_argument("sem_batch_schrodinger_bdf2: null argument");
+        const size_t n = b->inst_group.size();
+        for (size_t k = 0; k < n; ++k)
+            if (!re[k] || !im[k] || !V[k]) throw std::invalid_argument("sem_batch_schrodinger_bdf2: null instance array");
+        std::vector<std::exception_ptr> err(b->groups.size());
+        std::vector<sem_run_stats> st(b->groups.size());
+        const auto t0 = std::chrono::steady_clock::now();
+        std::vector<std::thread> th;
+        for (size_t g = 0; g < b->groups.size(); ++g)
+            th.emplace_back([&, g] {
+                try {
+                    batch_group_bdf2(b->groups[g], re, im, V, T, opts, ok, st[g]);
+                } catch (...) {
+                    err[g] = std::current_exception();
+                }
+            });
+        for (auto& t : th) t.join();
+        b->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        for (auto& e : err)
+            if (e) std::rethrow_exception(e);
+        sem_run_stats tot{};
+        for (const auto& x : st) {
+            tot.steps = std::max(tot.steps, x.steps);
+            tot.stage_solves += x.stage_solves;
+            tot.total_iters += x.total_iters;
+        }
+        b->stats = tot;
+        return (int)SEM_OK;
+    });
+}
+
+int sem_batch_last_run_stats(const sem_batch* b, sem_run_stats* out, double* seconds)
+{
+    if (!b || !out) return SEM_EINVAL;
+    *out = b->stats;
+    if (seconds) *seconds = b->seconds;
+    return SEM_OK;
+}
 
 int sem_stage_solve(sem_mesh* m, const double* V, double alpha, const double* b, double* z, int max_iter, double tol,
                     int precision, sem_solver_results* out)
