@@ -68,6 +68,104 @@ def config3(sem, n=128, p=8):
     return m, xy, q, 2.0 * V, re, im
 
 
+def config4(sem, n_inst=64, box=8.0):
+    """64 independent Schrodinger instances, seeds 3+i: n x n quads of [-box, box]^2 with n uniform in
+    [16, 96], degree uniform in {4, 6, 8, 10, 12}, node jitter 0.1 h, V = 0.5|x|^2 plus 1-8 Gaussian
+    bumps, a Gaussian wavepacket with random centre, width and momentum."""
+    inst = []
+    for i in range(n_inst):
+        seed = 3 + i
+        rng = np.random.default_rng(seed)
+        n = int(rng.integers(16, 97))
+        p = int(rng.choice([4, 6, 8, 10, 12]))
+        xy, q = sem.structured_mesh(n, (-box, box, -box, box), 0.1, seed)
+        m = sem.Mesh.from_arrays(p, xy, q)
+        X, Y = m.coords()
+        V = 0.5 * (X * X + Y * Y)
+        for _ in range(int(rng.integers(1, 9))):
+            c = rng.uniform(-box / 2, box / 2, 2)
+            h, w = rng.uniform(-5, 5), rng.uniform(0.3, 1.5)
+            V = V + h * np.exp(-((X - c[0]) ** 2 + (Y - c[1]) ** 2) / (2 * w * w))
+        c = rng.uniform(-2, 2, 2)
+        k = rng.uniform(-2, 2, 2)
+        w = rng.uniform(0.5, 1.5)
+        env = np.exp(-((X - c[0]) ** 2 + (Y - c[1]) ** 2) / (2 * w * w))
+        inst.append(dict(mesh=m, V=V, re=env * np.cos(k[0] * X + k[1] * Y), im=env * np.sin(k[0] * X + k[1] * Y),
+                         n=n, p=p))
+    return inst
+
+
+def bench_batched(sem, args):
+    """Config of 64 uneven BDF2 instances: aggregate DOF*steps/s, batched vs one at a time."""
+    dt, tol = 2e-3, args.batch_tol
+    inst = config4(sem, args.instances)
+    dof = np.array([x["mesh"].dof() for x in inst], dtype=np.float64)
+    log("instances: %d, degrees %s, total dof %d" % (len(inst), sorted({x["p"] for x in inst}), int(dof.sum())))
+    batch = sem.Batch([x["mesh"] for x in inst])
+    groups = batch.info()
+    t0 = time.perf_counter()
+    tuned = batch.autotune(sem.PREC_FP64, 24)
+    log("autotune %.1f s: %s" % (time.perf_counter() - t0, tuned))
+    opts = sem.SolverOpts(max_iter=100000, tol=tol, dt=dt)
+    T = (args.batch_steps - 0.5) * dt
+    batch.start_bdf2([x["re"] for x in inst], [x["im"] for x in inst], [x["V"] for x in inst], opts)
+    tb = 0.0
+    done = 0
+    while done < args.batch_steps:
+        k = min(10, args.batch_steps - done)
+        t0 = time.perf_counter()
+        assert batch.advance(T, k)
+        tb += time.perf_counter() - t0
+        done += k
+        log("  batched: %d steps, %.2f s" % (done, tb))
+    re, im, ok, _ = batch.read()
+    st = batch.last_run_stats()
+    rate_b = dof.sum() * args.batch_steps / tb
+    log("batched: %.3f s, %d steps, all ok %s, Lanczos steps summed over groups %d" %
+        (tb, st["steps"], all(ok), st["total_iters"]))
+    # one at a time, same shapes per degree, same graph mode
+    ts = 0.0
+    max_dev = 0.0
+    seq_steps = args.seq_steps
+    Ts = (seq_steps - 0.5) * dt
+    for k, x in enumerate(inst):
+        m = x["mesh"]
+        sem.set_step_shape(m, tuned[x["p"]], sem.PREC_FP64)
+        sem.set_graph_mode(m, True, 12)
+        a, b = x["re"].copy(), x["im"].copy()
+        t0 = time.perf_counter()
+        assert sem.schrodinger_bdf2(a, b, x["V"], m, Ts, opts)
+        ts += time.perf_counter() - t0
+        if k % 8 == 7:
+            log("  one at a time: %d instances, %.2f s" % (k + 1, ts))
+    rate_s = dof.sum() * seq_steps / ts
+    # bit-identity spot check: batch with seq_steps equals the one-at-a-time result
+    re2 = [x["re"].copy() for x in inst]
+    im2 = [x["im"].copy() for x in inst]
+    batch.schrodinger_bdf2(re2, im2, [x["V"] for x in inst], Ts, opts)
+    same = all(np.array_equal(re2[k], a) and np.array_equal(im2[k], b) for k in [len(inst) - 1])
+    line = {
+        "metric": "aggregate DOF*steps/s, 64 uneven BDF2 instances, batched vs one at a time",
+        "value": round(rate_b, 1),
+        "unit": "DOF*steps/s",
+        "n_gpus": 1,
+        "steps": args.batch_steps,
+        "higher_is_better": True,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "batched_bdf2_64_instances", "n": "U{16..96}", "degrees": [4, 6, 8, 10, 12],
+                   "dt": dt, "tol": tol, "total_dof": int(dof.sum())},
+        "seconds_batched": round(tb, 3),
+        "one_at_a_time": {"DOF*steps/s": round(rate_s, 1), "steps": seq_steps, "seconds": round(ts, 3)},
+        "speedup_batched_vs_one_at_a_time": round(rate_b / rate_s, 3),
+        "groups": [{"degree": g[0], "instances": g[1], "elements": g[2], "shape": list(tuned[g[0]])} for g in groups],
+        "all_converged": bool(all(ok)),
+        "batched_equals_single_run_bitwise": bool(same),
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def run_mode(sem, m, Vref, re, im, dt, tol, steps, warmup, precision, guess):
     opts = sem.SolverOpts(max_iter=100000, tol=tol, dt=dt, precision=precision, guess=guess, learn_steps=warmup)
     st = sem.Stepper(m, 1, re, im, Vref, opts)
@@ -105,6 +203,11 @@ def main():
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-glaplace", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--workload", default="dirk4s3", choices=["dirk4s3", "batched_bdf2"])
+    ap.add_argument("--instances", type=int, default=64)
+    ap.add_argument("--batch-steps", type=int, default=100)
+    ap.add_argument("--seq-steps", type=int, default=5)
+    ap.add_argument("--batch-tol", type=float, default=1e-8)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on one GPU only: the hot path is single-GPU here (multi-GPU sharding is a design "
@@ -115,6 +218,8 @@ def main():
 
     dev = sem.device_info()
     log("device:", dev)
+    if args.workload == "batched_bdf2":
+        return bench_batched(sem, args)
     m, xy, q, Vref, re, im = config3(sem, args.n, args.p)
     dt_ref = 1e-3 / 2.0
     log("mesh: E=%d N=%d dof=%d" % (m.n_elements, m.N, m.dof()))

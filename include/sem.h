@@ -173,6 +173,8 @@ int sem_batch_info(const sem_batch* b, int* n_groups, int* group_p, int* group_s
 /* sem_autotune on every group's mesh (operator SEM_OP_RES); shapes: 4 ints per group (optional) */
 int sem_batch_autotune(sem_batch* b, int precision, int steps, int* shapes);
 int sem_batch_set_step_shape(sem_batch* b, int group, int precision, const int* shape);
+/* sem_set_graph_mode on every group's mesh */
+int sem_batch_set_graph_mode(sem_batch* b, int on, int steps);
 /* schrodinger_bdf2 (schrodinger.hpp:22-128) on every instance to time T with
  * one set of options (FP64; reference or extrapolated guess). re[k], im[k]
  * (in/out) and V[k] are instance k's element-local arrays. ok[k] (optional):
@@ -180,6 +182,13 @@ int sem_batch_set_step_shape(sem_batch* b, int group, int precision, const int* 
  * instances fails to converge, as the reference stops that instance. */
 int sem_batch_schrodinger_bdf2(sem_batch* b, double* const* re, double* const* im, const double* const* V, double T,
                                const sem_solver_opts* opts, int* ok);
+/* The same as a session kept on the GPU: start, advance (the while (t < T)
+ * loop, also stopping after max_steps steps when max_steps > 0; success: every
+ * group converged), read (psi, per-instance success, time). */
+int sem_batch_start_bdf2(sem_batch* b, const double* const* re, const double* const* im, const double* const* V,
+                         const sem_solver_opts* opts);
+int sem_batch_advance(sem_batch* b, double T, long max_steps, int* success);
+int sem_batch_read(sem_batch* b, double* const* re, double* const* im, int* ok, double* t);
 /* statistics of the last batch run (steps, stage solves and Lanczos steps summed
  * over groups) and its wall time in seconds */
 int sem_batch_last_run_stats(const sem_batch* b, sem_run_stats* out, double* seconds);
@@ -223,6 +232,12 @@ int sem_kernel_timing(const sem_mesh* m, int which, long* launches, double* tota
  * SEM_PREC_MIXED for the FP32 inner solver). Results do not depend on it. */
 int sem_set_step_shape(sem_mesh* m, int precision, const int* shape);
 int sem_get_step_shape(const sem_mesh* m, int precision, int* shape);
+/* Graph mode: with on = 1 the MINRES solver captures `steps` Lanczos steps
+ * (a multiple of 3) plus the batch-closing kernel as one HIP graph per solve
+ * configuration and replays it between host checks of the stop flags
+ * (results are unchanged; kernel timing, when on, uses the direct launches).
+ * Off by default for a plain mesh, on for the meshes of a batch. */
+int sem_set_graph_mode(sem_mesh* m, int on, int steps);
 /* Autotuner: times each candidate shape on this mesh over `steps` Lanczos
  * steps of a synthetic solve of operator `kind` (SEM_OP_LAPLACE, _HELMHOLTZ or
  * _RES) and keeps the fastest shape per kernel. Outputs (optional):

@@ -108,6 +108,8 @@ def lib():
             "sem_kernel_timing": (C.c_int, [C.c_void_p, C.c_int, _lp, _dp, _dp]),
             "sem_set_step_shape": (C.c_int, [C.c_void_p, C.c_int, _ip]),
             "sem_get_step_shape": (C.c_int, [C.c_void_p, C.c_int, _ip]),
+            "sem_set_graph_mode": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
+            "sem_batch_set_graph_mode": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
             "sem_autotune": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, _ip, _ip, _dp]),
             "sem_stepper_create": (C.c_int, [C.c_void_p, C.c_int, _dp, _dp, _dp, C.POINTER(_Opts),
                                              C.POINTER(C.c_void_p)]),
@@ -124,6 +126,10 @@ def lib():
             "sem_batch_schrodinger_bdf2": (C.c_int, [C.c_void_p, C.POINTER(_dp), C.POINTER(_dp), C.POINTER(_dp),
                                                      C.c_double, C.POINTER(_Opts), _ip]),
             "sem_batch_last_run_stats": (C.c_int, [C.c_void_p, C.POINTER(_Stats), _dp]),
+            "sem_batch_start_bdf2": (C.c_int, [C.c_void_p, C.POINTER(_dp), C.POINTER(_dp), C.POINTER(_dp),
+                                               C.POINTER(_Opts)]),
+            "sem_batch_advance": (C.c_int, [C.c_void_p, C.c_double, C.c_long, _ip]),
+            "sem_batch_read": (C.c_int, [C.c_void_p, C.POINTER(_dp), C.POINTER(_dp), _ip, _dp]),
             "sem_mlp_create": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_ulonglong, C.POINTER(C.c_void_p)]),
             "sem_mlp_destroy": (None, [C.c_void_p]),
             "sem_mlp_info": (C.c_int, [C.c_void_p, _lp]),
@@ -418,6 +424,9 @@ class Batch:
         arr = (C.c_int * 4)(*[int(v) for v in shape])
         _check(lib().sem_batch_set_step_shape(self._h, int(group), int(precision), arr))
 
+    def set_graph_mode(self, on, steps=12):
+        _check(lib().sem_batch_set_graph_mode(self._h, int(bool(on)), int(steps)))
+
     def schrodinger_bdf2(self, psi_real, psi_imag, potentials, T, opts):
         """schrodinger_bdf2 on every instance; psi arrays updated in place; returns per-instance success."""
         n = len(self.meshes)
@@ -434,6 +443,30 @@ class Batch:
         o = opts._c()
         _check(lib().sem_batch_schrodinger_bdf2(self._h, re, im, vv, float(T), C.byref(o), ok))
         return [bool(v) for v in ok]
+
+    def start_bdf2(self, psi_real, psi_imag, potentials, opts):
+        """Start a BDF2 session on every instance (state stays on the GPU)."""
+        n = len(self.meshes)
+        arrs = [[_f64(a[k], self.meshes[k].size) for k in range(n)] for a in (psi_real, psi_imag, potentials)]
+        ptrs = [(_dp * n)(*[_ptr(a) for a in group]) for group in arrs]
+        o = opts._c()
+        _check(lib().sem_batch_start_bdf2(self._h, ptrs[0], ptrs[1], ptrs[2], C.byref(o)))
+
+    def advance(self, T, max_steps=0):
+        ok = C.c_int()
+        _check(lib().sem_batch_advance(self._h, float(T), int(max_steps), C.byref(ok)))
+        return bool(ok.value)
+
+    def read(self):
+        """(psi_real list, psi_imag list, per-instance success, t) of the session."""
+        n = len(self.meshes)
+        re = [np.zeros(m.size) for m in self.meshes]
+        im = [np.zeros(m.size) for m in self.meshes]
+        ok = (C.c_int * n)()
+        t = C.c_double()
+        _check(lib().sem_batch_read(self._h, (_dp * n)(*[_ptr(a) for a in re]), (_dp * n)(*[_ptr(a) for a in im]),
+                                    ok, C.byref(t)))
+        return re, im, [bool(v) for v in ok], t.value
 
     def last_run_stats(self):
         s = _Stats()
@@ -532,6 +565,11 @@ def set_step_shape(mesh, shape, precision=PREC_FP64):
     """Work shape (op chunks/workgroup, op grid, update chunks/workgroup, update grid) of the Lanczos step."""
     arr = (C.c_int * 4)(*[int(v) for v in shape])
     _check(lib().sem_set_step_shape(mesh._h, int(precision), arr))
+
+
+def set_graph_mode(mesh, on, steps=12):
+    """Replay batches of `steps` Lanczos steps (multiple of 3) as captured HIP graphs."""
+    _check(lib().sem_set_graph_mode(mesh._h, int(bool(on)), int(steps)))
 
 
 def get_step_shape(mesh, precision=PREC_FP64):

@@ -2,6 +2,7 @@
 // libsem_hip.so. Host code: it sequences the kernels of kernels_op.hip and
 // kernels_vec.hip on the handle's stream; all vector work stays on the GPU.
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -32,6 +33,21 @@ struct Results {
     int success = 0;
     int n_iter = 0;
     double residual = 0;
+};
+
+// Identity of a captured batch of Lanczos steps: every pointer and value the
+// captured launches bake in besides the mesh's own arrays.
+struct GraphKey {
+    std::array<const void*, 9> ptr{};
+    int kind = 0, tsize = 0, steps = 0;
+    double alpha = 0;
+    StepShape shape;
+    bool operator==(const GraphKey& o) const
+    {
+        return ptr == o.ptr && kind == o.kind && tsize == o.tsize && steps == o.steps && alpha == o.alpha &&
+               shape.op_cpb == o.shape.op_cpb && shape.op_grid == o.shape.op_grid &&
+               shape.upd_cpb == o.shape.upd_cpb && shape.upd_grid == o.shape.upd_grid;
+    }
 };
 
 struct KrylovOpts {
@@ -82,6 +98,38 @@ struct sem_mesh {
     KrylovScalars* sc_alt = nullptr;  // second state buffer of the Lanczos-step kernels
     KrylovScalars* h_sc = nullptr;    // pinned
     StepShape shape[2];               // Lanczos-step work shape (FP64, FP32)
+    // captured Lanczos-step batches (graph path), most recent last
+    bool use_graphs = false;
+    int graph_steps = 12;
+    struct CachedGraph {
+        GraphKey key;
+        hipGraph_t graph;
+        hipGraphExec_t exec;
+    };
+    std::vector<CachedGraph> graphs;
+    hipGraphExec_t find_graph(const GraphKey& k) const
+    {
+        for (const auto& g : graphs)
+            if (g.key == k) return g.exec;
+        return nullptr;
+    }
+    void add_graph(const GraphKey& k, hipGraph_t g, hipGraphExec_t e)
+    {
+        if (graphs.size() >= 16) {  // keep the cache small: drop the oldest
+            hipGraphExecDestroy(graphs.front().exec);
+            hipGraphDestroy(graphs.front().graph);
+            graphs.erase(graphs.begin());
+        }
+        graphs.push_back(CachedGraph{k, g, e});
+    }
+    void clear_graphs()
+    {
+        for (auto& g : graphs) {
+            hipGraphExecDestroy(g.exec);
+            hipGraphDestroy(g.graph);
+        }
+        graphs.clear();
+    }
     double* partial = nullptr;
     int n_partial = 0;
     std::unordered_map<std::string, DevBuf> pool;
@@ -100,6 +148,7 @@ struct sem_mesh {
     ~sem_mesh()
     {
         if (stream) hipStreamSynchronize(stream);
+        clear_graphs();
         for (hipEvent_t e : ev) hipEventDestroy(e);
         for (auto& kv : pool) hipFree(kv.second.p);
         hipFree(geo);
@@ -128,6 +177,7 @@ struct sem_mesh {
         if (it != pool.end() && it->second.bytes >= bytes) return (T*)it->second.p;
         if (it != pool.end()) {
             SEM_HIP(hipStreamSynchronize(stream));
+            clear_graphs();  // captured launches may name the old buffer
             hipFree(it->second.p);
             pool.erase(it);
         }
@@ -538,62 +588,102 @@ static Results minres_fused(sem_mesh* m, int kind, const T* cf, double alpha, co
             if (!m->h_sc[k].done) return false;
         return true;
     };
+    // launches of Lanczos step i, the b_-th of its batch (state buffer cur)
+    auto launch_step = [&](int i, int b_, int& cur, bool tim) {
+        MinresOpArgs<T> a{};
+        a.r2 = R[(i - 1) % 3];
+        a.r1 = R[(i + 1) % 3];
+        a.y = Yraw;
+        a.geo = m->geo_t<T>();
+        a.cf = cf;
+        a.alpha = (T)alpha;
+        a.len = m->len();
+        a.nchunks = nch;
+        a.obits = m->obits;
+        a.partial = m->partial;
+        a.it = itab;
+        a.st_in = buf[cur];
+        a.first = (b_ == 0);
+        if (b_ > 0) {
+            cur ^= 1;
+            a.st_out = buf[cur];
+            a.prev_partial = m->partial + nch;
+        }
+        if (tim) SEM_HIP(hipEventRecord(m->ev[4 * b_], s));
+        launch_minres_op_d<T>(m->L.N, kind, a, m->dmat<T>(), shp.op_cpb, shp.op_grid, s);
+        if (tim) SEM_HIP(hipEventRecord(m->ev[4 * b_ + 1], s));
+        MinresUpdArgs<T> u{};
+        u.yraw = Yraw;
+        u.y = R[i % 3];
+        u.r2 = R[(i - 1) % 3];
+        u.r1 = R[(i + 1) % 3];
+        u.wa = W[i % 3];
+        u.wb = W[(i + 1) % 3];
+        u.wn = W[(i + 2) % 3];
+        u.x = x;
+        u.len = m->len();
+        u.ncomp = nc;
+        u.nchunks = nch;
+        u.obits = m->obits;
+        u.eside = m->eside;
+        u.ecorner = m->ecorner;
+        u.c_off = m->c_off;
+        u.c_pos = m->c_pos;
+        u.partial = m->partial + nch;
+        u.it = itab;
+        u.st = buf[cur];
+        u.op_partial = m->partial;
+        if (tim) SEM_HIP(hipEventRecord(m->ev[4 * b_ + 2], s));
+        launch_minres_upd<T>(m->L.N, u, shp.upd_cpb, shp.upd_grid, s);
+        if (tim) SEM_HIP(hipEventRecord(m->ev[4 * b_ + 3], s));
+    };
     int i = 1;
     bool done = o.maxit < 1;
+    if (m->use_graphs && !m->timing && !done) {
+        // Graph path: one captured batch of G Lanczos steps plus the closing
+        // kernel, replayed until every instance has stopped. G is a multiple
+        // of 3, so every batch starts at a step i = 1 (mod 3) and sees the
+        // same buffers; the first host check comes after about the expected
+        // iteration count. Steps past an instance's stop exit at once, so the
+        // results equal the eager path's.
+        const int G = m->graph_steps;
+        GraphKey key{};
+        key.ptr = {(const void*)R[0], (const void*)R[1], (const void*)R[2], (const void*)W[0], (const void*)W[1],
+                   (const void*)W[2], (const void*)Yraw, (const void*)x, (const void*)cf};
+        key.kind = kind;
+        key.tsize = (int)sizeof(T);
+        key.alpha = alpha;
+        key.steps = G;
+        key.shape = shp;
+        hipGraphExec_t ex = m->find_graph(key);
+        if (!ex) {
+            hipGraph_t gr = nullptr;
+            SEM_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            int cur = 0;
+            for (int b_ = 0; b_ < G; ++b_) launch_step(1 + b_, b_, cur, false);
+            launch_minres_close(kind, itab, buf[cur], m->sc, m->partial + nch, s);
+            SEM_HIP(hipStreamEndCapture(s, &gr));
+            SEM_HIP(hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0));
+            m->add_graph(key, gr, ex);
+        }
+        int reps = std::max(1, (first_batch + G - 1) / G);
+        while (!done) {
+            for (int r = 0; r < reps; ++r) SEM_HIP(hipGraphLaunch(ex, s));
+            i += reps * G;
+            m->read_scalars();
+            done = all_done() || i > o.maxit + 1;
+            reps = 1;
+        }
+    }
     while (!done) {
         int launched = 0;
         const int i0 = i;
         int cur = 0;
         const int nbatch = (i == 1) ? first_batch : m->batch;
         for (int b_ = 0; b_ < nbatch && i <= o.maxit + 1; ++b_, ++i) {
-            MinresOpArgs<T> a{};
-            a.r2 = R[(i - 1) % 3];
-            a.r1 = R[(i + 1) % 3];
-            a.y = Yraw;
-            a.geo = m->geo_t<T>();
-            a.cf = cf;
-            a.alpha = (T)alpha;
-            a.len = m->len();
-            a.nchunks = nch;
-            a.obits = m->obits;
-            a.partial = m->partial;
-            a.it = itab;
-            a.st_in = buf[cur];
-            a.first = (b_ == 0);
-            if (b_ > 0) {
-                cur ^= 1;
-                a.st_out = buf[cur];
-                a.prev_partial = m->partial + nch;
-            }
             const bool tim = m->timing == 2 || (m->timing == 1 && i % 4 == 1);
-            if (tim) SEM_HIP(hipEventRecord(m->ev[4 * b_], s));
-            launch_minres_op_d<T>(m->L.N, kind, a, m->dmat<T>(), shp.op_cpb, shp.op_grid, s);
-            if (tim) SEM_HIP(hipEventRecord(m->ev[4 * b_ + 1], s));
+            launch_step(i, b_, cur, tim);
             ++launched;
-            MinresUpdArgs<T> u{};
-            u.yraw = Yraw;
-            u.y = R[i % 3];
-            u.r2 = R[(i - 1) % 3];
-            u.r1 = R[(i + 1) % 3];
-            u.wa = W[i % 3];
-            u.wb = W[(i + 1) % 3];
-            u.wn = W[(i + 2) % 3];
-            u.x = x;
-            u.len = m->len();
-            u.ncomp = nc;
-            u.nchunks = nch;
-            u.obits = m->obits;
-            u.eside = m->eside;
-            u.ecorner = m->ecorner;
-            u.c_off = m->c_off;
-            u.c_pos = m->c_pos;
-            u.partial = m->partial + nch;
-            u.it = itab;
-            u.st = buf[cur];
-            u.op_partial = m->partial;
-            if (tim) SEM_HIP(hipEventRecord(m->ev[4 * b_ + 2], s));
-            launch_minres_upd<T>(m->L.N, u, shp.upd_cpb, shp.upd_grid, s);
-            if (tim) SEM_HIP(hipEventRecord(m->ev[4 * b_ + 3], s));
         }
         launch_minres_close(kind, itab, buf[cur], m->sc, m->partial + nch, s);
         m->read_scalars();
@@ -1326,6 +1416,7 @@ static sem_mesh* make_union_mesh(const std::vector<const sem_mesh*>& parts)
     m->inst_el_off = el_off;
     m->inst_el_count = el_count;
     m->ghost = ghost;
+    m->use_graphs = true;  // the Lanczos-step batches of a batched mesh replay as graphs
     setup_device(m.get());
     return m.release();
 }
@@ -1341,8 +1432,13 @@ struct sem_batch {
     std::vector<Group> groups;
     std::vector<int> inst_group, inst_slot;
     std::vector<long> inst_ref_len;  // E * N * N of each instance
+    std::vector<std::unique_ptr<sem::Stepper>> session;  // one BDF2 session per group
     sem_run_stats stats{};
     double seconds = 0;
+    ~sem_batch()
+    {
+        session.clear();  // before the group meshes
+    }
 };
 
 namespace sem {
@@ -1364,9 +1460,9 @@ static void unpack_union(const sem_mesh* um, int slot, const std::vector<double>
         std::copy(src.begin() + c * rl + e0 * NN, src.begin() + c * rl + (e0 + ek) * NN, dst + c * ek * NN);
 }
 
-// schrodinger_bdf2 (schrodinger.hpp:22-128) on every instance of one group
-static void batch_group_bdf2(sem_batch::Group& g, double* const* re, double* const* im, const double* const* V,
-                             double T, const sem_solver_opts* o, int* ok, sem_run_stats& stats)
+// schrodinger_bdf2 (schrodinger.hpp:22-128) session on every instance of one group
+static Stepper* batch_group_start(sem_batch::Group& g, const double* const* re, const double* const* im,
+                                  const double* const* V, const sem_solver_opts* o)
 {
     sem_mesh* um = g.m.get();
     const size_t rl = (size_t)um->L.E * um->L.N * um->L.N;
@@ -1377,18 +1473,23 @@ static void batch_group_bdf2(sem_batch::Group& g, double* const* re, double* con
         pack_union(um, (int)s, im[k], ui, 1);
         pack_union(um, (int)s, V[k], uv, 1);
     }
-    std::unique_ptr<Stepper> st(stepper_create(um, 0, ur.data(), ui.data(), uv.data(), o));
+    Stepper* st = stepper_create(um, 0, ur.data(), ui.data(), uv.data(), o);
     st->inst_ok.assign(um->n_inst, 1);
-    const bool all = stepper_advance(*st, T, 0, nullptr, nullptr);
-    (void)all;
-    stepper_read(*st, ur.data(), ui.data());
+    return st;
+}
+
+static void batch_group_read(sem_batch::Group& g, Stepper& st, double* const* re, double* const* im, int* ok)
+{
+    sem_mesh* um = g.m.get();
+    const size_t rl = (size_t)um->L.E * um->L.N * um->L.N;
+    std::vector<double> ur(rl), ui(rl);
+    stepper_read(st, ur.data(), ui.data());
     for (size_t s = 0; s < g.members.size(); ++s) {
         const int k = g.members[s];
-        unpack_union(um, (int)s, ur, re[k], 1);
-        unpack_union(um, (int)s, ui, im[k], 1);
-        if (ok) ok[k] = st->inst_ok[s];
+        if (re) unpack_union(um, (int)s, ur, re[k], 1);
+        if (im) unpack_union(um, (int)s, ui, im[k], 1);
+        if (ok) ok[k] = st.inst_ok[s];
     }
-    stats = st->stats;
 }
 
 // ------------------------------------------------------- generic solvers --
@@ -1955,45 +2056,91 @@ int sem_batch_autotune(sem_batch* b, int precision, int steps, int* shapes)
     });
 }
 
+int sem_batch_set_graph_mode(sem_batch* b, int on, int steps)
+{
+    if (!b) return SEM_EINVAL;
+    for (auto& g : b->groups) {
+        const int rc = sem_set_graph_mode(g.m.get(), on, steps);
+        if (rc != SEM_OK) return rc;
+    }
+    return SEM_OK;
+}
+
 int sem_batch_set_step_shape(sem_batch* b, int group, int precision, const int* shape)
 {
     if (!b || group < 0 || group >= (int)b->groups.size()) return SEM_EINVAL;
     return sem_set_step_shape(b->groups[group].m.get(), precision, shape);
 }
 
-int sem_batch_schrodinger_bdf2(sem_batch* b, double* const* re, double* const* im, const double* const* V, double T,
-                               const sem_solver_opts* opts, int* ok)
+int sem_batch_start_bdf2(sem_batch* b, const double* const* re, const double* const* im, const double* const* V,
+                         const sem_solver_opts* opts)
 {
     return guarded([&] {
-        if (!b || !re || !im || !V || !opts) throw std::invalid_argument("sem_batch_schrodinger_bdf2: null argument");
-        const size_t n = b->inst_group.size();
-        for (size_t k = 0; k < n; ++k)
-            if (!re[k] || !im[k] || !V[k]) throw std::invalid_argument("sem_batch_schrodinger_bdf2: null instance array");
-        std::vector<std::exception_ptr> err(b->groups.size());
-        std::vector<sem_run_stats> st(b->groups.size());
+        if (!b || !re || !im || !V || !opts) throw std::invalid_argument("sem_batch_start_bdf2: null argument");
+        for (size_t k = 0; k < b->inst_group.size(); ++k)
+            if (!re[k] || !im[k] || !V[k]) throw std::invalid_argument("sem_batch_start_bdf2: null instance array");
+        b->session.clear();
+        for (auto& g : b->groups) b->session.emplace_back(batch_group_start(g, re, im, V, opts));
+        b->stats = sem_run_stats{};
+        b->seconds = 0;
+        return (int)SEM_OK;
+    });
+}
+
+int sem_batch_advance(sem_batch* b, double T, long max_steps, int* success)
+{
+    return guarded([&] {
+        if (!b || b->session.size() != b->groups.size()) throw std::invalid_argument("sem_batch_advance: no session");
+        const size_t ng = b->groups.size();
+        std::vector<std::exception_ptr> err(ng);
+        std::vector<int> okg(ng, 1);
         const auto t0 = std::chrono::steady_clock::now();
         std::vector<std::thread> th;
-        for (size_t g = 0; g < b->groups.size(); ++g)
+        for (size_t g = 0; g < ng; ++g)
             th.emplace_back([&, g] {
                 try {
-                    batch_group_bdf2(b->groups[g], re, im, V, T, opts, ok, st[g]);
+                    okg[g] = stepper_advance(*b->session[g], T, max_steps, nullptr, nullptr) ? 1 : 0;
                 } catch (...) {
                     err[g] = std::current_exception();
                 }
             });
         for (auto& t : th) t.join();
-        b->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        b->seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         for (auto& e : err)
             if (e) std::rethrow_exception(e);
         sem_run_stats tot{};
-        for (const auto& x : st) {
-            tot.steps = std::max(tot.steps, x.steps);
-            tot.stage_solves += x.stage_solves;
-            tot.total_iters += x.total_iters;
+        for (const auto& st : b->session) {
+            tot.steps = std::max(tot.steps, st->stats.steps);
+            tot.stage_solves += st->stats.stage_solves;
+            tot.total_iters += st->stats.total_iters;
         }
         b->stats = tot;
+        if (success) {
+            *success = 1;
+            for (int v : okg) *success = *success && v;
+        }
         return (int)SEM_OK;
     });
+}
+
+int sem_batch_read(sem_batch* b, double* const* re, double* const* im, int* ok, double* t)
+{
+    return guarded([&] {
+        if (!b || b->session.size() != b->groups.size()) throw std::invalid_argument("sem_batch_read: no session");
+        for (size_t g = 0; g < b->groups.size(); ++g) batch_group_read(b->groups[g], *b->session[g], re, im, ok);
+        if (t) *t = b->session.empty() ? 0.0 : b->session[0]->t;
+        return (int)SEM_OK;
+    });
+}
+
+int sem_batch_schrodinger_bdf2(sem_batch* b, double* const* re, double* const* im, const double* const* V, double T,
+                               const sem_solver_opts* opts, int* ok)
+{
+    int rc = sem_batch_start_bdf2(b, re, im, V, opts);
+    if (rc == SEM_OK) rc = sem_batch_advance(b, T, 0, nullptr);
+    if (rc == SEM_OK) rc = sem_batch_read(b, re, im, ok, nullptr);
+    if (b) b->session.clear();
+    return rc;
 }
 
 int sem_batch_last_run_stats(const sem_batch* b, sem_run_stats* out, double* seconds)
@@ -2162,6 +2309,18 @@ int sem_set_step_shape(sem_mesh* m, int precision, const int* shape)
         if ((shape[q] != 1 && shape[q] != 2) || shape[q + 1] < 0) return SEM_EINVAL;
     m->shape[precision == SEM_PREC_FP64 ? 0 : 1] = StepShape{shape[0], shape[1], shape[2], shape[3]};
     return SEM_OK;
+}
+
+int sem_set_graph_mode(sem_mesh* m, int on, int steps)
+{
+    if (!m || (on && (steps < 3 || steps % 3 != 0 || steps > 3000))) return SEM_EINVAL;
+    return guarded([&] {
+        m->sync();
+        m->clear_graphs();
+        m->use_graphs = on != 0;
+        if (on) m->graph_steps = steps;
+        return (int)SEM_OK;
+    });
 }
 
 int sem_get_step_shape(const sem_mesh* m, int precision, int* shape)

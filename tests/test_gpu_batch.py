@@ -56,15 +56,17 @@ def test_batch_matches_single_runs_bitwise(sem):
     batch.close()
 
 
-def test_batch_persistent_shapes_bitwise(sem):
+def test_batch_shapes_and_graphs_bitwise(sem):
     dt, steps = 2e-3, 3
     T = (steps - 0.5) * dt
     inst = [instance(sem, n, 4, s) for n, s in ((7, 11), (4, 12), (9, 13))]
     opts = sem.SolverOpts(max_iter=5000, tol=1e-12, dt=dt)
     batch = sem.Batch([x[0] for x in inst])
     out = []
-    for shape in ((1, 0, 1, 0), (2, 5, 1, 3), (1, 2, 2, 0)):
+    for shape, graphs in (((1, 0, 1, 0), False), ((1, 0, 1, 0), True), ((2, 5, 1, 3), True), ((1, 2, 2, 0), False),
+                          ((1, 2, 2, 0), True)):
         batch.set_step_shape(0, shape)
+        batch.set_graph_mode(graphs, 6)
         re = [x[4].copy() for x in inst]
         im = [x[5].copy() for x in inst]
         assert all(batch.schrodinger_bdf2(re, im, [x[3] for x in inst], T, opts))
@@ -89,3 +91,23 @@ def test_batch_oracle_parity(sem, orc):
         assert ok and int(stats[0]) == steps
         err = rel(np.concatenate([re[k], im[k]]), np.concatenate([ro, io]))
         assert err < 1e-9, (k, err)
+
+
+def test_batch_session_matches_one_shot(sem):
+    dt, steps = 2e-3, 4
+    T = (steps - 0.5) * dt
+    inst = [instance(sem, n, p, s) for n, p, s in ((5, 4, 31), (3, 6, 32), (6, 4, 33))]
+    opts = sem.SolverOpts(max_iter=5000, tol=1e-12, dt=dt)
+    batch = sem.Batch([x[0] for x in inst])
+    re = [x[4].copy() for x in inst]
+    im = [x[5].copy() for x in inst]
+    assert all(batch.schrodinger_bdf2(re, im, [x[3] for x in inst], T, opts))
+    batch.start_bdf2([x[4] for x in inst], [x[5] for x in inst], [x[3] for x in inst], opts)
+    assert batch.advance(T, 2)
+    _, _, _, t = batch.read()
+    assert abs(t - 2 * dt) < 1e-15
+    assert batch.advance(T, 0)
+    r2, i2, ok, t = batch.read()
+    assert all(ok) and batch.last_run_stats()["steps"] == steps
+    for k in range(len(inst)):
+        assert np.array_equal(r2[k], re[k]) and np.array_equal(i2[k], im[k])

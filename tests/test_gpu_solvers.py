@@ -155,3 +155,27 @@ def test_step_shapes_bit_identical(sem, precision):
     assert (chosen[0], chosen[1]) in [(r[0], r[1]) for r in rows]
     z, r = sem.stage_solve(m, V, 2e-3, b, None, 3000, 1e-11, precision)
     assert np.array_equal(z, ref[0])
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_graph_mode_bit_identical(sem, precision):
+    # Lanczos-step batches replayed as captured graphs give the eager path's
+    # bits, on every replay (several solves reuse one captured graph)
+    xy, q = structured(16, (-2, 2, -2, 2), 0.1, 5)
+    m = sem.Mesh.from_arrays(5, xy, q)
+    rng = np.random.default_rng(13)
+    V = m.field(lambda x, y: x * x + 0.5 * y * y)
+    bs = [np.concatenate([sem.gproj(rng.standard_normal(m.size), m), sem.gproj(rng.standard_normal(m.size), m)])
+          for _ in range(2)]
+    sem.set_graph_mode(m, False)
+    ref = [sem.stage_solve(m, V, 3e-3, b, None, 3000, 1e-11, precision) for b in bs]
+    for steps in (3, 12):
+        sem.set_graph_mode(m, True, steps)
+        for rep in range(3):
+            for b, (z0, r0) in zip(bs, ref):
+                z, r = sem.stage_solve(m, V, 3e-3, b, None, 3000, 1e-11, precision)
+                assert r.success and (r.n_iter, r.residual) == (r0.n_iter, r0.residual), (steps, rep)
+                assert np.array_equal(z, z0), (steps, rep)
+    with pytest.raises(sem.SemError):
+        sem.set_graph_mode(m, True, 4)
+    sem.set_graph_mode(m, False)
