@@ -166,6 +166,85 @@ def bench_batched(sem, args):
     return 0
 
 
+def config5(sem, n=256, p=7):
+    """256 x 256 quads on [-12, 12]^2 at degree 7; V = 0.5|x|^2 plus 64 Gaussian bumps (seed 4);
+    Gaussian wavepacket of width 1 with momentum (2, 1); mapped to the reference's equation as for
+    the DIRK4S3 workload (V_ref = 2V, psi conjugated)."""
+    xy, q = sem.structured_mesh(n, (-12.0, 12.0, -12.0, 12.0), 0.0, 0)
+    m = sem.Mesh.from_arrays(p, xy, q)
+    X, Y = m.coords()
+    rng = np.random.default_rng(4)
+    h = rng.uniform(-5, 5, 64)
+    w = rng.uniform(0.3, 1.5, 64)
+    c = rng.uniform(-12, 12, (64, 2))
+    V = 0.5 * (X * X + Y * Y)
+    for k in range(64):
+        V = V + h[k] * np.exp(-((X - c[k, 0]) ** 2 + (Y - c[k, 1]) ** 2) / (2 * w[k] ** 2))
+    env = np.exp(-(X * X + Y * Y) / 2.0)
+    return m, 2.0 * V, env * np.cos(2 * X + Y), -env * np.sin(2 * X + Y)
+
+
+def bench_learned(sem, args):
+    """Learned vs extrapolated stage guesses with DIRK4S5: the network trains on the first
+    learn_steps steps' stage solutions and is frozen after; both arms stop at the b-relative
+    true FP64 residual tol (the reference's test with x0 = 0)."""
+    m, Vref, re, im = config5(sem, args.n5, args.p5)
+    log("mesh: E=%d N=%d dof=%d" % (m.n_elements, m.N, m.dof()))
+    for name, prec in (("fp64", sem.PREC_FP64), ("fp32", sem.PREC_MIXED)):
+        shape, _ = sem.autotune(m, sem.OP_SCHRO_RES, prec, 24)
+        log("autotune %s: %s" % (name, shape))
+    dt_ref = 1e-3 / 2.0
+    out = {}
+    psi = {}
+    for name, guess in (("extrapolated", sem.GUESS_EXTRAPOLATED), ("learned", sem.GUESS_LEARNED)):
+        opts = sem.SolverOpts(max_iter=100000, tol=args.tol5, dt=dt_ref, precision=sem.PREC_MIXED, guess=guess,
+                              learn_steps=args.learn_steps)
+        st = sem.Stepper(m, 2, re, im, Vref, opts)
+        phases = {}
+        for phase, nsteps in (("training", args.learn_steps), ("frozen", args.steps5 - args.learn_steps)):
+            s0 = st.stats()
+            t = 0.0
+            done = 0
+            while done < nsteps:
+                k = min(10, nsteps - done)
+                t0 = time.perf_counter()
+                assert st.advance(1e30, k)
+                t += time.perf_counter() - t0
+                done += k
+                log("  %s %s: %d steps, %.2f s" % (name, phase, done, t))
+            s1 = st.stats()
+            phases[phase] = {"steps": nsteps, "seconds": round(t, 3),
+                             "stage_solves": s1["stage_solves"] - s0["stage_solves"],
+                             "inner_iters": s1["total_iters"] - s0["total_iters"],
+                             "refinements": s1["refinements"] - s0["refinements"],
+                             "learned_guess_chosen": s1["learned_guess_chosen"] - s0["learned_guess_chosen"],
+                             "max_true_rel_residual": s1["max_true_rel_residual"]}
+        phases["mlp_last_loss"] = st.stats()["mlp_last_loss"]
+        a, b, _ = st.read()
+        st.close()
+        psi[name] = np.concatenate([a, b])
+        out[name] = phases
+    fe, fl = out["extrapolated"]["frozen"], out["learned"]["frozen"]
+    line = {
+        "metric": "DIRK4S5 MINRES iterations and wall time, learned vs extrapolated stage guess (frozen network)",
+        "value": round(fe["inner_iters"] / max(fl["inner_iters"], 1), 4),
+        "unit": "iteration ratio extrapolated/learned",
+        "n_gpus": 1,
+        "higher_is_better": True,
+        "dtype": "f64+f32+bf16",
+        "data": "synthetic",
+        "config": {"workload": "learned_dirk4s5_%dx%d_p%d" % (args.n5, args.n5, args.p5), "dof": m.dof(),
+                   "dt": 1e-3, "tol": args.tol5, "stop": "true FP64 residual <= tol*||b|| + tol",
+                   "steps": args.steps5, "learn_steps": args.learn_steps},
+        "wall_time_ratio_frozen": round(fe["seconds"] / max(fl["seconds"], 1e-9), 4),
+        "modes": out,
+        "rel_l2_learned_vs_extrapolated": float(np.linalg.norm(psi["learned"] - psi["extrapolated"]) /
+                                                np.linalg.norm(psi["extrapolated"])),
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def run_mode(sem, m, Vref, re, im, dt, tol, steps, warmup, precision, guess):
     opts = sem.SolverOpts(max_iter=100000, tol=tol, dt=dt, precision=precision, guess=guess, learn_steps=warmup)
     st = sem.Stepper(m, 1, re, im, Vref, opts)
@@ -203,7 +282,12 @@ def main():
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-glaplace", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
-    ap.add_argument("--workload", default="dirk4s3", choices=["dirk4s3", "batched_bdf2"])
+    ap.add_argument("--workload", default="dirk4s3", choices=["dirk4s3", "batched_bdf2", "learned_dirk4s5"])
+    ap.add_argument("--n5", type=int, default=256)
+    ap.add_argument("--p5", type=int, default=7)
+    ap.add_argument("--steps5", type=int, default=100)
+    ap.add_argument("--learn-steps", type=int, default=50)
+    ap.add_argument("--tol5", type=float, default=1e-10)
     ap.add_argument("--instances", type=int, default=64)
     ap.add_argument("--batch-steps", type=int, default=100)
     ap.add_argument("--seq-steps", type=int, default=5)
@@ -220,6 +304,8 @@ def main():
     log("device:", dev)
     if args.workload == "batched_bdf2":
         return bench_batched(sem, args)
+    if args.workload == "learned_dirk4s5":
+        return bench_learned(sem, args)
     m, xy, q, Vref, re, im = config3(sem, args.n, args.p)
     dt_ref = 1e-3 / 2.0
     log("mesh: E=%d N=%d dof=%d" % (m.n_elements, m.N, m.dof()))

@@ -78,9 +78,10 @@ typedef struct sem_run_stats {
     long total_iters;        /* inner Krylov iterations, all stages */
     long refinements;        /* FP64 refinement sweeps (mixed precision) */
     double last_error_estimate; /* DIRK4S5 embedded estimate (schrodinger.hpp:262-263) */
-    double max_true_rel_residual; /* max over stage solves of ||b - A x|| / ||b||, FP64 */
-    long mlp_train_steps;    /* Adam steps taken by the guess network */
+    double max_true_rel_residual; /* max over FP64-refined stage solves of ||b - A x|| / ||b|| */
+    long mlp_train_steps;    /* Adam steps taken by the guess networks */
     double mlp_last_loss;    /* last training loss (mean squared error, scaled targets) */
+    long learned_guess_chosen; /* stage solves that started from the network's guess (not the fallback) */
 } sem_run_stats;
 
 typedef struct sem_mesh sem_mesh;
@@ -156,6 +157,10 @@ int sem_stepper_read(sem_stepper* s, double* re, double* im, double* t);
 int sem_stepper_stats(const sem_stepper* s, sem_run_stats* out);
 /* learned guess: Adam steps per stage solve while training (default 10) and learning rate (default 3e-3) */
 int sem_stepper_set_learning(sem_stepper* s, int epochs, float lr);
+/* learned guess: on (default) starts each stage solve from the network's guess
+ * only when its true FP64 residual is below the extrapolation's (two extra
+ * operator applies per stage solve); off always starts from the network's guess */
+int sem_stepper_set_guard(sem_stepper* s, int on);
 void sem_stepper_destroy(sem_stepper* s);
 /* Batched instances (no counterpart in the reference, which runs one problem
  * per call): independent meshes advanced together. Instances of one degree

@@ -50,7 +50,7 @@ class _Stats(C.Structure):
     _fields_ = [("steps", C.c_int), ("stage_solves", C.c_int), ("total_iters", C.c_long),
                 ("refinements", C.c_long), ("last_error_estimate", C.c_double),
                 ("max_true_rel_residual", C.c_double), ("mlp_train_steps", C.c_long),
-                ("mlp_last_loss", C.c_double)]
+                ("mlp_last_loss", C.c_double), ("learned_guess_chosen", C.c_long)]
 
 
 STEP_CB = C.CFUNCTYPE(C.c_int, _dp, _dp, C.c_void_p)
@@ -118,6 +118,7 @@ def lib():
             "sem_stepper_stats": (C.c_int, [C.c_void_p, C.POINTER(_Stats)]),
             "sem_stepper_destroy": (None, [C.c_void_p]),
             "sem_stepper_set_learning": (C.c_int, [C.c_void_p, C.c_int, C.c_float]),
+            "sem_stepper_set_guard": (C.c_int, [C.c_void_p, C.c_int]),
             "sem_batch_create": (C.c_int, [C.c_int, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)]),
             "sem_batch_destroy": (None, [C.c_void_p]),
             "sem_batch_info": (C.c_int, [C.c_void_p, _ip, _ip, _ip, _lp]),
@@ -529,6 +530,10 @@ class Stepper:
         s = _Stats()
         _check(lib().sem_stepper_stats(self._h, C.byref(s)))
         return {k: getattr(s, k) for k, _ in _Stats._fields_}
+
+    def set_guard(self, on):
+        """Learned guess: keep the extrapolation when its true residual is smaller (default on)."""
+        _check(lib().sem_stepper_set_guard(self._h, int(bool(on))))
 
     def close(self):
         if getattr(self, "_h", None) is not None and self._h.value and _lib is not None:

@@ -93,6 +93,7 @@ struct sem_mesh {
     int4* d_chunk_info = nullptr;
     std::vector<Results> inst_results;   // per-instance results of the last Krylov solve
     std::vector<uint8_t> ghost;          // per element: padding element of a batched mesh
+    double last_true_rel = 0;            // ||b - A x|| / ||b|| after the last FP64-refined solve
     // scalars: one record per instance
     KrylovScalars* sc = nullptr;
     KrylovScalars* sc_alt = nullptr;  // second state buffer of the Lanczos-step kernels
@@ -894,8 +895,8 @@ static Results solve_mixed(sem_mesh* m, int kind, const double* cf64, const floa
         lincomb<double>(m, r, 0, {{1.0, b}, {-1.0, Ax}}, n);
     }
     double r0 = std::sqrt(wdot_host<double>(m, r, r, nc));
-    double ref = r0;
-    if (o.rel_to_b) ref = std::sqrt(wdot_host<double>(m, b, b, nc));
+    const double bnorm = std::sqrt(wdot_host<double>(m, b, b, nc));
+    const double ref = o.rel_to_b ? bnorm : r0;
     const double target = o.tol * ref + o.atol;
     Results res;
     res.residual = r0;
@@ -923,6 +924,7 @@ static Results solve_mixed(sem_mesh* m, int kind, const double* cf64, const floa
     res.success = rn < target;
     res.n_iter = (int)inner_total;
     res.residual = rn;
+    m->last_true_rel = rn / std::max(bnorm, 1e-300);
     return res;
 }
 
@@ -974,6 +976,9 @@ static void merge_owner(sem_mesh* m, const double* xdev, double* u_host)
 
 // Schrodinger stepper session: the state psi, the potential and the stage
 // vectors stay resident on the GPU between advance() calls.
+// one-hot stage columns of the guess network's input (at most 8 stages)
+constexpr int kStageCols = 8;
+
 struct Stepper {
     sem_mesh* m = nullptr;
     int scheme = 1;  // 0 bdf2, 1 dirk4s3, 2 dirk4s5
@@ -993,8 +998,10 @@ struct Stepper {
     double alpha_bdf = 0;
     sem_run_stats stats{};
     std::vector<int> inst_ok;  // per instance: every solve so far succeeded
-    // learned guess (guess = 2)
+    // learned guess (guess = 2): one network for all stages, told the stage
+    // by one-hot input columns
     std::unique_ptr<Mlp> mlp;
+    bool guard = true;  // keep the extrapolation when its residual is smaller
     double* Vraw = nullptr;
     double* ext = nullptr;
     double* tmp = nullptr;
@@ -1034,9 +1041,10 @@ static Results solve_res(Stepper& st, double alpha, const double* b, double* x, 
     ko.rel_to_b = rel_to_b;
     Results r;
     const long ref0 = m->stats.refinements;
-    if (st.opts.precision == SEM_PREC_MIXED)
+    if (st.opts.precision == SEM_PREC_MIXED) {
         r = solve_mixed(m, K_RES, st.V.c64, st.V.c32, alpha, b, x, zero, ko, 2e-5);
-    else
+        st.stats.max_true_rel_residual = std::max(st.stats.max_true_rel_residual, m->last_true_rel);
+    } else
         r = minres_fused<double>(m, K_RES, st.V.c64, alpha, b, x, zero, ko);
     if (st.inst_ok.size() != (size_t)m->n_inst) st.inst_ok.assign(m->n_inst, 1);
     for (int k = 0; k < m->n_inst; ++k) st.inst_ok[k] = st.inst_ok[k] && m->inst_results[k].success;
@@ -1053,10 +1061,11 @@ static double rms_dev(sem_mesh* m, const double* a, int ncomp)
     return std::sqrt(s / std::max(1.0, (double)ncomp * m->hm.dof));
 }
 
-static MlpScatterArgs scatter_args(Stepper& st, double* out, const double* base, double scale)
+static MlpScatterArgs scatter_args(Stepper& st, int stage, double* out, const double* base, double scale)
 {
     sem_mesh* m = st.m;
     MlpScatterArgs a{};
+    (void)stage;
     a.Y = st.mlp->Y;
     a.ld = st.mlp->out_pad;
     a.out = out;
@@ -1085,6 +1094,8 @@ static void mlp_features(Stepper& st, int stage, const double* b)
         f.scale[g] = sc[g];
     }
     f.ngroups = 7;
+    f.extra = kStageCols;
+    for (int k = 0; k < kStageCols; ++k) f.extra_val[k] = (k == stage) ? 1.f : 0.f;
     f.X = st.mlp->X;
     f.ld = st.mlp->in_pad;
     f.E = m->L.E;
@@ -1097,7 +1108,11 @@ static Results stage_solve(Stepper& st, double alpha, const double* b, double* z
 {
     sem_mesh* m = st.m;
     bool zero = true;
+    // the network trains on the stage solutions of the first learn_steps steps,
+    // which start from the extrapolated guess (so the training window follows
+    // the fallback's trajectory); afterwards it is frozen and proposes guesses
     const bool learned = st.opts.guess == 2 && st.have_prev2;
+    const bool training = learned && st.stats.steps < st.learn_steps;
     if (learned) {
         // extrapolation plus the network's correction, made consistent
         lincomb<double>(m, st.ext, 1, {{2.0, st.zprev[stage]}, {-1.0, st.zprev2[stage]}}, st.n);
@@ -1107,13 +1122,31 @@ static Results stage_solve(Stepper& st, double alpha, const double* b, double* z
             st.s_feat[stage][2] = 1.0 / std::max(rms_dev(m, b, 2), 1e-300);
             st.have_scale[stage] = true;
         }
-        if (st.have_out[stage]) {
-            st.mlp->reserve(m->L.E);
+        if (st.have_out[stage] && !training) {
+            Mlp& net = *st.mlp;
+            net.reserve(m->L.E);
             mlp_features(st, stage, b);
-            st.mlp->forward(m->L.E);
-            launch_scatter_output(scatter_args(st, z, st.ext, st.s_out[stage]), m->stream);
+            net.forward(m->L.E);
+            launch_scatter_output(scatter_args(st, stage, z, st.ext, st.s_out[stage]), m->stream);
             dss<double>(m, DSS_UNMASK, z, 2);
             dss<double>(m, DSS_ZERO_BND, z, 2);
+            // guard: the extrapolation stays the fallback; keep whichever
+            // guess has the smaller true FP64 residual ||b - Res z||
+            if (!st.guard) {
+                st.stats.learned_guess_chosen++;
+            } else {
+            double* Az = m->buf<double>("guard_Az", st.n);
+            double* rz = m->buf<double>("guard_r", st.n);
+            apply_op<double>(m, K_RES, z, Az, st.V.c64, alpha);
+            lincomb<double>(m, rz, 0, {{1.0, b}, {-1.0, Az}}, st.n);
+            wdot_dev<double>(m, rz, rz, 2, 2);
+            apply_op<double>(m, K_RES, st.ext, Az, st.V.c64, alpha);
+            lincomb<double>(m, rz, 0, {{1.0, b}, {-1.0, Az}}, st.n);
+            wdot_dev<double>(m, rz, rz, 2, 3);
+            m->read_scalars();
+            if (m->h_sc[0].red[3] < m->h_sc[0].red[2]) launch_cast<double, double>(z, st.ext, st.n, m->stream);
+            else st.stats.learned_guess_chosen++;
+            }
         } else {
             launch_cast<double, double>(z, st.ext, st.n, m->stream);
         }
@@ -1126,20 +1159,21 @@ static Results stage_solve(Stepper& st, double alpha, const double* b, double* z
         zero = false;
     }
     Results r = solve_res(st, alpha, b, z, zero, !zero);
-    if (learned && r.success && st.stats.steps < st.learn_steps) {
+    if (training && r.success) {
         // train on this stage's solution: target = (z - extrapolation) / s_out
         if (!st.have_out[stage]) {
             lincomb<double>(m, st.tmp, 0, {{1.0, z}, {-1.0, st.ext}}, st.n);
             st.s_out[stage] = std::max(rms_dev(m, st.tmp, 2), 1e-300);
             st.have_out[stage] = true;
         }
-        st.mlp->reserve(m->L.E);
+        Mlp& net = *st.mlp;
+        net.reserve(m->L.E);
         mlp_features(st, stage, b);
-        launch_gather_target(scatter_args(st, nullptr, st.ext, 1.0 / st.s_out[stage]), st.mlp->T, z, m->stream);
+        launch_gather_target(scatter_args(st, stage, nullptr, st.ext, 1.0 / st.s_out[stage]), net.T, z, m->stream);
         for (int ep = 0; ep < st.epochs; ++ep) {
             const bool last = ep + 1 == st.epochs;
-            const double loss = st.mlp->backward(m->L.E, last);
-            st.mlp->adam_step();
+            const double loss = net.backward(m->L.E, last);
+            net.adam_step();
             st.stats.mlp_train_steps++;
             if (last) st.stats.mlp_last_loss = loss;
         }
@@ -1226,7 +1260,7 @@ static Stepper* stepper_create(sem_mesh* m, int scheme, const double* re, const 
         }
     if (o->guess == 2) {
         const int N = m->L.N, NN = N * N;
-        st->mlp.reset(new Mlp(7 * NN, 256, 256, 2 * NN, 0x5EED0001ull, m->stream));
+        st->mlp.reset(new Mlp(7 * NN + kStageCols, 256, 256, 2 * NN, 0x5EED0001ull, m->stream));
         st->mlp->lr = 3e-3f;
         st->learn_steps = o->learn_steps > 0 ? o->learn_steps : 50;
         st->ext = st->alloc();
@@ -1981,6 +2015,13 @@ int sem_stepper_set_learning(sem_stepper* h, int epochs, float lr)
     }
     h->st->epochs = epochs;
     h->st->mlp->lr = lr;
+    return SEM_OK;
+}
+
+int sem_stepper_set_guard(sem_stepper* h, int on)
+{
+    if (!h) return SEM_EINVAL;
+    h->st->guard = on != 0;
     return SEM_OK;
 }
 

@@ -209,7 +209,10 @@ __global__ __launch_bounds__(256) void k_gather_features(MlpFeatureArgs a)
         float* row = a.X + (size_t)e * a.ld;
         for (int g = 0; g < a.ngroups; ++g) row[g * NN + n] = (float)(a.scale[g] * a.src[g][p]);
         if (n == 0)
-            for (int c = a.ngroups * NN; c < a.ld; ++c) row[c] = 0.f;
+            for (int c = a.ngroups * NN; c < a.ld; ++c) {
+                const int q = c - a.ngroups * NN;
+                row[c] = q < a.extra ? a.extra_val[q] : 0.f;
+            }
     }
 }
 

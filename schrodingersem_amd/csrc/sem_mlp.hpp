@@ -42,11 +42,14 @@ struct Mlp {
     void colsum(const float* X, int rows, int cols, float* out);
 };
 
-// X[e][g*N*N + i + j*N] = scale[g] * src[g][pos(e, i, j)], padding columns zero
+// X[e][g*N*N + i + j*N] = scale[g] * src[g][pos(e, i, j)], then `extra`
+// constant columns extra_val[0..extra), padding columns zero
 struct MlpFeatureArgs {
     const double* src[8];
     double scale[8];
     int ngroups;
+    int extra;
+    float extra_val[8];
     float* X;
     int ld;
     int E, N, K;
