@@ -14,7 +14,7 @@
  *     Calls on one handle are not re-entrant; separate handles may run
  *     concurrently on separate host threads.
  *   - FP64 everywhere at the ABI. Internally the solver may run its inner
- *     operator applies in FP32 (SEM_PREC_MIXED), always judged by an FP64
+ *     operator applies in FP32 (SEM_PREC_MIXED), always stopped by an FP64
  *     residual.
  */
 #ifndef SEM_H
@@ -215,9 +215,10 @@ int sem_pcg_tridiag(long n, double lo, double di, double up, const double* b, do
  * random data (kind, precision 0 = FP64, 1 = FP32); returns mean microseconds
  * per launch and the algorithmic bytes per launch. */
 int sem_time_element_op(sem_mesh* m, int kind, int precision, int reps, double* us_per_launch, double* bytes_per_launch);
-/* Live timing of the two MINRES Lanczos-step kernels: on = 1 brackets the
- * launches of every fourth Lanczos step with HIP events on the handle's
- * stream, on = 2 every step, 0 turns it off. which = 0 / 1:
+/* Live timing of the two MINRES Lanczos-step kernels: on = 1 times the
+ * launches of every fourth Lanczos step with HIP events that take the
+ * kernels' own start and end timestamps (hipExtLaunchKernel), on = 2 every
+ * step, 0 turns it off. which = 0 / 1:
  * operator kernel (operator apply, y -= cr1*r1, alpha partial) FP64 / FP32;
  * 2 / 3: update kernel (interface gather, y -= c2*r2, beta^2 partial, w/x
  * update) FP64 / FP32. Returns launches, summed microseconds and the

@@ -564,8 +564,9 @@ static Results minres_fused(sem_mesh* m, int kind, const T* cf, double alpha, co
             SEM_HIP(hipEventCreate(&e));
             m->ev.push_back(e);
         }
-        // algorithmic bytes of one launch: the operator apply (inputs, geometry,
-        // outputs; SURVEY-model fields) plus r1 read for y -= cr1*r1, per component
+        // algorithmic bytes of one launch: the operator apply (its inputs, geometry
+        // and outputs, each read or written once, as the reference stores them)
+        // plus r1 read for y -= cr1*r1, per component
         const double EN2 = (double)m->L.E * m->L.N * m->L.N;
         const double op_vals = (kind == K_RES) ? 9 : (kind == K_HELMHOLTZ ? 7 : 5);
         m->kt_bytes[ti] = (op_vals + 1.0 * nc) * sizeof(T) * EN2;
@@ -610,9 +611,8 @@ static Results minres_fused(sem_mesh* m, int kind, const T* cf, double alpha, co
             a.st_out = buf[cur];
             a.prev_partial = m->partial + nch;
         }
-        if (tim) SEM_HIP(hipEventRecord(m->ev[4 * b_], s));
-        launch_minres_op_d<T>(m->L.N, kind, a, m->dmat<T>(), shp.op_cpb, shp.op_grid, s);
-        if (tim) SEM_HIP(hipEventRecord(m->ev[4 * b_ + 1], s));
+        launch_minres_op_d<T>(m->L.N, kind, a, m->dmat<T>(), shp.op_cpb, shp.op_grid, s, tim ? m->ev[4 * b_] : nullptr,
+                              tim ? m->ev[4 * b_ + 1] : nullptr);
         MinresUpdArgs<T> u{};
         u.yraw = Yraw;
         u.y = R[i % 3];
@@ -634,9 +634,8 @@ static Results minres_fused(sem_mesh* m, int kind, const T* cf, double alpha, co
         u.it = itab;
         u.st = buf[cur];
         u.op_partial = m->partial;
-        if (tim) SEM_HIP(hipEventRecord(m->ev[4 * b_ + 2], s));
-        launch_minres_upd<T>(m->L.N, u, shp.upd_cpb, shp.upd_grid, s);
-        if (tim) SEM_HIP(hipEventRecord(m->ev[4 * b_ + 3], s));
+        launch_minres_upd<T>(m->L.N, u, shp.upd_cpb, shp.upd_grid, s, tim ? m->ev[4 * b_ + 2] : nullptr,
+                             tim ? m->ev[4 * b_ + 3] : nullptr);
     };
     int i = 1;
     bool done = o.maxit < 1;

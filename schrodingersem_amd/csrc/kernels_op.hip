@@ -11,6 +11,8 @@
 //   K u (i,l) = sum_k D(k,i) G1(k,l) + sum_k D(k,l) G2(i,k),
 //   G1 = gA u_xi - gB u_eta, G2 = gC u_eta - gB u_xi,
 // which is F W + W G of glaplace.hpp:22-25 with the weights moved inside.
+#include <hip/hip_ext.h>
+
 #include "krylov_device.hpp"
 #include "sem_internal.hpp"
 
@@ -128,28 +130,37 @@ __device__ __forceinline__ bool is_interior(int i, int l)
     return i > 0 && i < N - 1 && l > 0 && l < N - 1;
 }
 
-// Column loaders / storers of the device layout.
-template <int N, typename T>
+// Column loaders / storers of the device layout. NT: nontemporal accesses,
+// for launches whose arrays do not fit the Infinity Cache (streamed once).
+template <int N, typename T, bool NT = false>
 struct Col {
     size_t base;
     bool act;
     __device__ __forceinline__ void ld(const T* __restrict__ p, T (&r)[N]) const
     {
 #pragma unroll
-        for (int i = 0; i < N; ++i) r[i] = act ? p[base + (size_t)i * Shape<N>::KN] : T(0);
+        for (int i = 0; i < N; ++i) {
+            const T* q = p + base + (size_t)i * Shape<N>::KN;
+            if constexpr (NT) r[i] = act ? __builtin_nontemporal_load(q) : T(0);
+            else r[i] = act ? *q : T(0);
+        }
     }
     __device__ __forceinline__ void st(T* __restrict__ p, const T (&r)[N]) const
     {
         if (act) {
 #pragma unroll
-            for (int i = 0; i < N; ++i) p[base + (size_t)i * Shape<N>::KN] = r[i];
+            for (int i = 0; i < N; ++i) {
+                T* q = p + base + (size_t)i * Shape<N>::KN;
+                if constexpr (NT) __builtin_nontemporal_store(r[i], q);
+                else *q = r[i];
+            }
         }
     }
 };
 
 // Element operator of every sem_op_kind; ring entries are written raw (the DSS
 // kernel assembles them), interior entries are final and scaled by `post`.
-template <int N, typename T, int KIND>
+template <int N, typename T, int KIND, bool NT>
 __global__ __launch_bounds__(64) void k_element_op(OpArgs<T> a, Dm<N, T> D)
 {
     using S = Shape<N>;
@@ -158,7 +169,7 @@ __global__ __launch_bounds__(64) void k_element_op(OpArgs<T> a, Dm<N, T> D)
     const bool act = lane < S::KN;
     const int el = act ? lane / N : 0;
     const int l = act ? lane % N : 0;
-    const Col<N, T> c{(size_t)blockIdx.x * S::K * N * N + lane, act};
+    const Col<N, T, NT> c{(size_t)blockIdx.x * S::K * N * N + lane, act};
     const size_t len = a.len;
     T ga[N], gb[N], gc[N];
     c.ld(a.geo, ga);
@@ -577,46 +588,69 @@ static Dm<N, T> make_dm(const T* d)
     return m;
 }
 
+// Arrays of one element-operator launch (inputs, geometry, coefficient, outputs)
+// larger than the 256 MB Infinity Cache are streamed with nontemporal accesses;
+// smaller ones stay cacheable for the kernel that reads the result next.
+constexpr double kMallBytes = 256.0 * 1024 * 1024;
+
+template <int N, typename T, bool NT>
+static void element_op_NT(int kind, const OpArgs<T>& a, const Dm<N, T>& D, hipStream_t s)
+{
+    dim3 g(a.nchunks), b(64);
+    switch (kind) {
+    case K_LAPLACE: hipLaunchKernelGGL((k_element_op<N, T, K_LAPLACE, NT>), g, b, 0, s, a, D); break;
+    case K_HELMHOLTZ: hipLaunchKernelGGL((k_element_op<N, T, K_HELMHOLTZ, NT>), g, b, 0, s, a, D); break;
+    case K_MASS: hipLaunchKernelGGL((k_element_op<N, T, K_MASS, NT>), g, b, 0, s, a, D); break;
+    case K_RES: hipLaunchKernelGGL((k_element_op<N, T, K_RES, NT>), g, b, 0, s, a, D); break;
+    case K_F: hipLaunchKernelGGL((k_element_op<N, T, K_F, NT>), g, b, 0, s, a, D); break;
+    case K_L: hipLaunchKernelGGL((k_element_op<N, T, K_L, NT>), g, b, 0, s, a, D); break;
+    }
+}
+
 template <int N, typename T>
 static void element_op_N(int kind, const OpArgs<T>& a, const T* d, hipStream_t s)
 {
     auto D = make_dm<N, T>(d);
-    dim3 g(a.nchunks), b(64);
-    switch (kind) {
-    case K_LAPLACE: hipLaunchKernelGGL((k_element_op<N, T, K_LAPLACE>), g, b, 0, s, a, D); break;
-    case K_HELMHOLTZ: hipLaunchKernelGGL((k_element_op<N, T, K_HELMHOLTZ>), g, b, 0, s, a, D); break;
-    case K_MASS: hipLaunchKernelGGL((k_element_op<N, T, K_MASS>), g, b, 0, s, a, D); break;
-    case K_RES: hipLaunchKernelGGL((k_element_op<N, T, K_RES>), g, b, 0, s, a, D); break;
-    case K_F: hipLaunchKernelGGL((k_element_op<N, T, K_F>), g, b, 0, s, a, D); break;
-    case K_L: hipLaunchKernelGGL((k_element_op<N, T, K_L>), g, b, 0, s, a, D); break;
-    }
+    const double comps = (kind == K_RES || kind == K_F) ? 2 : 1;
+    const double arrays = 2 * comps + 4 + ((kind == K_HELMHOLTZ || kind == K_RES || kind == K_L) ? 1 : 0);
+    if (arrays * (double)a.len * sizeof(T) > kMallBytes) element_op_NT<N, T, true>(kind, a, D, s);
+    else element_op_NT<N, T, false>(kind, a, D, s);
 }
 
+// Launch; with events, they take the kernel's own start and end timestamps.
+#define SEM_LAUNCH_TIMED(KERNEL, G, B, S, E0, E1, ...)                                  \
+    do {                                                                                \
+        if (E0) hipExtLaunchKernelGGL(KERNEL, G, B, 0, S, E0, E1, 0, __VA_ARGS__);      \
+        else hipLaunchKernelGGL(KERNEL, G, B, 0, S, __VA_ARGS__);                       \
+    } while (0)
+
 template <int N, typename T, int CPB, bool P>
-static void minres_op_NC(int kind, const MinresOpArgs<T>& a, const Dm<N, T>& D, int grid, hipStream_t s)
+static void minres_op_NC(int kind, const MinresOpArgs<T>& a, const Dm<N, T>& D, int grid, hipStream_t s,
+                         hipEvent_t e0, hipEvent_t e1)
 {
     const int items = a.nchunks / CPB;
     dim3 g(P ? std::min(grid, items) : items);
     dim3 b1(64 * CPB), b2(128 * CPB);
     switch (kind) {
-    case K_LAPLACE: hipLaunchKernelGGL((k_minres_op<N, T, K_LAPLACE, CPB, P>), g, b1, 0, s, a, D); break;
-    case K_HELMHOLTZ: hipLaunchKernelGGL((k_minres_op<N, T, K_HELMHOLTZ, CPB, P>), g, b1, 0, s, a, D); break;
-    case K_RES: hipLaunchKernelGGL((k_minres_op<N, T, K_RES, CPB, P>), g, b2, 0, s, a, D); break;
+    case K_LAPLACE: SEM_LAUNCH_TIMED((k_minres_op<N, T, K_LAPLACE, CPB, P>), g, b1, s, e0, e1, a, D); break;
+    case K_HELMHOLTZ: SEM_LAUNCH_TIMED((k_minres_op<N, T, K_HELMHOLTZ, CPB, P>), g, b1, s, e0, e1, a, D); break;
+    case K_RES: SEM_LAUNCH_TIMED((k_minres_op<N, T, K_RES, CPB, P>), g, b2, s, e0, e1, a, D); break;
     default: throw std::runtime_error("minres: unsupported operator kind");
     }
 }
 
 // grid > 0 selects the persistent form (grid-stride over work items)
 template <int N, typename T>
-static void minres_op_N(int kind, const MinresOpArgs<T>& a, const T* d, int cpb, int grid, hipStream_t s)
+static void minres_op_N(int kind, const MinresOpArgs<T>& a, const T* d, int cpb, int grid, hipStream_t s,
+                        hipEvent_t e0, hipEvent_t e1)
 {
     auto D = make_dm<N, T>(d);
     if (cpb == 2) {
-        if (grid > 0) minres_op_NC<N, T, 2, true>(kind, a, D, grid, s);
-        else minres_op_NC<N, T, 2, false>(kind, a, D, grid, s);
+        if (grid > 0) minres_op_NC<N, T, 2, true>(kind, a, D, grid, s, e0, e1);
+        else minres_op_NC<N, T, 2, false>(kind, a, D, grid, s, e0, e1);
     } else {
-        if (grid > 0) minres_op_NC<N, T, 1, true>(kind, a, D, grid, s);
-        else minres_op_NC<N, T, 1, false>(kind, a, D, grid, s);
+        if (grid > 0) minres_op_NC<N, T, 1, true>(kind, a, D, grid, s, e0, e1);
+        else minres_op_NC<N, T, 1, false>(kind, a, D, grid, s, e0, e1);
     }
 }
 
@@ -628,22 +662,22 @@ static void pcg_op_N(const PcgOpArgs<T>& a, int nchunks, const T* d, hipStream_t
 }
 
 template <int N, typename T, int CPB, bool P>
-static void minres_upd_NC(const MinresUpdArgs<T>& a, int grid, hipStream_t s)
+static void minres_upd_NC(const MinresUpdArgs<T>& a, int grid, hipStream_t s, hipEvent_t e0, hipEvent_t e1)
 {
     const int items = a.nchunks / CPB;
     dim3 g(P ? std::min(grid, items) : items), b(128 * a.ncomp * CPB);
-    hipLaunchKernelGGL((k_minres_upd<N, T, CPB, P>), g, b, 0, s, a);
+    SEM_LAUNCH_TIMED((k_minres_upd<N, T, CPB, P>), g, b, s, e0, e1, a);
 }
 
 template <int N, typename T>
-static void minres_upd_N(const MinresUpdArgs<T>& a, int cpb, int grid, hipStream_t s)
+static void minres_upd_N(const MinresUpdArgs<T>& a, int cpb, int grid, hipStream_t s, hipEvent_t e0, hipEvent_t e1)
 {
     if (cpb == 2) {
-        if (grid > 0) minres_upd_NC<N, T, 2, true>(a, grid, s);
-        else minres_upd_NC<N, T, 2, false>(a, grid, s);
+        if (grid > 0) minres_upd_NC<N, T, 2, true>(a, grid, s, e0, e1);
+        else minres_upd_NC<N, T, 2, false>(a, grid, s, e0, e1);
     } else {
-        if (grid > 0) minres_upd_NC<N, T, 1, true>(a, grid, s);
-        else minres_upd_NC<N, T, 1, false>(a, grid, s);
+        if (grid > 0) minres_upd_NC<N, T, 1, true>(a, grid, s, e0, e1);
+        else minres_upd_NC<N, T, 1, false>(a, grid, s, e0, e1);
     }
 }
 
@@ -676,19 +710,21 @@ void launch_element_op_d(int N, int kind, const OpArgs<T>& a, const T* d, hipStr
 }
 
 template <typename T>
-void launch_minres_op_d(int N, int kind, const MinresOpArgs<T>& a, const T* d, int cpb, int grid, hipStream_t s)
+void launch_minres_op_d(int N, int kind, const MinresOpArgs<T>& a, const T* d, int cpb, int grid, hipStream_t s,
+                        hipEvent_t e0, hipEvent_t e1)
 {
     if (a.nchunks % kChunkAlign != 0) throw std::runtime_error("internal error: chunk count not aligned");
-#define CALL_MO(n) minres_op_N<n, T>(kind, a, d, cpb, grid, s)
+#define CALL_MO(n) minres_op_N<n, T>(kind, a, d, cpb, grid, s, e0, e1)
     SEM_DISPATCH_N(N, CALL_MO)
 #undef CALL_MO
 }
 
 template <typename T>
-void launch_minres_upd(int N, const MinresUpdArgs<T>& a, int cpb, int grid, hipStream_t s)
+void launch_minres_upd(int N, const MinresUpdArgs<T>& a, int cpb, int grid, hipStream_t s, hipEvent_t e0,
+                       hipEvent_t e1)
 {
     if (a.nchunks % kChunkAlign != 0) throw std::runtime_error("internal error: chunk count not aligned");
-#define CALL_MU(n) minres_upd_N<n, T>(a, cpb, grid, s)
+#define CALL_MU(n) minres_upd_N<n, T>(a, cpb, grid, s, e0, e1)
     SEM_DISPATCH_N(N, CALL_MU)
 #undef CALL_MU
 }
@@ -705,13 +741,15 @@ void launch_pcg_op_d(int N, const PcgOpArgs<T>& a, int nchunks, const T* d, hipS
 // not) so the two halves of the kernel set build in parallel.
 #ifndef SEM_OP_FP32
 template void launch_element_op_d<double>(int, int, const OpArgs<double>&, const double*, hipStream_t);
-template void launch_minres_op_d<double>(int, int, const MinresOpArgs<double>&, const double*, int, int, hipStream_t);
-template void launch_minres_upd<double>(int, const MinresUpdArgs<double>&, int, int, hipStream_t);
+template void launch_minres_op_d<double>(int, int, const MinresOpArgs<double>&, const double*, int, int, hipStream_t,
+                                          hipEvent_t, hipEvent_t);
+template void launch_minres_upd<double>(int, const MinresUpdArgs<double>&, int, int, hipStream_t, hipEvent_t, hipEvent_t);
 template void launch_pcg_op_d<double>(int, const PcgOpArgs<double>&, int, const double*, hipStream_t);
 #else
 template void launch_element_op_d<float>(int, int, const OpArgs<float>&, const float*, hipStream_t);
-template void launch_minres_op_d<float>(int, int, const MinresOpArgs<float>&, const float*, int, int, hipStream_t);
-template void launch_minres_upd<float>(int, const MinresUpdArgs<float>&, int, int, hipStream_t);
+template void launch_minres_op_d<float>(int, int, const MinresOpArgs<float>&, const float*, int, int, hipStream_t,
+                                          hipEvent_t, hipEvent_t);
+template void launch_minres_upd<float>(int, const MinresUpdArgs<float>&, int, int, hipStream_t, hipEvent_t, hipEvent_t);
 #endif
 
 }  // namespace sem

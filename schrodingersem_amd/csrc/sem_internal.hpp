@@ -224,8 +224,10 @@ struct PcgOpArgs {
 // (column-major, N*N values) copied into the kernel-argument segment.
 template <typename T>
 void launch_element_op_d(int N, int kind, const OpArgs<T>& a, const T* d, hipStream_t s);
+// e0/e1 (optional): events that take the kernel's own start and end timestamps
 template <typename T>
-void launch_minres_op_d(int N, int kind, const MinresOpArgs<T>& a, const T* d, int cpb, int grid, hipStream_t s);
+void launch_minres_op_d(int N, int kind, const MinresOpArgs<T>& a, const T* d, int cpb, int grid, hipStream_t s,
+                        hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
 template <typename T>
 void launch_pcg_op_d(int N, const PcgOpArgs<T>& a, int nchunks, const T* d, hipStream_t s);
 // beta recurrence closing a batch of Lanczos steps, one workgroup per instance
@@ -234,6 +236,7 @@ void launch_minres_close(int kind, const InstTable& it, const KrylovScalars* in,
                          const double* partial, hipStream_t s);
 // update kernel of a Lanczos step
 template <typename T>
-void launch_minres_upd(int N, const MinresUpdArgs<T>& a, int cpb, int grid, hipStream_t s);
+void launch_minres_upd(int N, const MinresUpdArgs<T>& a, int cpb, int grid, hipStream_t s, hipEvent_t e0 = nullptr,
+                       hipEvent_t e1 = nullptr);
 
 }  // namespace sem
