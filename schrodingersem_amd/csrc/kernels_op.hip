@@ -240,6 +240,38 @@ __global__ __launch_bounds__(64) void k_element_op(OpArgs<T> a, Dm<N, T> D)
 }
 
 
+// Workgroups are handed to the 8 XCDs round robin by workgroup index, and
+// each XCD has its own L2. This maps workgroup b to a work item so that every
+// XCD gets one contiguous range of items: neighbouring elements (whose ring
+// values the update kernel gathers) then share an L2. A bijection on [0, n).
+__device__ __forceinline__ int xcd_item(int b, int n)
+{
+    constexpr int nx = 8;
+    const int q = n / nx, r = n % nx;
+    const int x = b % nx, k = b / nx;
+    return (x < r) ? x * (q + 1) + k : r * (q + 1) + (x - r) * q + k;
+}
+
+// Persistent form of the same idea: XCD x owns the contiguous item range
+// that xcd_item gives it, and its workgroups stride through that range.
+// Grids smaller than 8 fall back to a plain grid stride.
+template <typename F>
+__device__ __forceinline__ void for_items_persistent(int n, F&& body)
+{
+    constexpr int nx = 8;
+    const int G = gridDim.x, b = blockIdx.x;
+    if (G < nx) {
+        for (int it = b; it < n; it += G) body(it);
+        return;
+    }
+    const int q = n / nx, r = n % nx;
+    const int x = b % nx, k = b / nx;
+    const int gx = (G - x + nx - 1) / nx;  // workgroups on XCD x
+    const int lo = (x < r) ? x * (q + 1) : r * (q + 1) + (x - r) * q;
+    const int len = q + (x < r ? 1 : 0);
+    for (int it = lo + k; it < lo + len; it += gx) body(it);
+}
+
 // owner test of entry (i, l) of an element with ownership bits (bit s-1:
 // side s owned; bit 4+k: corner k owned)
 template <int N>
@@ -269,7 +301,7 @@ __device__ __forceinline__ bool owns(int i, int l, unsigned bits)
 // where forcing occupancy costs more in register spills than it gains.
 template <int N, typename T, int KIND, int CPB, bool PERSIST>
 __global__ __launch_bounds__(64 * CPB * ((KIND == K_RES) ? 2 : 1))
-__attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 5 : 1, 8))) void k_minres_op(MinresOpArgs<T> a, Dm<N, T> D)
+__attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 1, 8))) void k_minres_op(MinresOpArgs<T> a, Dm<N, T> D)
 {
     using S = Shape<N>;
     constexpr int NC = (KIND == K_RES) ? 2 : 1;
@@ -364,9 +396,9 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 5 : 1, 8))) void k_minres_op
         __syncthreads();
     };
     if constexpr (PERSIST) {
-        for (int item = blockIdx.x; item < n_items; item += gridDim.x) body(item);
+        for_items_persistent(n_items, body);
     } else if ((int)blockIdx.x < n_items) {
-        body(blockIdx.x);
+        body(xcd_item(blockIdx.x, n_items));
     }
 }
 
@@ -533,9 +565,9 @@ __global__ __launch_bounds__(256 * CPB) void k_minres_upd(MinresUpdArgs<T> a)
         __syncthreads();
     };
     if constexpr (PERSIST) {
-        for (int item = blockIdx.x; item < n_items; item += gridDim.x) body(item);
+        for_items_persistent(n_items, body);
     } else if ((int)blockIdx.x < n_items) {
-        body(blockIdx.x);
+        body(xcd_item(blockIdx.x, n_items));
     }
 }
 

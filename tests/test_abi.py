@@ -14,7 +14,8 @@ HEADER = os.path.join(ROOT, "include", "sem.h")
 
 
 def declared_functions():
-    txt = open(HEADER).read()
+    with open(HEADER) as fh:
+        txt = fh.read()
     txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
     names = set(re.findall(r"\b(sem_[a-z0-9_]+)\s*\(", txt))
     return sorted(n for n in names if not n.endswith("_cb"))

@@ -44,7 +44,7 @@ def torch_loss(p, X, T, dims, in_pad, out_pad, off):
     t = torch.tensor(T.astype(np.float64))
     loss = ((y[:, :n_out] - t[:, :n_out]) ** 2).mean()
     loss.backward()
-    return float(loss), pt.grad.detach().numpy(), y.detach().numpy()
+    return float(loss.detach()), pt.grad.detach().numpy(), y.detach().numpy()
 
 
 def test_forward_and_gradients_match_autograd():
