@@ -565,19 +565,20 @@ static Results minres_fused(sem_mesh* m, int kind, const T* cf, double alpha, co
             m->ev.push_back(e);
         }
         // algorithmic bytes of one launch: the operator apply (its inputs, geometry
-        // and outputs, each read or written once, as the reference stores them)
-        // plus r1 read for y -= cr1*r1, per component
+        // and outputs, each read or written once, as the reference stores them),
+        // r1 read for y -= cr1*r1, and the w/x update (wa, wb, x in; w, x out),
+        // per component
         const double EN2 = (double)m->L.E * m->L.N * m->L.N;
         const double op_vals = (kind == K_RES) ? 9 : (kind == K_HELMHOLTZ ? 7 : 5);
-        m->kt_bytes[ti] = (op_vals + 1.0 * nc) * sizeof(T) * EN2;
-        // update kernel: yraw, r2 in and y out, then r1, wa, wb, x in and wn, x out, per component
-        m->kt_bytes[2 + ti] = 9.0 * nc * sizeof(T) * EN2;
+        m->kt_bytes[ti] = (op_vals + 6.0 * nc) * sizeof(T) * EN2;
+        // update kernel: yraw and r2 in, y out, per component
+        m->kt_bytes[2 + ti] = 3.0 * nc * sizeof(T) * EN2;
     }
-    // Scalar state: the operator kernel of step i > 1 (not first in its batch)
-    // applies the beta recurrence of step i-1 in every workgroup and stores
-    // it in the other of the two buffers; the update kernel of step i reads
-    // that buffer and stores alpha there. A closing kernel ends each batch
-    // with the state back in m->sc.
+    // Scalar state: the operator kernel of step i (when not first in its
+    // batch) applies the beta recurrence of step i-1 in every workgroup, then
+    // the pending w/x update, and stores the state in the other of the two
+    // buffers; the update kernel of step i reads that buffer and stores alpha
+    // there. A closing kernel ends each batch with the state back in m->sc.
     KrylovScalars* buf[2] = {m->sc, m->sc_alt};
     const InstTable itab = m->it();
     auto max_iter = [&]() {
@@ -603,25 +604,22 @@ static Results minres_fused(sem_mesh* m, int kind, const T* cf, double alpha, co
         a.nchunks = nch;
         a.obits = m->obits;
         a.partial = m->partial;
+        a.wa = W[i % 3];
+        a.wb = W[(i + 1) % 3];
+        a.wn = W[(i + 2) % 3];
+        a.x = x;
         a.it = itab;
         a.st_in = buf[cur];
         a.first = (b_ == 0);
-        if (b_ > 0) {
-            cur ^= 1;
-            a.st_out = buf[cur];
-            a.prev_partial = m->partial + nch;
-        }
+        cur ^= 1;
+        a.st_out = buf[cur];
+        a.prev_partial = m->partial + nch;
         launch_minres_op_d<T>(m->L.N, kind, a, m->dmat<T>(), shp.op_cpb, shp.op_grid, s, tim ? m->ev[4 * b_] : nullptr,
                               tim ? m->ev[4 * b_ + 1] : nullptr);
         MinresUpdArgs<T> u{};
         u.yraw = Yraw;
         u.y = R[i % 3];
         u.r2 = R[(i - 1) % 3];
-        u.r1 = R[(i + 1) % 3];
-        u.wa = W[i % 3];
-        u.wb = W[(i + 1) % 3];
-        u.wn = W[(i + 2) % 3];
-        u.x = x;
         u.len = m->len();
         u.ncomp = nc;
         u.nchunks = nch;

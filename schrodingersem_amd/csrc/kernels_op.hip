@@ -320,6 +320,7 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 1, 8))) void k_minres_op
     const int n_items = a.nchunks / CPB;
     int cur_inst = -1;
     MinresState sc;
+    bool pend = false;
     auto body = [&](int item) {
         const int c0 = item * CPB;
         const int4 ci = chunk_instance(a.it, c0);
@@ -332,13 +333,17 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 1, 8))) void k_minres_op
                 const int p0 = ci.y;
                 const double s2 = group_sum(a.prev_partial + p0, ci.z - p0, red, NW);
                 if (!sc.done) minres_beta_update(sc, s2);
-                else sc.p3_pending = 0;  // the update kernel of the stopped step applied it
-                if (c0 == p0 && threadIdx.x == 0) *static_cast<MinresState*>(a.st_out + inst) = sc;
+                else sc.p3_pending = 0;  // applied by the kernel of the stopping step
             }
+            // this kernel applies the pending w/x update; the state it stores drops it
+            pend = sc.p3_pending != 0;
+            sc.p3_pending = 0;
+            if (c0 == ci.y && threadIdx.x == 0) *static_cast<MinresState*>(a.st_out + inst) = sc;
         }
         const int chunk = c0 + sub;
         const Col<N, T> c{(size_t)chunk * S::K * N * N + lane, act};
         double part = 0;
+        T r[N];  // r1 of this wave's component
         if (!sc.done) {
             const T ib = (T)sc.ib, cr1 = (T)sc.cr1;
             const int has_r1 = sc.has_r1;
@@ -376,14 +381,32 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 1, 8))) void k_minres_op
 #pragma unroll
                 for (int i = 0; i < N; ++i) qm[i] = qs[i];
             }
-            T r[N];
-            if (has_r1) c.ld(a.r1 + w * len, r);
+            if (has_r1 || pend) c.ld(a.r1 + w * len, r);
 #pragma unroll
             for (int i = 0; i < N; ++i) {
                 if (has_r1 && owns<N>(i, l, bits)) y[i] = y[i] - cr1 * r[i];
                 part += (double)qm[i] * (double)y[i];
             }
             c.st(a.y + w * len, y);
+        } else if (pend) {
+            c.ld(a.r1 + w * len, r);
+        }
+        if (pend) {
+            // pending w/x update of the previous step on this component
+            // (minres.hpp:82-86): w_new = ig*((ibp*r1 - eo*wa) - dl*wb), x += phi*w_new
+            const T ibp = (T)sc.p3_ibp, ig = (T)sc.p3_ig, eo = (T)sc.p3_eo, dl = (T)sc.p3_dl, phi = (T)sc.p3_phi;
+            T wa[N], wb[N], x[N], wn[N];
+            c.ld(a.wa + w * len, wa);
+            c.ld(a.wb + w * len, wb);
+            c.ld(a.x + w * len, x);
+#pragma unroll
+            for (int i = 0; i < N; ++i) {
+                const T v = ibp * r[i];
+                wn[i] = ig * ((v - eo * wa[i]) - dl * wb[i]);
+                x[i] = x[i] + phi * wn[i];
+            }
+            c.st(a.wn + w * len, wn);
+            c.st(a.x + w * len, x);
         }
         part = wave_sum(part);
         if (lane == 0) wpart[wv] = part;
@@ -432,27 +455,23 @@ void launch_minres_close(int kind, const InstTable& it, const KrylovScalars* in,
 #endif
 
 
-// Second kernel of a Lanczos step: work items of CPB chunks, 2*NC waves per
-// chunk. Waves 0..NC-1 of a chunk finish component w of y: interface copies
-// gather the unassembled values of all copies of their node (edge: owner +
-// other, global_sum.hpp:28-34; corner: left fold from +0.0 in
-// connected_elements order, global_sum.hpp:54-63), so every copy computes the
-// same sum bit for bit; Dirichlet entries become zero; then y -= c2*r2
-// (minres.hpp:55) and the beta^2 partial with unique-DOF weights
-// (minres.hpp:63). Waves NC..2NC-1 run the pending w/x update of the previous
-// step on component w - NC (w_new = ig*((ibp*r1 - eo*wa) - dl*wb),
-// x += phi*w_new; minres.hpp:82-86).
+// Second kernel of a Lanczos step: work items of CPB chunks, one wave per
+// chunk and component. It finishes component w of y: interface copies gather
+// the unassembled values of all copies of their node (edge: owner + other,
+// global_sum.hpp:28-34; corner: left fold from +0.0 in connected_elements
+// order, global_sum.hpp:54-63), so every copy computes the same sum bit for
+// bit; Dirichlet entries become zero; then y -= c2*r2 (minres.hpp:55) and the
+// beta^2 partial with unique-DOF weights (minres.hpp:63).
 template <int N, typename T, int CPB, bool PERSIST>
-__global__ __launch_bounds__(256 * CPB) void k_minres_upd(MinresUpdArgs<T> a)
+__global__ __launch_bounds__(128 * CPB) void k_minres_upd(MinresUpdArgs<T> a)
 {
     using S = Shape<N>;
     constexpr int K = S::K, KN = S::KN;
-    __shared__ double wpart[4 * CPB];
+    __shared__ double wpart[2 * CPB];
     __shared__ double red[4];
     const int nc = a.ncomp;
-    const int wpc = 2 * nc;  // waves per chunk
     const int wall = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int sub = wall / wpc, wv = wall % wpc;
+    const int sub = wall / nc, wv = wall % nc;
     const bool act = lane < KN;
     const size_t len = a.len;
     const int n_items = a.nchunks / CPB;
@@ -463,14 +482,14 @@ __global__ __launch_bounds__(256 * CPB) void k_minres_upd(MinresUpdArgs<T> a)
         const int4 ci = chunk_instance(a.it, c0);
         const int inst = ci.x;
         const KrylovScalars* sc = a.st + inst;
-        const int done = sc->done, p3 = sc->p3_pending;
+        const int done = sc->done;
         if (inst != cur_inst) {
             cur_inst = inst;
             // alpha = dot(v, y) from the operator kernel's partials (minres.hpp:54),
             // c2 = alpha / beta (minres.hpp:55); the instance's first workgroup keeps alpha
             if (!done) {
                 const int p0 = ci.y;
-                const double alpha = group_sum(a.op_partial + p0, ci.z - p0, red, wpc * CPB);
+                const double alpha = group_sum(a.op_partial + p0, ci.z - p0, red, nc * CPB);
                 c2d = alpha / sc->beta;
                 if (c0 == p0 && threadIdx.x == 0) a.st[inst].alpha = alpha;
             }
@@ -478,27 +497,7 @@ __global__ __launch_bounds__(256 * CPB) void k_minres_upd(MinresUpdArgs<T> a)
         const int chunk = c0 + sub;
         const size_t base = (size_t)chunk * K * N * N + lane;
         double part = 0;
-        if (act && wv >= nc && p3) {
-            const size_t o = (size_t)(wv - nc) * len + base;
-            const T ibp = (T)sc->p3_ibp, ig = (T)sc->p3_ig, eo = (T)sc->p3_eo, dl = (T)sc->p3_dl,
-                    phi = (T)sc->p3_phi;
-            T r[N], wa[N], wb[N], x[N];
-#pragma unroll
-            for (int i = 0; i < N; ++i) {
-                r[i] = a.r1[o + (size_t)i * KN];
-                wa[i] = a.wa[o + (size_t)i * KN];
-                wb[i] = a.wb[o + (size_t)i * KN];
-                x[i] = a.x[o + (size_t)i * KN];
-            }
-#pragma unroll
-            for (int i = 0; i < N; ++i) {
-                const T v = ibp * r[i];
-                const T wn = ig * ((v - eo * wa[i]) - dl * wb[i]);
-                a.wn[o + (size_t)i * KN] = wn;
-                a.x[o + (size_t)i * KN] = x[i] + phi * wn;
-            }
-        }
-        if (act && wv < nc && !done) {
+        if (act && !done) {
             const int el = lane / N, j = lane % N;
             const int e = chunk * K + el;
             const T c2 = (T)c2d;
@@ -558,8 +557,8 @@ __global__ __launch_bounds__(256 * CPB) void k_minres_upd(MinresUpdArgs<T> a)
         if (lane == 0) wpart[wall] = part;
         __syncthreads();
         if (threadIdx.x < CPB) {
-            double t = wpart[threadIdx.x * wpc];
-            if (nc == 2) t += wpart[threadIdx.x * wpc + 1];
+            double t = wpart[threadIdx.x * nc];
+            if (nc == 2) t += wpart[threadIdx.x * nc + 1];
             a.partial[c0 + threadIdx.x] = t;
         }
         __syncthreads();
@@ -697,7 +696,7 @@ template <int N, typename T, int CPB, bool P>
 static void minres_upd_NC(const MinresUpdArgs<T>& a, int grid, hipStream_t s, hipEvent_t e0, hipEvent_t e1)
 {
     const int items = a.nchunks / CPB;
-    dim3 g(P ? std::min(grid, items) : items), b(128 * a.ncomp * CPB);
+    dim3 g(P ? std::min(grid, items) : items), b(64 * a.ncomp * CPB);
     SEM_LAUNCH_TIMED((k_minres_upd<N, T, CPB, P>), g, b, s, e0, e1, a);
 }
 

@@ -157,6 +157,12 @@ struct MinresOpArgs {
     const T* r2;  // R[i-1]: v_i = ib * r2
     const T* r1;  // R[i-2]
     T* y;         // element results: final inside, ring copies unassembled (owner copy minus cr1*r1)
+    // pending w/x update of the previous step (minres.hpp:82-86):
+    // w_new = ig*((ibp*r1 - eo*wa) - dl*wb), x += phi*w_new
+    const T* wa;  // W[i-3]
+    const T* wb;  // W[i-2]
+    T* wn;        // W[i-1]
+    T* x;
     const T* geo;
     const T* cf;
     T alpha;
@@ -176,18 +182,12 @@ struct MinresOpArgs {
 };
 
 // Arguments of the MINRES update kernel: interface assembly by gather,
-// y -= c2*r2, beta^2 partial, and the pending w/x update of the previous step
-// (w_new = ig*((ibp*r1 - eo*wa) - dl*wb); x += phi*w_new, minres.hpp:82-86).
+// y -= c2*r2 and the beta^2 partial.
 template <typename T>
 struct MinresUpdArgs {
     const T* yraw;
     T* y;         // R[i]
     const T* r2;  // R[i-1]
-    const T* r1;  // R[i-2]
-    const T* wa;  // W[i-3]
-    const T* wb;  // W[i-2]
-    T* wn;        // W[i-1]
-    T* x;
     size_t len;
     int ncomp, nchunks;
     const uint8_t* obits;
