@@ -343,14 +343,16 @@ def main():
     shape = tuned["fp64" if best == "fp64" else "fp32"]["shape"]
     # HBM bytes per launch of the same kernel from the committed rocprofv3 PMC summary of this
     # workload (FETCH_SIZE and WRITE_SIZE, scaled by the copy-kernel calibration), when present
+    # (the work shape does not change the bytes a launch moves; the most dispatched variant is used)
     traffic = None
     prof = os.path.join(ROOT, "profiles", "hbm_traffic_dirk4s3_%dx%d_p%d.json" % (args.n, args.n, args.p))
     if os.path.exists(prof):
-        kname = "void sem::k_minres_op<%d, %s, 3, %d, %s>" % (args.p + 1, "double" if best == "fp64" else "float",
-                                                             shape[0], "true" if shape[1] else "false")
-        entry = json.load(open(prof))["kernels"].get(kname)
-        if entry:
-            traffic = round(entry["traffic_bytes"])
+        with open(prof) as fh:
+            kern = json.load(fh)["kernels"]
+        prefix = "void sem::k_minres_op<%d, %s, 3," % (args.p + 1, "double" if best == "fp64" else "float")
+        cands = [k for k in kern if k.startswith(prefix)]
+        if cands:
+            traffic = round(kern[cands[0]]["traffic_bytes"])
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "kernel": "k_minres_op<%d,%s,RES,%d,%s>" % (args.p + 1, "double" if best == "fp64" else "float",
