@@ -238,6 +238,13 @@ int sem_kernel_timing(const sem_mesh* m, int which, long* launches, double* tota
  * SEM_PREC_MIXED for the FP32 inner solver). Results do not depend on it. */
 int sem_set_step_shape(sem_mesh* m, int precision, const int* shape);
 int sem_get_step_shape(const sem_mesh* m, int precision, int* shape);
+/* MINRES step mode per precision (SEM_PREC_FP64, or SEM_PREC_MIXED for the
+ * FP32 inner solver): 0 two launches per Lanczos step; 1 one fused launch per
+ * step that applies the operator before the 1/beta scaling (same recurrence,
+ * results equal to rounding; graph mode, when on, takes precedence).
+ * Defaults: FP64 0, FP32 1. */
+int sem_set_step_mode(sem_mesh* m, int precision, int mode);
+int sem_get_step_mode(const sem_mesh* m, int precision, int* mode);
 /* Graph mode: with on = 1 the MINRES solver captures `steps` Lanczos steps
  * (a multiple of 3) plus the batch-closing kernel as one HIP graph per solve
  * configuration and replays it between host checks of the stop flags

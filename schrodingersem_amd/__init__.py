@@ -109,6 +109,8 @@ def lib():
             "sem_set_step_shape": (C.c_int, [C.c_void_p, C.c_int, _ip]),
             "sem_get_step_shape": (C.c_int, [C.c_void_p, C.c_int, _ip]),
             "sem_set_graph_mode": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
+            "sem_set_step_mode": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
+            "sem_get_step_mode": (C.c_int, [C.c_void_p, C.c_int, _ip]),
             "sem_batch_set_graph_mode": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
             "sem_autotune": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, _ip, _ip, _dp]),
             "sem_stepper_create": (C.c_int, [C.c_void_p, C.c_int, _dp, _dp, _dp, C.POINTER(_Opts),
@@ -570,6 +572,19 @@ def set_step_shape(mesh, shape, precision=PREC_FP64):
     """Work shape (op chunks/workgroup, op grid, update chunks/workgroup, update grid) of the Lanczos step."""
     arr = (C.c_int * 4)(*[int(v) for v in shape])
     _check(lib().sem_set_step_shape(mesh._h, int(precision), arr))
+
+
+def set_step_mode(mesh, mode, precision=None):
+    """MINRES Lanczos step: 0 two launches per step, 1 one fused launch per step
+    (precision PREC_FP64 or PREC_MIXED; None sets both)."""
+    for p in ((PREC_FP64, PREC_MIXED) if precision is None else (precision,)):
+        _check(lib().sem_set_step_mode(mesh._h, int(p), int(mode)))
+
+
+def get_step_mode(mesh, precision=PREC_FP64):
+    v = C.c_int()
+    _check(lib().sem_get_step_mode(mesh._h, int(precision), C.byref(v)))
+    return v.value
 
 
 def set_graph_mode(mesh, on, steps=12):

@@ -99,6 +99,10 @@ struct sem_mesh {
     KrylovScalars* sc_alt = nullptr;  // second state buffer of the Lanczos-step kernels
     KrylovScalars* h_sc = nullptr;    // pinned
     StepShape shape[2];               // Lanczos-step work shape (FP64, FP32)
+    // MINRES Lanczos step per precision (FP64, FP32): 0 two launches per step,
+    // 1 one fused launch (not with graphs); FP32 defaults to fused (faster on
+    // the DIRK4S3 workload), FP64 to two launches
+    int step_mode[2] = {0, 1};
     // captured Lanczos-step batches (graph path), most recent last
     bool use_graphs = false;
     int graph_steps = 12;
@@ -506,6 +510,170 @@ static double wdot_host(sem_mesh* m, const T* a, const T* b, int ncomp)
     return m->h_sc->red[7];
 }
 
+// ---------------------------------------------------------- fused MINRES --
+// minres.hpp:19-107 with one launch per Lanczos step (deferred scaling, see
+// MinresFusedArgs): step i finishes y_{i-1} from A y_{i-2}, applies A to it and
+// writes the sums its successor needs; the Givens update of iteration i-2 and
+// alpha_{i-1} are evaluated at the start of step i in every workgroup. Same
+// recurrence as the two-launch path; the operator is applied before the 1/beta
+// scaling instead of after, so results agree to rounding. Buffers: y_k in
+// Y[k % 3], A y in Z[step % 2], w_k in W[k % 3], sums in S[step % 2].
+template <typename T>
+static Results minres_fused1(sem_mesh* m, int kind, const T* cf, double alpha, const T* b, T* x, bool x_zero,
+                             const KrylovOpts& o)
+{
+    const int nc = op_ncomp(kind);
+    const size_t n = (size_t)nc * m->len();
+    const std::string tp = sizeof(T) == 8 ? "d" : "f";
+    // the Y and W slots are the two-launch path's R and W buffers
+    T* Y[3] = {m->buf<T>("mr_R0" + tp, n), m->buf<T>("mr_R1" + tp, n), m->buf<T>("mr_R2" + tp, n)};
+    T* W[3] = {m->buf<T>("mr_W0" + tp, n), m->buf<T>("mr_W1" + tp, n), m->buf<T>("mr_W2" + tp, n)};
+    T* Z[2] = {m->buf<T>("mr_Yraw" + tp, n), m->buf<T>("mr_Z1" + tp, n)};
+    const int nch = m->L.nchunks;
+    double* S[2] = {m->buf<double>("mr_S0", 3 * (size_t)nch), m->buf<double>("mr_S1", 3 * (size_t)nch)};
+    hipStream_t s = m->stream;
+    const int ni = m->n_inst;
+    if (x_zero) {
+        launch_fill<T>(x, T(0), n, s);
+        launch_cast<T, T>(Y[0], b, n, s);
+    } else {
+        T* Ax = m->buf<T>("mr_Ax" + tp, n);
+        apply_op<T>(m, kind, x, Ax, cf, alpha);
+        lincomb<T>(m, Y[0], 0, {{1.0, b}, {-1.0, Ax}}, n);  // y_0 = r1 = b - A(x)
+    }
+    for (int q = 0; q < 3; ++q) launch_fill<T>(W[q], T(0), n, s);
+    wdot_dev<T>(m, Y[0], Y[0], nc, 0);
+    launch_minres_init(m->sc, ni, 0, o.maxit, o.tol, s);
+    if (o.rel_to_b) wdot_dev<T>(m, b, b, nc, 1);
+    if (o.rel_to_b || o.atol != o.tol) {
+        m->read_scalars();
+        for (int k = 0; k < ni; ++k) {
+            if (o.rel_to_b) m->h_sc[k].b_norm = std::sqrt(m->h_sc[k].red[1]);
+            m->h_sc[k].atol = o.atol;
+        }
+        SEM_HIP(hipMemcpyAsync(m->sc, m->h_sc, ni * sizeof(KrylovScalars), hipMemcpyHostToDevice, s));
+    }
+    const int ti = sizeof(T) == 8 ? 0 : 1;
+    const StepShape shp = m->shape[ti];
+    const int slot = o.hint_slot >= 0 ? std::min(o.hint_slot, 7) : ti;
+    // two more launches than iterations: the stop test of iteration k is
+    // evaluated at the start of step k + 2
+    const int first_batch = std::max(m->batch, (int)(0.85 * m->iter_hint[slot]) + 2);
+    if (m->timing) {
+        while ((int)m->ev.size() < 4 * first_batch) {
+            hipEvent_t e;
+            SEM_HIP(hipEventCreate(&e));
+            m->ev.push_back(e);
+        }
+        // algorithmic bytes of one launch: finishing y (z in, y_{i-2}, y_{i-3} in,
+        // y out), the operator apply (its inputs and outputs) and the w/x update,
+        // per component
+        const double EN2 = (double)m->L.E * m->L.N * m->L.N;
+        const double geo_cf = (kind == K_RES) ? 5 : (kind == K_HELMHOLTZ ? 4 : 3);
+        m->kt_bytes[ti] = (geo_cf + 10.0 * nc) * sizeof(T) * EN2;
+    }
+    KrylovScalars* buf[2] = {m->sc, m->sc_alt};
+    const InstTable itab = m->it();
+    auto max_iter = [&]() {
+        int v = 0;
+        for (int k = 0; k < ni; ++k) v = std::max(v, m->h_sc[k].iter);
+        return v;
+    };
+    auto all_done = [&]() {
+        for (int k = 0; k < ni; ++k)
+            if (!m->h_sc[k].done) return false;
+        return true;
+    };
+    int i = 1;
+    bool done = o.maxit < 1;
+    while (!done) {
+        int launched = 0;
+        const int i0 = i;
+        int cur = 0;
+        const int nbatch = (i == 1) ? first_batch : m->batch;
+        for (int b_ = 0; b_ < nbatch && i <= o.maxit + 2; ++b_, ++i) {
+            MinresFusedArgs<T> a{};
+            a.z_in = Z[(i + 1) % 2];
+            a.z_out = Z[i % 2];
+            a.y2 = Y[(i + 1) % 3];
+            a.y3 = Y[i % 3];
+            a.y_out = Y[(i + 2) % 3];
+            a.wa = W[(i + 2) % 3];
+            a.wb = W[i % 3];
+            a.wn = W[(i + 1) % 3];
+            a.x = x;
+            a.geo = m->geo_t<T>();
+            a.cf = cf;
+            a.alpha = (T)alpha;
+            a.len = m->len();
+            a.nchunks = nch;
+            a.obits = m->obits;
+            a.eside = m->eside;
+            a.ecorner = m->ecorner;
+            a.c_off = m->c_off;
+            a.c_pos = m->c_pos;
+            a.sums = S[i % 2];
+            a.prev_sums = S[(i + 1) % 2];
+            a.it = itab;
+            a.st_in = buf[cur];
+            cur ^= 1;
+            a.st_out = buf[cur];
+            a.step = i;
+            a.first = (b_ == 0);
+            const bool tim = m->timing == 2 || (m->timing == 1 && i % 4 == 1);
+            launch_minres_fused<T>(m->L.N, kind, a, m->dmat<T>(), shp.op_cpb, shp.op_grid, s,
+                                   tim ? m->ev[4 * b_] : nullptr, tim ? m->ev[4 * b_ + 1] : nullptr);
+            ++launched;
+        }
+        // the batch ended after step i-1; close with what step i would compute first
+        launch_minres_fused_close(itab, buf[cur], m->sc, S[(i + 1) % 2], i, s);
+        m->read_scalars();
+        if (m->timing) {
+            const int last = max_iter() + 2;
+            for (int b_ = 0; b_ < launched && i0 + b_ <= last; ++b_) {
+                if (m->timing == 1 && (i0 + b_) % 4 != 1) continue;
+                float ms = 0;
+                SEM_HIP(hipEventElapsedTime(&ms, m->ev[4 * b_], m->ev[4 * b_ + 1]));
+                m->kt_us[ti] += 1000.0 * ms;
+                m->kt_launches[ti] += 1;
+            }
+        }
+        done = all_done() || i > o.maxit + 2;
+    }
+    bool pending = false;
+    for (int k = 0; k < ni; ++k) pending |= m->h_sc[k].p3_pending != 0;
+    if (pending) {
+        // iteration k = iter of each instance: v = y_{k-1}, w_{k-2}, w_{k-1} -> w_k;
+        // the flush kernel's rotation (R[(k+2)%3], W[(k+1)%3], W[(k+2)%3] -> W[k%3]) matches
+        MinresFlushArgs<T> fa{};
+        for (int q = 0; q < 3; ++q) {
+            fa.R[q] = Y[q];
+            fa.W[q] = W[q];
+        }
+        fa.x = x;
+        fa.len = m->len();
+        fa.chunk_len = m->L.K * m->L.N * m->L.N;
+        fa.ncomp = nc;
+        launch_minres_flush<T>(fa, itab, nch, m->sc, s);
+    }
+    for (int k = 0; k < ni; ++k) {
+        Results& r = m->inst_results[k];
+        const KrylovScalars& h = m->h_sc[k];
+        r.success = h.success;
+        r.n_iter = r.success ? h.iter : o.maxit + 1;
+        if (o.maxit < 1) r.n_iter = 1;
+        r.residual = h.phibar;
+    }
+    m->iter_hint[slot] = max_iter();
+    Results all = m->inst_results[0];
+    for (int k = 1; k < ni; ++k) {
+        all.success = all.success && m->inst_results[k].success;
+        all.n_iter = std::max(all.n_iter, m->inst_results[k].n_iter);
+        all.residual = std::max(all.residual, m->inst_results[k].residual);
+    }
+    return all;
+}
+
 // ---------------------------------------------------------------- MINRES --
 // minres.hpp:19-107 with IdentityPreconditioner, two fused launches per
 // Lanczos step (the scalar recurrence runs inside them). Vectors are T (FP64 or
@@ -518,6 +686,8 @@ template <typename T>
 static Results minres_fused(sem_mesh* m, int kind, const T* cf, double alpha, const T* b, T* x, bool x_zero,
                             const KrylovOpts& o)
 {
+    if (m->step_mode[sizeof(T) == 8 ? 0 : 1] == 1 && !m->use_graphs)
+        return minres_fused1<T>(m, kind, cf, alpha, b, x, x_zero, o);
     const int nc = op_ncomp(kind);
     const size_t n = (size_t)nc * m->len();
     const std::string tp = sizeof(T) == 8 ? "d" : "f";
@@ -796,13 +966,15 @@ static void autotune_steps(sem_mesh* m, int kind, int steps, std::vector<double>
         }
     StepShape best = keep;
     double b_op = 1e300, b_up = 1e300;
+    // in fused mode only the operator (fused) kernel runs; the update shape stays
+    const bool fused = m->step_mode[ti] == 1 && !m->use_graphs;
     for (size_t c = 0; c < cand.size(); ++c) {
         if (us[2 * c] < b_op) {
             b_op = us[2 * c];
             best.op_cpb = cand[c].first;
             best.op_grid = cand[c].second;
         }
-        if (us[2 * c + 1] < b_up) {
+        if (!fused && us[2 * c + 1] < b_up) {
             b_up = us[2 * c + 1];
             best.upd_cpb = cand[c].first;
             best.upd_grid = cand[c].second;
@@ -2346,6 +2518,21 @@ int sem_set_step_shape(sem_mesh* m, int precision, const int* shape)
     for (int q = 0; q < 4; q += 2)
         if ((shape[q] != 1 && shape[q] != 2) || shape[q + 1] < 0) return SEM_EINVAL;
     m->shape[precision == SEM_PREC_FP64 ? 0 : 1] = StepShape{shape[0], shape[1], shape[2], shape[3]};
+    return SEM_OK;
+}
+
+int sem_set_step_mode(sem_mesh* m, int precision, int mode)
+{
+    if (!m || (mode != 0 && mode != 1) || (precision != SEM_PREC_FP64 && precision != SEM_PREC_MIXED))
+        return SEM_EINVAL;
+    m->step_mode[precision == SEM_PREC_FP64 ? 0 : 1] = mode;
+    return SEM_OK;
+}
+
+int sem_get_step_mode(const sem_mesh* m, int precision, int* mode)
+{
+    if (!m || !mode || (precision != SEM_PREC_FP64 && precision != SEM_PREC_MIXED)) return SEM_EINVAL;
+    *mode = m->step_mode[precision == SEM_PREC_FP64 ? 0 : 1];
     return SEM_OK;
 }
 

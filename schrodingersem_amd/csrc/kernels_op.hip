@@ -455,6 +455,60 @@ void launch_minres_close(int kind, const InstTable& it, const KrylovScalars* in,
 #endif
 
 
+// Assembled values of column j of element e (lane position base of the
+// chunk's first plane) from element-local values yr: interior planes are the
+// element's own; ring entries gather every copy of their node in the
+// reference's order (edge: owner + other, global_sum.hpp:28-34; corner: left
+// fold from +0.0 in connected_elements order, global_sum.hpp:54-63), so every
+// copy gets the same bits; Dirichlet entries are zero.
+template <int N, typename T>
+__device__ __forceinline__ void gather_column(const T* __restrict__ yr, size_t base, int e, int j,
+                                              const int* __restrict__ eside, const int* __restrict__ ecorner,
+                                              const int* __restrict__ c_off, const int* __restrict__ c_pos, T (&v)[N])
+{
+    using S = Shape<N>;
+    constexpr int K = S::K, KN = S::KN;
+    auto side_pos = [&](int d, int k) -> long {
+        const int e2 = d >> 3, s2 = ((d >> 1) & 3) + 1;
+        const int k2 = (d & 1) ? N - 1 - k : k;
+        int i2, j2;
+        if (s2 == 1) { i2 = k2; j2 = 0; }
+        else if (s2 == 2) { i2 = N - 1; j2 = k2; }
+        else if (s2 == 3) { i2 = k2; j2 = N - 1; }
+        else { i2 = 0; j2 = k2; }
+        return (long)(e2 / K) * K * N * N + (long)i2 * KN + (long)(e2 % K) * N + j2;
+    };
+#pragma unroll
+    for (int i = 0; i < N; ++i) v[i] = yr[base + (size_t)i * KN];
+    if (j == 0 || j == N - 1) {
+        // column on side 1 (j = 0) or side 3 (j = N-1): planes 1..N-2; corners at planes 0, N-1
+        const int dcol = eside[4 * e + (j == 0 ? 0 : 2)];
+        const int c_lo = ecorner[4 * e + (j == 0 ? 0 : 3)];
+        const int c_hi = ecorner[4 * e + (j == 0 ? 1 : 2)];
+        T pv[N];
+        if (dcol >= 0) {
+#pragma unroll
+            for (int i = 1; i < N - 1; ++i) pv[i] = yr[side_pos(dcol, i)];
+        }
+        T s0 = 0, s1 = 0;
+        if (c_lo >= 0)
+            for (int q = c_off[c_lo]; q < c_off[c_lo + 1]; ++q) s0 += yr[c_pos[q]];
+        if (c_hi >= 0)
+            for (int q = c_off[c_hi]; q < c_off[c_hi + 1]; ++q) s1 += yr[c_pos[q]];
+#pragma unroll
+        for (int i = 1; i < N - 1; ++i) v[i] = (dcol >= 0) ? v[i] + pv[i] : T(0);
+        v[0] = s0;
+        v[N - 1] = s1;
+    } else {
+        // plane 0 is side 4, plane N-1 is side 2
+        const int d4 = eside[4 * e + 3], d2 = eside[4 * e + 1];
+        const T p0 = (d4 >= 0) ? yr[side_pos(d4, j)] : T(0);
+        const T p1 = (d2 >= 0) ? yr[side_pos(d2, j)] : T(0);
+        v[0] = (d4 >= 0) ? v[0] + p0 : T(0);
+        v[N - 1] = (d2 >= 0) ? v[N - 1] + p1 : T(0);
+    }
+}
+
 // Second kernel of a Lanczos step: work items of CPB chunks, one wave per
 // chunk and component. It finishes component w of y: interface copies gather
 // the unassembled values of all copies of their node (edge: owner + other,
@@ -502,50 +556,10 @@ __global__ __launch_bounds__(128 * CPB) void k_minres_upd(MinresUpdArgs<T> a)
             const int e = chunk * K + el;
             const T c2 = (T)c2d;
             const unsigned bits = a.obits[e];
-            const T* yr = a.yraw + (size_t)wv * len;
-            auto side_pos = [&](int d, int k) -> long {
-                const int e2 = d >> 3, s2 = ((d >> 1) & 3) + 1;
-                const int k2 = (d & 1) ? N - 1 - k : k;
-                int i2, j2;
-                if (s2 == 1) { i2 = k2; j2 = 0; }
-                else if (s2 == 2) { i2 = N - 1; j2 = k2; }
-                else if (s2 == 3) { i2 = k2; j2 = N - 1; }
-                else { i2 = 0; j2 = k2; }
-                return (long)(e2 / K) * K * N * N + (long)i2 * KN + (long)(e2 % K) * N + j2;
-            };
             T v[N], r[N];
+            gather_column<N>(a.yraw + (size_t)wv * len, base, e, j, a.eside, a.ecorner, a.c_off, a.c_pos, v);
 #pragma unroll
-            for (int i = 0; i < N; ++i) {
-                v[i] = yr[base + (size_t)i * KN];
-                r[i] = a.r2[(size_t)wv * len + base + (size_t)i * KN];
-            }
-            if (j == 0 || j == N - 1) {
-                // column on side 1 (j = 0) or side 3 (j = N-1): planes 1..N-2; corners at planes 0, N-1
-                const int dcol = a.eside[4 * e + (j == 0 ? 0 : 2)];
-                const int c_lo = a.ecorner[4 * e + (j == 0 ? 0 : 3)];
-                const int c_hi = a.ecorner[4 * e + (j == 0 ? 1 : 2)];
-                T pv[N];
-                if (dcol >= 0) {
-#pragma unroll
-                    for (int i = 1; i < N - 1; ++i) pv[i] = yr[side_pos(dcol, i)];
-                }
-                T s0 = 0, s1 = 0;
-                if (c_lo >= 0)
-                    for (int q = a.c_off[c_lo]; q < a.c_off[c_lo + 1]; ++q) s0 += yr[a.c_pos[q]];
-                if (c_hi >= 0)
-                    for (int q = a.c_off[c_hi]; q < a.c_off[c_hi + 1]; ++q) s1 += yr[a.c_pos[q]];
-#pragma unroll
-                for (int i = 1; i < N - 1; ++i) v[i] = (dcol >= 0) ? v[i] + pv[i] : T(0);
-                v[0] = s0;
-                v[N - 1] = s1;
-            } else {
-                // plane 0 is side 4, plane N-1 is side 2
-                const int d4 = a.eside[4 * e + 3], d2 = a.eside[4 * e + 1];
-                const T p0 = (d4 >= 0) ? yr[side_pos(d4, j)] : T(0);
-                const T p1 = (d2 >= 0) ? yr[side_pos(d2, j)] : T(0);
-                v[0] = (d4 >= 0) ? v[0] + p0 : T(0);
-                v[N - 1] = (d2 >= 0) ? v[N - 1] + p1 : T(0);
-            }
+            for (int i = 0; i < N; ++i) r[i] = a.r2[(size_t)wv * len + base + (size_t)i * KN];
 #pragma unroll
             for (int i = 0; i < N; ++i) {
                 const T yv = v[i] - c2 * r[i];
@@ -569,6 +583,189 @@ __global__ __launch_bounds__(128 * CPB) void k_minres_upd(MinresUpdArgs<T> a)
         body(xcd_item(blockIdx.x, n_items));
     }
 }
+
+// Fused Lanczos step (one launch per step, deferred scaling; see
+// MinresFusedArgs). Work items of CPB chunks, one wave per chunk and output
+// component. Every workgroup first evaluates the recurrence of this step
+// (minres_fused_prologue) from the previous step's per-chunk sums; the
+// workgroup holding an instance's first chunk stores the state in the other
+// buffer.
+template <int N, typename T, int KIND, int CPB, bool PERSIST>
+__global__ __launch_bounds__(64 * CPB * ((KIND == K_RES) ? 2 : 1))
+__attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 1, 8))) void k_minres_fused(MinresFusedArgs<T> a, Dm<N, T> D)
+{
+    using S = Shape<N>;
+    constexpr int NC = (KIND == K_RES) ? 2 : 1;
+    constexpr int NW = NC * CPB;
+    __shared__ T lds[NW][2][S::LDS];
+    __shared__ T xch[NC == 2 ? NW : 1][64 * N];  // component exchange (Res)
+    __shared__ double wpart[3][NW];
+    __shared__ double red[12];
+    const int wv = threadIdx.x >> 6;
+    const int w = wv % NC;    // output component of this wave
+    const int sub = wv / NC;  // chunk of the work item
+    const int lane = threadIdx.x & 63;
+    const bool act = lane < S::KN;
+    const int el = act ? lane / N : 0;
+    const int l = act ? lane % N : 0;
+    const size_t len = a.len;
+    const int nch = a.nchunks;
+    const int n_items = nch / CPB;
+    int cur_inst = -1;
+    MinresState sc;
+    bool pend = false;
+    auto body = [&](int item) {
+        const int c0 = item * CPB;
+        const int4 ci = chunk_instance(a.it, c0);
+        const int inst = ci.x;
+        if (inst != cur_inst) {
+            cur_inst = inst;
+            sc = *static_cast<const MinresState*>(a.st_in + inst);
+            if (!a.first && a.step >= 2) {
+                double P[3];
+                group_sum3(a.prev_sums + ci.y, nch, ci.z - ci.y, red, NW, P);
+                minres_fused_prologue(sc, a.step, P);
+            }
+            // this kernel applies the pending w/x update; the state it stores drops it
+            pend = sc.p3_pending != 0;
+            sc.p3_pending = 0;
+            if (c0 == ci.y && threadIdx.x == 0) *static_cast<MinresState*>(a.st_out + inst) = sc;
+        }
+        const int chunk = c0 + sub;
+        const size_t base = (size_t)chunk * S::K * N * N + lane;
+        const Col<N, T> c{base, act};
+        const unsigned bits = act ? a.obits[chunk * S::K + el] : 0u;
+        double p0 = 0, p1 = 0, p2 = 0;
+        T r3[N];  // y_{i-3} of this component
+        const bool has_r1 = sc.has_r1 != 0;
+        if (!sc.done) {
+            T yc[N];  // y_{i-1} of this component
+            if (a.step == 1) {
+                c.ld(a.y_out + w * len, yc);
+            } else {
+                const T ib = (T)sc.ib, cr1 = (T)sc.cr1, c2 = (T)(sc.alpha * sc.ib);
+                T v[N], r2[N];
+                if (act) gather_column<N>(a.z_in + w * len, base, chunk * S::K + el, l, a.eside, a.ecorner, a.c_off,
+                                          a.c_pos, v);
+                c.ld(a.y2 + w * len, r2);
+                if (has_r1) c.ld(a.y3 + w * len, r3);
+#pragma unroll
+                for (int i = 0; i < N; ++i) {
+                    T t = act ? v[i] * ib : T(0);
+                    if (has_r1) t = t - cr1 * r3[i];
+                    yc[i] = t - c2 * r2[i];
+                    if (owns<N>(i, l, bits)) p2 += (double)yc[i] * (double)r2[i];
+                }
+                c.st(a.y_out + w * len, yc);
+            }
+#pragma unroll
+            for (int i = 0; i < N; ++i)
+                if (owns<N>(i, l, bits)) p0 += (double)yc[i] * (double)yc[i];
+            // stiffness input: the other component for Res, the only one otherwise
+            T qs[N];
+            if constexpr (NC == 2) {
+                T* mine = xch[wv];
+                T* other = xch[sub * NC + (1 - w)];
+#pragma unroll
+                for (int i = 0; i < N; ++i) mine[i * 64 + lane] = yc[i];
+                __syncthreads();
+#pragma unroll
+                for (int i = 0; i < N; ++i) qs[i] = other[i * 64 + lane];
+                __syncthreads();
+            } else {
+#pragma unroll
+                for (int i = 0; i < N; ++i) qs[i] = yc[i];
+            }
+            T ga[N], gb[N], gc[N], z[N];
+            c.ld(a.geo, ga);
+            c.ld(a.geo + len, gb);
+            c.ld(a.geo + 2 * len, gc);
+            stiffness<N>(D, lds[wv][0], lds[wv][1], el, l, act, qs, ga, gb, gc, z);
+            if constexpr (KIND == K_RES) {
+                T cf[N], gj[N];
+                c.ld(a.cf, cf);
+                c.ld(a.geo + 3 * len, gj);
+#pragma unroll
+                for (int i = 0; i < N; ++i) {
+                    const T t = a.alpha * (z[i] + cf[i] * qs[i]);
+                    z[i] = (w == 0) ? gj[i] * yc[i] + t : t - gj[i] * yc[i];
+                }
+            } else if constexpr (KIND == K_HELMHOLTZ) {
+                T cf[N];
+                c.ld(a.cf, cf);
+#pragma unroll
+                for (int i = 0; i < N; ++i) z[i] = cf[i] * qs[i] + z[i];
+            }
+#pragma unroll
+            for (int i = 0; i < N; ++i) p1 += (double)yc[i] * (double)z[i];
+            c.st(a.z_out + w * len, z);
+        }
+        if (pend) {
+            // pending w/x update of iteration i-2 (minres.hpp:82-86) with
+            // v = y_{i-3}/beta_{i-2}: w_new = ig*((v - eo*wa) - dl*wb), x += phi*w_new
+            if (sc.done || !has_r1) c.ld(a.y3 + w * len, r3);
+            const T ibp = (T)sc.p3_ibp, ig = (T)sc.p3_ig, eo = (T)sc.p3_eo, dl = (T)sc.p3_dl, phi = (T)sc.p3_phi;
+            T wa[N], wb[N], x[N], wn[N];
+            c.ld(a.wa + w * len, wa);
+            c.ld(a.wb + w * len, wb);
+            c.ld(a.x + w * len, x);
+#pragma unroll
+            for (int i = 0; i < N; ++i) {
+                const T v = ibp * r3[i];
+                wn[i] = ig * ((v - eo * wa[i]) - dl * wb[i]);
+                x[i] = x[i] + phi * wn[i];
+            }
+            c.st(a.wn + w * len, wn);
+            c.st(a.x + w * len, x);
+        }
+        p0 = wave_sum(p0);
+        p1 = wave_sum(p1);
+        p2 = wave_sum(p2);
+        if (lane == 0) {
+            wpart[0][wv] = p0;
+            wpart[1][wv] = p1;
+            wpart[2][wv] = p2;
+        }
+        __syncthreads();
+        if (threadIdx.x < 3 * CPB) {
+            const int q = threadIdx.x / CPB, ch = threadIdx.x % CPB;
+            double t = wpart[q][ch * NC];
+            if constexpr (NC == 2) t += wpart[q][ch * NC + 1];
+            a.sums[(size_t)q * nch + c0 + ch] = t;
+        }
+        __syncthreads();
+    };
+    if constexpr (PERSIST) {
+        for_items_persistent(n_items, body);
+    } else if ((int)blockIdx.x < n_items) {
+        body(xcd_item(blockIdx.x, n_items));
+    }
+}
+
+#ifndef SEM_OP_FP32
+// Closes a batch of fused steps: the recurrence step `step` would evaluate at
+// its start, with the same reduction; one workgroup per instance.
+__global__ __launch_bounds__(256) void k_minres_fused_close(InstTable it, const KrylovScalars* in, KrylovScalars* out,
+                                                            const double* sums, int nch, int step)
+{
+    __shared__ double red[12];
+    const int k = blockIdx.x;
+    const int p0 = it.inst_off[k];
+    double P[3];
+    group_sum3(sums + p0, nch, it.inst_off[k + 1] - p0, red, 4, P);
+    if (threadIdx.x == 0) {
+        MinresState sc = *static_cast<const MinresState*>(in + k);
+        minres_fused_prologue(sc, step, P);
+        *static_cast<MinresState*>(out + k) = sc;
+    }
+}
+
+void launch_minres_fused_close(const InstTable& it, const KrylovScalars* in, KrylovScalars* out, const double* sums,
+                               int step, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_minres_fused_close, dim3(it.n_inst), dim3(256), 0, s, it, in, out, sums, it.nchunks, step);
+}
+#endif
 
 // PCG step (pcg.hpp:70-80, identity preconditioner): p = r (first step) or
 // r + beta*p; Ap on interior entries; pAp partial over interior entries.
@@ -712,6 +909,35 @@ static void minres_upd_N(const MinresUpdArgs<T>& a, int cpb, int grid, hipStream
     }
 }
 
+template <int N, typename T, int CPB, bool P>
+static void minres_fused_NC(int kind, const MinresFusedArgs<T>& a, const Dm<N, T>& D, int grid, hipStream_t s,
+                            hipEvent_t e0, hipEvent_t e1)
+{
+    const int items = a.nchunks / CPB;
+    dim3 g(P ? std::min(grid, items) : items);
+    dim3 b1(64 * CPB), b2(128 * CPB);
+    switch (kind) {
+    case K_LAPLACE: SEM_LAUNCH_TIMED((k_minres_fused<N, T, K_LAPLACE, CPB, P>), g, b1, s, e0, e1, a, D); break;
+    case K_HELMHOLTZ: SEM_LAUNCH_TIMED((k_minres_fused<N, T, K_HELMHOLTZ, CPB, P>), g, b1, s, e0, e1, a, D); break;
+    case K_RES: SEM_LAUNCH_TIMED((k_minres_fused<N, T, K_RES, CPB, P>), g, b2, s, e0, e1, a, D); break;
+    default: throw std::runtime_error("minres: unsupported operator kind");
+    }
+}
+
+template <int N, typename T>
+static void minres_fused_N(int kind, const MinresFusedArgs<T>& a, const T* d, int cpb, int grid, hipStream_t s,
+                           hipEvent_t e0, hipEvent_t e1)
+{
+    auto D = make_dm<N, T>(d);
+    if (cpb == 2) {
+        if (grid > 0) minres_fused_NC<N, T, 2, true>(kind, a, D, grid, s, e0, e1);
+        else minres_fused_NC<N, T, 2, false>(kind, a, D, grid, s, e0, e1);
+    } else {
+        if (grid > 0) minres_fused_NC<N, T, 1, true>(kind, a, D, grid, s, e0, e1);
+        else minres_fused_NC<N, T, 1, false>(kind, a, D, grid, s, e0, e1);
+    }
+}
+
 #define SEM_DISPATCH_N(N, CALL)                                   \
     switch (N) {                                                  \
     case 2: CALL(2); break;                                       \
@@ -751,6 +977,16 @@ void launch_minres_op_d(int N, int kind, const MinresOpArgs<T>& a, const T* d, i
 }
 
 template <typename T>
+void launch_minres_fused(int N, int kind, const MinresFusedArgs<T>& a, const T* d, int cpb, int grid, hipStream_t s,
+                         hipEvent_t e0, hipEvent_t e1)
+{
+    if (a.nchunks % kChunkAlign != 0) throw std::runtime_error("internal error: chunk count not aligned");
+#define CALL_MF(n) minres_fused_N<n, T>(kind, a, d, cpb, grid, s, e0, e1)
+    SEM_DISPATCH_N(N, CALL_MF)
+#undef CALL_MF
+}
+
+template <typename T>
 void launch_minres_upd(int N, const MinresUpdArgs<T>& a, int cpb, int grid, hipStream_t s, hipEvent_t e0,
                        hipEvent_t e1)
 {
@@ -775,12 +1011,16 @@ template void launch_element_op_d<double>(int, int, const OpArgs<double>&, const
 template void launch_minres_op_d<double>(int, int, const MinresOpArgs<double>&, const double*, int, int, hipStream_t,
                                           hipEvent_t, hipEvent_t);
 template void launch_minres_upd<double>(int, const MinresUpdArgs<double>&, int, int, hipStream_t, hipEvent_t, hipEvent_t);
+template void launch_minres_fused<double>(int, int, const MinresFusedArgs<double>&, const double*, int, int, hipStream_t,
+                                           hipEvent_t, hipEvent_t);
 template void launch_pcg_op_d<double>(int, const PcgOpArgs<double>&, int, const double*, hipStream_t);
 #else
 template void launch_element_op_d<float>(int, int, const OpArgs<float>&, const float*, hipStream_t);
 template void launch_minres_op_d<float>(int, int, const MinresOpArgs<float>&, const float*, int, int, hipStream_t,
                                           hipEvent_t, hipEvent_t);
 template void launch_minres_upd<float>(int, const MinresUpdArgs<float>&, int, int, hipStream_t, hipEvent_t, hipEvent_t);
+template void launch_minres_fused<float>(int, int, const MinresFusedArgs<float>&, const float*, int, int, hipStream_t,
+                                           hipEvent_t, hipEvent_t);
 #endif
 
 }  // namespace sem

@@ -68,6 +68,64 @@ __device__ __forceinline__ double group_sum(const double* __restrict__ p, int n,
     return t;
 }
 
+// Three sums at once: p[a*stride .. a*stride + n) for a = 0, 1, 2, each with
+// the decomposition and order of group_sum (so each equals group_sum of its
+// array); the loads of the three arrays are issued together. sh: 12 doubles.
+template <int M>
+__device__ __forceinline__ void group_sums3(const double* __restrict__ p, int stride, int n, int g0, int gstep,
+                                            double* sh)
+{
+    const int lane = threadIdx.x & 63;
+    constexpr int KB = 16 / M;
+    double s[3][M];
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int j = 0; j < M; ++j) s[a][j] = 0;
+    for (int kb = 0; kb * 256 < n; kb += KB) {
+        double v[3][M][KB];
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int j = 0; j < M; ++j)
+#pragma unroll
+                for (int u = 0; u < KB; ++u) {
+                    const int k = (kb + u) * 256 + (g0 + j * gstep) * 64 + lane;
+                    v[a][j][u] = k < n ? p[(size_t)a * stride + k] : 0.0;
+                }
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int j = 0; j < M; ++j)
+#pragma unroll
+                for (int u = 0; u < KB; ++u) {
+                    const int k = (kb + u) * 256 + (g0 + j * gstep) * 64 + lane;
+                    if (k < n) s[a][j] += v[a][j][u];
+                }
+    }
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int j = 0; j < M; ++j) {
+            const double t = wave_sum(s[a][j]);
+            if (lane == 0) sh[4 * a + g0 + j * gstep] = t;
+        }
+}
+
+__device__ __forceinline__ void group_sum3(const double* __restrict__ p, int stride, int n, double* sh, int nw,
+                                           double (&out)[3])
+{
+    const int wv = threadIdx.x >> 6;
+    if (nw == 1) group_sums3<4>(p, stride, n, 0, 1, sh);
+    else if (nw == 2) group_sums3<2>(p, stride, n, wv, 2, sh);
+    else if (nw == 3) { if (wv == 0) group_sums3<2>(p, stride, n, 0, 3, sh); else group_sums3<1>(p, stride, n, wv, 1, sh); }
+    else if (wv < 4) group_sums3<1>(p, stride, n, wv, 1, sh);
+    __syncthreads();
+#pragma unroll
+    for (int a = 0; a < 3; ++a) out[a] = ((sh[4 * a] + sh[4 * a + 1]) + sh[4 * a + 2]) + sh[4 * a + 3];
+    __syncthreads();
+}
+
 // beta = sqrt(s), s = dot(y, y) of the step just finished, and the Givens
 // update of minres.hpp:62-99 (stop test minres.hpp:92 with atol = tol there).
 // Contraction is off so that every workgroup, whatever code surrounds the
@@ -111,6 +169,29 @@ __device__ __forceinline__ void minres_beta_update(MinresState& sc, double s)
         sc.done = 1;
     } else if (sc.iter >= sc.maxit) {
         sc.done = 1;
+    }
+}
+
+// Recurrence at the start of fused Lanczos step i >= 2 (deferred scaling: step
+// i finishes y_{i-1} = A y_{i-2} / beta_{i-1} - (beta_{i-1}/beta_{i-2}) y_{i-3}
+// - (alpha_{i-1}/beta_{i-1}) y_{i-2}, then applies A to it). P are the sums of
+// step i-1: P0 = ||y_{i-2}||^2 = beta_{i-1}^2, P1 = y_{i-2}.A y_{i-2},
+// P2 = y_{i-2}.y_{i-3}. For i >= 3 this is the Givens update of iteration i-2
+// (minres.hpp:62-99) followed by alpha_{i-1} = v.(A v - (beta/beta_old) r1)
+// (minres.hpp:50-54) written with those sums.
+__device__ __forceinline__ void minres_fused_prologue(MinresState& sc, int step, const double (&P)[3])
+{
+#pragma clang fp contract(off)
+    if (step >= 3) {
+        if (sc.done) {
+            sc.p3_pending = 0;  // applied by the step that stopped
+            return;
+        }
+        minres_beta_update(sc, P[0]);
+        if (sc.done) return;
+        sc.alpha = P[1] * (sc.ib * sc.ib) - P[2] / sc.beta_old;
+    } else if (step == 2) {
+        sc.alpha = P[1] * (sc.ib * sc.ib);
     }
 }
 

@@ -201,6 +201,42 @@ struct MinresUpdArgs {
     const double* op_partial;  // the operator kernel's per-chunk alpha partials
 };
 
+// Arguments of the fused Lanczos-step kernel (one launch per step, deferred
+// scaling). Step i: finishes y_{i-1} from z_in = A y_{i-2} (raw, element-local;
+// interface copies gathered), y_{i-2} and y_{i-3}; stores it; applies A to it
+// (z_out, raw); writes per-chunk sums ||y_{i-1}||^2, y_{i-1}.z_out and
+// y_{i-1}.y_{i-2}; applies the pending w/x update of iteration i-2 with
+// v = y_{i-3}/beta_{i-2}. Step 1 reads y_0 = r0 from y_out and stores nothing.
+template <typename T>
+struct MinresFusedArgs {
+    const T* z_in;
+    T* z_out;
+    const T* y2;  // y_{i-2}
+    const T* y3;  // y_{i-3}
+    T* y_out;     // y_{i-1}
+    const T* wa;  // w_{i-4}
+    const T* wb;  // w_{i-3}
+    T* wn;        // w_{i-2}
+    T* x;
+    const T* geo;
+    const T* cf;
+    T alpha;
+    size_t len;
+    int nchunks;
+    const uint8_t* obits;
+    const int* eside;
+    const int* ecorner;
+    const int* c_off;
+    const int* c_pos;
+    double* sums;              // 3 * nchunks per-chunk sums of this step (out)
+    const double* prev_sums;   // the same of the previous step
+    InstTable it;
+    const KrylovScalars* st_in;
+    KrylovScalars* st_out;
+    int step;   // i (1-based)
+    int first;  // first launch of a batch: the state already holds this step's recurrence
+};
+
 // Work-item shape of the two Lanczos-step kernels: CPB chunks per workgroup
 // and the grid size (0: one workgroup per item).
 struct StepShape {
@@ -234,6 +270,14 @@ void launch_pcg_op_d(int N, const PcgOpArgs<T>& a, int nchunks, const T* d, hipS
 // (reads in[k], writes out[k])
 void launch_minres_close(int kind, const InstTable& it, const KrylovScalars* in, KrylovScalars* out,
                          const double* partial, hipStream_t s);
+// fused Lanczos step (one launch per step)
+template <typename T>
+void launch_minres_fused(int N, int kind, const MinresFusedArgs<T>& a, const T* d, int cpb, int grid, hipStream_t s,
+                         hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
+// recurrence closing a batch of fused steps: what step `step` would compute
+// at its start, one workgroup per instance (reads in[k], writes out[k])
+void launch_minres_fused_close(const InstTable& it, const KrylovScalars* in, KrylovScalars* out, const double* sums,
+                               int step, hipStream_t s);
 // update kernel of a Lanczos step
 template <typename T>
 void launch_minres_upd(int N, const MinresUpdArgs<T>& a, int cpb, int grid, hipStream_t s, hipEvent_t e0 = nullptr,

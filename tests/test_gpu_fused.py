@@ -1,0 +1,98 @@
+"""The fused MINRES path (one launch per Lanczos step, operator applied before
+the 1/beta scaling): same recurrence as the two-launch path, so it meets the
+same oracle tolerances, and its own work shapes give identical bits."""
+import numpy as np
+import pytest
+
+from conftest import SMALL_MESH, rel, structured
+
+pytestmark = pytest.mark.gpu
+
+
+def res_problem(sem, orc, p=4, seed=7):
+    m = sem.load_mesh(p, SMALL_MESH)
+    o = orc.Mesh.load(p, SMALL_MESH)
+    rng = np.random.default_rng(seed)
+    V = m.field(lambda x, y: 0.5 * (x * x + y * y))
+    b = np.concatenate([o.galerkin(2, rng.standard_normal(m.size)), o.galerkin(2, rng.standard_normal(m.size))])
+    return m, o, V, b
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_fused_stage_solve_matches_oracle(sem, orc, precision):
+    m, o, V, b = res_problem(sem, orc)
+    alpha = 0.25 * 1e-2
+    zo, ok, it, res = o.stage_solve(V, alpha, b, None, 5000, 1e-12)
+    zu = np.concatenate([o.unmask(zo[:m.size]), o.unmask(zo[m.size:])])
+    sem.set_step_mode(m, 0)
+    z0, r0 = sem.stage_solve(m, V, alpha, b, None, 5000, 1e-12, precision)
+    sem.set_step_mode(m, 1)
+    assert sem.get_step_mode(m) == 1 and sem.get_step_mode(m, sem.PREC_MIXED) == 1
+    z1, r1 = sem.stage_solve(m, V, alpha, b, None, 5000, 1e-12, precision)
+    assert r1.success and r0.success and ok
+    assert rel(z1, zu) < 1e-10
+    if precision == 0:
+        assert abs(r1.n_iter - it) <= 2 and abs(r1.n_iter - r0.n_iter) <= 2
+        assert abs(r1.residual - r0.residual) <= 1e-3 * r0.residual + 1e-14
+
+
+def test_fused_helmholtz_and_laplace(sem, orc):
+    m = sem.load_mesh(5, SMALL_MESH)
+    o = orc.Mesh.load(5, SMALL_MESH)
+    F = m.field(lambda x, y: np.cos(x) * y)
+    c = np.full(m.size, 3.0)
+    uo, ok, it, res = o.helmholtz(np.zeros(m.size), c, F, 5000, 1e-12)
+    sem.set_step_mode(m, 1)
+    u = np.zeros(m.size)
+    r = sem.helmholtz(u, c, F, m, 5000, 1e-12)
+    assert r.success and ok and abs(r.n_iter - it) <= 2
+    assert rel(o.unmask(u), o.unmask(uo)) < 1e-10
+
+
+def test_fused_nonconvergence_semantics(sem, orc):
+    m, o, V, b = res_problem(sem, orc, seed=9)
+    for mode in (0, 1):
+        sem.set_step_mode(m, mode)
+        z, r = sem.stage_solve(m, V, 0.5, b, None, 7, 1e-14)
+        assert not r.success and r.n_iter == 8, (mode, r.n_iter)
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_fused_shapes_bit_identical(sem, precision):
+    xy, q = structured(20, (-2, 2, -2, 2), 0.1, 3)
+    m = sem.Mesh.from_arrays(6, xy, q)
+    sem.set_step_mode(m, 1)
+    rng = np.random.default_rng(11)
+    V = m.field(lambda x, y: x * x + y * y)
+    b = np.concatenate([sem.gproj(rng.standard_normal(m.size), m), sem.gproj(rng.standard_normal(m.size), m)])
+    ref = None
+    for shape in [(1, 0, 1, 0), (2, 0, 2, 0), (1, 3, 1, 0), (2, 64, 1, 0), (1, 1, 1, 0), (2, 512, 1, 0)]:
+        sem.set_step_shape(m, shape, precision)
+        z, r = sem.stage_solve(m, V, 2e-3, b, None, 3000, 1e-11, precision)
+        assert r.success
+        if ref is None:
+            ref = (z, r.n_iter, r.residual)
+        else:
+            assert np.array_equal(z, ref[0]), shape
+            assert (r.n_iter, r.residual) == ref[1:], shape
+
+
+def test_fused_stepper_matches_two_launch(sem):
+    xy, q = structured(12, (-3, 3, -3, 3), 0.0, 0)
+    m = sem.Mesh.from_arrays(5, xy, q)
+    X, Y = m.coords()
+    V = X * X + Y * Y
+    env = np.exp(-(X * X + Y * Y) / 2)
+    re, im = env * np.cos(2 * X + Y), -env * np.sin(2 * X + Y)
+    out = {}
+    for mode in (0, 1):
+        sem.set_step_mode(m, mode)
+        for name, prec, guess in (("fp64", 0, 0), ("mixed_learned", 1, 2)):
+            opts = sem.SolverOpts(max_iter=20000, tol=1e-12, dt=5e-3, precision=prec, guess=guess, learn_steps=4)
+            st = sem.Stepper(m, 1, re, im, V, opts)
+            assert st.advance(1e30, 8)
+            a, b, _ = st.read()
+            out[(mode, name)] = np.concatenate([a, b])
+            st.close()
+    for name in ("fp64", "mixed_learned"):
+        assert rel(out[(1, name)], out[(0, name)]) < 1e-9, name

@@ -136,6 +136,7 @@ def test_step_shapes_bit_identical(sem, precision):
     # workgroup per item or a persistent grid) gives the same bits
     xy, q = structured(20, (-2, 2, -2, 2), 0.1, 3)
     m = sem.Mesh.from_arrays(6, xy, q)
+    sem.set_step_mode(m, 0)  # the two-launch step (the fused one has its own test)
     rng = np.random.default_rng(11)
     V = m.field(lambda x, y: x * x + y * y)
     b = np.concatenate([sem.gproj(rng.standard_normal(m.size), m), sem.gproj(rng.standard_normal(m.size), m)])
@@ -163,6 +164,7 @@ def test_graph_mode_bit_identical(sem, precision):
     # bits, on every replay (several solves reuse one captured graph)
     xy, q = structured(16, (-2, 2, -2, 2), 0.1, 5)
     m = sem.Mesh.from_arrays(5, xy, q)
+    sem.set_step_mode(m, 0)  # graphs replay the two-launch step
     rng = np.random.default_rng(13)
     V = m.field(lambda x, y: x * x + 0.5 * y * y)
     bs = [np.concatenate([sem.gproj(rng.standard_normal(m.size), m), sem.gproj(rng.standard_normal(m.size), m)])
