@@ -26,9 +26,10 @@ value = stage solves/s of the fastest mode whose final psi agrees with it to
 
 Before timing, the autotuner picks the work shape of the two Lanczos-step
 kernels on this mesh for each precision (results do not depend on it).
-Also measured: live HIP-event timing of the dominant kernel (the fused
-MINRES Lanczos-step operator kernel, sampled every fourth Lanczos step) over
-the timed region -> roofline, and of the update kernel beside it; glaplace
+Also measured: live HIP-event timing of the dominant kernel (the MINRES
+Lanczos-step kernel of the reported mode: the fused one-launch step for the
+FP32 inner solver, the operator kernel of the two-launch step for FP64;
+sampled every fourth Lanczos step) over the timed region -> roofline; glaplace
 GB/s on a 512 x 512, degree 7 mesh (671 MB of algorithmic traffic, larger
 than the 256 MB Infinity Cache); the CPU oracle timed on one stage solve of
 the same workload (bounded sample) -> cpu_baseline.
@@ -341,6 +342,9 @@ def main():
     avg_us = total_us / max(launches, 1)
     achieved = bpl / (avg_us * 1e-6) / 1e9 if launches else 0.0
     shape = tuned["fp64" if best == "fp64" else "fp32"]["shape"]
+    best_prec = sem.PREC_FP64 if best == "fp64" else sem.PREC_MIXED
+    fused = sem.get_step_mode(m, best_prec) == 1
+    kbase = "k_minres_fused" if fused else "k_minres_op"
     # HBM bytes per launch of the same kernel from the committed rocprofv3 PMC summary of this
     # workload (FETCH_SIZE and WRITE_SIZE, scaled by the copy-kernel calibration), when present
     # (the work shape does not change the bytes a launch moves; the most dispatched variant is used)
@@ -349,20 +353,22 @@ def main():
     if os.path.exists(prof):
         with open(prof) as fh:
             kern = json.load(fh)["kernels"]
-        prefix = "void sem::k_minres_op<%d, %s, 3," % (args.p + 1, "double" if best == "fp64" else "float")
+        prefix = "void sem::%s<%d, %s, 3," % (kbase, args.p + 1, "double" if best == "fp64" else "float")
         cands = [k for k in kern if k.startswith(prefix)]
         if cands:
             traffic = round(kern[cands[0]]["traffic_bytes"])
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "kernel": "k_minres_op<%d,%s,RES,%d,%s>" % (args.p + 1, "double" if best == "fp64" else "float",
-                                                          shape[0], "true" if shape[1] else "false"),
+                "kernel": "%s<%d,%s,RES,%d,%s>" % (kbase, args.p + 1, "double" if best == "fp64" else "float",
+                                                 shape[0], "true" if shape[1] else "false"),
                 "avg_us": round(avg_us, 2), "launches": launches, "bytes_per_launch": bpl}
     ul, uus, ubpl = res[best]["kernel_upd"]
-    upd = {"kernel": "k_minres_upd<%d,%s,%d,%s>" % (args.p + 1, "double" if best == "fp64" else "float", shape[2],
-                                                    "true" if shape[3] else "false"),
-           "avg_us": round(uus / max(ul, 1), 2), "launches": ul, "bytes_per_launch": ubpl,
-           "GB/s": round(ubpl / (uus / max(ul, 1)) / 1e3, 1) if ul else None}
+    upd = None
+    if ul:  # the two-launch step's second kernel (absent in the fused step)
+        upd = {"kernel": "k_minres_upd<%d,%s,%d,%s>" % (args.p + 1, "double" if best == "fp64" else "float",
+                                                        shape[2], "true" if shape[3] else "false"),
+               "avg_us": round(uus / ul, 2), "launches": ul, "bytes_per_launch": ubpl,
+               "GB/s": round(ubpl / (uus / ul) / 1e3, 1)}
 
     copy = {}
     for v in (0, 1, 2, 4):
@@ -430,6 +436,8 @@ def main():
         "roofline": roofline,
         "update_kernel": upd,
         "step_shape_autotuned": tuned,
+        "step_mode": {"fp64": "fused" if sem.get_step_mode(m, sem.PREC_FP64) else "two launches",
+                      "fp32": "fused" if sem.get_step_mode(m, sem.PREC_MIXED) else "two launches"},
         "glaplace": glap,
         "cpu_baseline": cpu,
         "copy_GBs_measured": round(copy_gbs, 1),
