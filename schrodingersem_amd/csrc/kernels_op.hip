@@ -592,7 +592,7 @@ __global__ __launch_bounds__(128 * CPB) void k_minres_upd(MinresUpdArgs<T> a)
 // buffer.
 template <int N, typename T, int KIND, int CPB, bool PERSIST>
 __global__ __launch_bounds__(64 * CPB * ((KIND == K_RES) ? 2 : 1))
-__attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 1, 8))) void k_minres_fused(MinresFusedArgs<T> a, Dm<N, T> D)
+__attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 3 : 1, 8))) void k_minres_fused(MinresFusedArgs<T> a, Dm<N, T> D)
 {
     using S = Shape<N>;
     constexpr int NC = (KIND == K_RES) ? 2 : 1;
