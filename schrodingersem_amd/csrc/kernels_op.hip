@@ -254,22 +254,32 @@ __device__ __forceinline__ int xcd_item(int b, int n)
 
 // Persistent form of the same idea: XCD x owns the contiguous item range
 // that xcd_item gives it, and its workgroups stride through that range.
-// Grids smaller than 8 fall back to a plain grid stride.
+// Grids smaller than 8 fall back to a plain grid stride. body(item) returns
+// -1, or an item index to skip ahead to (the end of a stopped instance's
+// items); the walk continues at the first of its own items at or after it.
 template <typename F>
 __device__ __forceinline__ void for_items_persistent(int n, F&& body)
 {
     constexpr int nx = 8;
     const int G = gridDim.x, b = blockIdx.x;
+    int start, end, step;
     if (G < nx) {
-        for (int it = b; it < n; it += G) body(it);
-        return;
+        start = b;
+        end = n;
+        step = G;
+    } else {
+        const int q = n / nx, r = n % nx;
+        const int x = b % nx, k = b / nx;
+        const int lo = (x < r) ? x * (q + 1) : r * (q + 1) + (x - r) * q;
+        start = lo + k;
+        end = lo + q + (x < r ? 1 : 0);
+        step = (G - x + nx - 1) / nx;  // workgroups on XCD x
     }
-    const int q = n / nx, r = n % nx;
-    const int x = b % nx, k = b / nx;
-    const int gx = (G - x + nx - 1) / nx;  // workgroups on XCD x
-    const int lo = (x < r) ? x * (q + 1) : r * (q + 1) + (x - r) * q;
-    const int len = q + (x < r ? 1 : 0);
-    for (int it = lo + k; it < lo + len; it += gx) body(it);
+    for (int it = start; it < end;) {
+        const int skip = body(it);
+        if (skip > it) it = start + (skip - start + step - 1) / step * step;
+        else it += step;
+    }
 }
 
 // owner test of entry (i, l) of an element with ownership bits (bit s-1:
@@ -330,16 +340,21 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 1, 8))) void k_minres_op
             sc = *static_cast<const MinresState*>(a.st_in + inst);
             if (!a.first) {
                 // the previous step's beta recurrence, evaluated by every workgroup
-                const int p0 = ci.y;
-                const double s2 = group_sum(a.prev_partial + p0, ci.z - p0, red, NW);
-                if (!sc.done) minres_beta_update(sc, s2);
-                else sc.p3_pending = 0;  // applied by the kernel of the stopping step
+                if (!sc.done) {
+                    const int p0 = ci.y;
+                    minres_beta_update(sc, group_sum(a.prev_partial + p0, ci.z - p0, red, NW));
+                } else {
+                    sc.p3_pending = 0;  // applied by the kernel of the stopping step
+                }
             }
             // this kernel applies the pending w/x update; the state it stores drops it
             pend = sc.p3_pending != 0;
             sc.p3_pending = 0;
             if (c0 == ci.y && threadIdx.x == 0) *static_cast<MinresState*>(a.st_out + inst) = sc;
         }
+        // a stopped instance has no work left: skip the rest of its items
+        // (its partials are never read again)
+        if (sc.done && !pend) return ci.z / CPB;
         const int chunk = c0 + sub;
         const Col<N, T> c{(size_t)chunk * S::K * N * N + lane, act};
         double part = 0;
@@ -417,6 +432,7 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 1, 8))) void k_minres_op
             a.partial[c0 + threadIdx.x] = t;
         }
         __syncthreads();
+        return -1;
     };
     if constexpr (PERSIST) {
         for_items_persistent(n_items, body);
@@ -548,10 +564,11 @@ __global__ __launch_bounds__(128 * CPB) void k_minres_upd(MinresUpdArgs<T> a)
                 if (c0 == p0 && threadIdx.x == 0) a.st[inst].alpha = alpha;
             }
         }
+        if (done) return ci.z / CPB;  // stopped: its partials are never read again
         const int chunk = c0 + sub;
         const size_t base = (size_t)chunk * K * N * N + lane;
         double part = 0;
-        if (act && !done) {
+        if (act) {
             const int el = lane / N, j = lane % N;
             const int e = chunk * K + el;
             const T c2 = (T)c2d;
@@ -576,6 +593,7 @@ __global__ __launch_bounds__(128 * CPB) void k_minres_upd(MinresUpdArgs<T> a)
             a.partial[c0 + threadIdx.x] = t;
         }
         __syncthreads();
+        return -1;
     };
     if constexpr (PERSIST) {
         for_items_persistent(n_items, body);
@@ -622,8 +640,8 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 3 : 1, 8))) void k_minres_fu
             cur_inst = inst;
             sc = *static_cast<const MinresState*>(a.st_in + inst);
             if (!a.first && a.step >= 2) {
-                double P[3];
-                group_sum3(a.prev_sums + ci.y, nch, ci.z - ci.y, red, NW, P);
+                double P[3] = {0.0, 0.0, 0.0};  // not read once the instance has stopped
+                if (!sc.done) group_sum3(a.prev_sums + ci.y, nch, ci.z - ci.y, red, NW, P);
                 minres_fused_prologue(sc, a.step, P);
             }
             // this kernel applies the pending w/x update; the state it stores drops it
@@ -631,6 +649,9 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 3 : 1, 8))) void k_minres_fu
             sc.p3_pending = 0;
             if (c0 == ci.y && threadIdx.x == 0) *static_cast<MinresState*>(a.st_out + inst) = sc;
         }
+        // a stopped instance has no work left: skip the rest of its items
+        // (its sums are never read again)
+        if (sc.done && !pend) return ci.z / CPB;
         const int chunk = c0 + sub;
         const size_t base = (size_t)chunk * S::K * N * N + lane;
         const Col<N, T> c{base, act};
@@ -734,6 +755,7 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 3 : 1, 8))) void k_minres_fu
             a.sums[(size_t)q * nch + c0 + ch] = t;
         }
         __syncthreads();
+        return -1;
     };
     if constexpr (PERSIST) {
         for_items_persistent(n_items, body);
