@@ -78,7 +78,7 @@ struct sem_mesh {
     int* bnd = nullptr;
     int dss_nb = 1;
     int* eside = nullptr;    // per element side: neighbour descriptor or -1
-    int* ecorner = nullptr;  // per element corner: interior corner index or -1
+    int4* ecq = nullptr;     // per element corner: copy positions in fold order (see the gather kernels)
     // D matrix
     std::vector<double> D64;
     std::vector<float> D32;
@@ -164,7 +164,7 @@ struct sem_mesh {
         hipFree(c_pos);
         hipFree(bnd);
         hipFree(eside);
-        hipFree(ecorner);
+        hipFree(ecq);
         hipFree(sc);
         hipFree(sc_alt);
         hipFree(d_inst_off);
@@ -375,6 +375,22 @@ static void setup_device(sem_mesh* m)
                 }
             }
     }
+    // per element corner: the positions of every copy of the corner node in
+    // fold order when there are at most 4 (-1 padded), {-2, corner index, 0, 0}
+    // when there are more, all -1 on the Dirichlet boundary
+    std::vector<int4> ecq(ecorner.size(), make_int4(-1, -1, -1, -1));
+    for (size_t q = 0; q < ecorner.size(); ++q) {
+        const int ci = ecorner[q];
+        if (ci < 0) continue;
+        const int b = c_off[ci], n = c_off[ci + 1] - b;
+        if (n > 4) {
+            ecq[q] = make_int4(-2, ci, 0, 0);
+        } else {
+            int v[4] = {-1, -1, -1, -1};
+            for (int t = 0; t < n; ++t) v[t] = c_pos[b + t];
+            ecq[q] = make_int4(v[0], v[1], v[2], v[3]);
+        }
+    }
     m->n_pairs = (int)pairs.size();
     m->n_corners = (int)c_off.size() - 1;
     m->n_bnd = (int)bnd.size();
@@ -389,7 +405,7 @@ static void setup_device(sem_mesh* m)
     up(m->bnd, bnd);
     up(m->obits, m->h_obits);
     up(m->eside, eside);
-    up(m->ecorner, ecorner);
+    up(m->ecq, ecq);
     const long total = (long)m->n_pairs + m->n_corners + m->n_bnd;
     m->dss_nb = (int)std::max<long>(1, (total + 255) / 256);
 
@@ -609,7 +625,7 @@ static Results minres_fused1(sem_mesh* m, int kind, const T* cf, double alpha, c
             a.nchunks = nch;
             a.obits = m->obits;
             a.eside = m->eside;
-            a.ecorner = m->ecorner;
+            a.ecq = m->ecq;
             a.c_off = m->c_off;
             a.c_pos = m->c_pos;
             a.sums = S[i % 2];
@@ -795,7 +811,7 @@ static Results minres_fused(sem_mesh* m, int kind, const T* cf, double alpha, co
         u.nchunks = nch;
         u.obits = m->obits;
         u.eside = m->eside;
-        u.ecorner = m->ecorner;
+        u.ecq = m->ecq;
         u.c_off = m->c_off;
         u.c_pos = m->c_pos;
         u.partial = m->partial + nch;

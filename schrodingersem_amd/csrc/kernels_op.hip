@@ -472,19 +472,30 @@ void launch_minres_close(int kind, const InstTable& it, const KrylovScalars* in,
 
 
 // Assembled values of column j of element e (lane position base of the
-// chunk's first plane) from element-local values yr: interior planes are the
-// element's own; ring entries gather every copy of their node in the
-// reference's order (edge: owner + other, global_sum.hpp:28-34; corner: left
-// fold from +0.0 in connected_elements order, global_sum.hpp:54-63), so every
-// copy gets the same bits; Dirichlet entries are zero.
+// chunk's first plane) from element-local values yr, in two phases so that
+// the loads are in flight while the caller does other work: issue() loads the
+// element's own column and every other copy of its ring nodes, finish()
+// combines them in v. Interior planes are the element's own; ring entries sum
+// every copy of their node in the reference's order (edge: owner + other,
+// global_sum.hpp:28-34; corner: left fold from +0.0 in connected_elements
+// order, global_sum.hpp:54-63), so every copy gets the same bits; Dirichlet
+// entries are zero.
 template <int N, typename T>
-__device__ __forceinline__ void gather_column(const T* __restrict__ yr, size_t base, int e, int j,
-                                              const int* __restrict__ eside, const int* __restrict__ ecorner,
-                                              const int* __restrict__ c_off, const int* __restrict__ c_pos, T (&v)[N])
-{
-    using S = Shape<N>;
-    constexpr int K = S::K, KN = S::KN;
-    auto side_pos = [&](int d, int k) -> long {
+struct ColumnGather {
+    static constexpr int NE = (N - 2 > 2) ? N - 2 : 2;
+    T v[N];        // own column, then the assembled column
+    T o[NE];       // other copies: planes 1..N-2 (side column) or planes 0 and N-1
+    T c[2][4];     // copies of the low and high corner nodes (side column)
+    int d0, d1;    // side descriptors
+    unsigned cm;   // copies present: bits 0-3 low corner, 4-7 high corner
+    bool side;     // column on side 1 (j = 0) or side 3 (j = N-1)
+
+    // position of entry k of side descriptor d's side (d: (element << 3) |
+    // (side - 1) << 1 | reversed)
+    __device__ static __forceinline__ long side_pos(int d, int k)
+    {
+        using S = Shape<N>;
+        constexpr int K = S::K, KN = S::KN;
         const int e2 = d >> 3, s2 = ((d >> 1) & 3) + 1;
         const int k2 = (d & 1) ? N - 1 - k : k;
         int i2, j2;
@@ -493,37 +504,69 @@ __device__ __forceinline__ void gather_column(const T* __restrict__ yr, size_t b
         else if (s2 == 3) { i2 = k2; j2 = N - 1; }
         else { i2 = 0; j2 = k2; }
         return (long)(e2 / K) * K * N * N + (long)i2 * KN + (long)(e2 % K) * N + j2;
-    };
-#pragma unroll
-    for (int i = 0; i < N; ++i) v[i] = yr[base + (size_t)i * KN];
-    if (j == 0 || j == N - 1) {
-        // column on side 1 (j = 0) or side 3 (j = N-1): planes 1..N-2; corners at planes 0, N-1
-        const int dcol = eside[4 * e + (j == 0 ? 0 : 2)];
-        const int c_lo = ecorner[4 * e + (j == 0 ? 0 : 3)];
-        const int c_hi = ecorner[4 * e + (j == 0 ? 1 : 2)];
-        T pv[N];
-        if (dcol >= 0) {
-#pragma unroll
-            for (int i = 1; i < N - 1; ++i) pv[i] = yr[side_pos(dcol, i)];
-        }
-        T s0 = 0, s1 = 0;
-        if (c_lo >= 0)
-            for (int q = c_off[c_lo]; q < c_off[c_lo + 1]; ++q) s0 += yr[c_pos[q]];
-        if (c_hi >= 0)
-            for (int q = c_off[c_hi]; q < c_off[c_hi + 1]; ++q) s1 += yr[c_pos[q]];
-#pragma unroll
-        for (int i = 1; i < N - 1; ++i) v[i] = (dcol >= 0) ? v[i] + pv[i] : T(0);
-        v[0] = s0;
-        v[N - 1] = s1;
-    } else {
-        // plane 0 is side 4, plane N-1 is side 2
-        const int d4 = eside[4 * e + 3], d2 = eside[4 * e + 1];
-        const T p0 = (d4 >= 0) ? yr[side_pos(d4, j)] : T(0);
-        const T p1 = (d2 >= 0) ? yr[side_pos(d2, j)] : T(0);
-        v[0] = (d4 >= 0) ? v[0] + p0 : T(0);
-        v[N - 1] = (d2 >= 0) ? v[N - 1] + p1 : T(0);
     }
-}
+
+    __device__ __forceinline__ void issue(const T* __restrict__ yr, size_t base, int e, int j,
+                                          const int* __restrict__ eside, const int4* __restrict__ ecq,
+                                          const int* __restrict__ c_off, const int* __restrict__ c_pos)
+    {
+        constexpr int KN = Shape<N>::KN;
+#pragma unroll
+        for (int i = 0; i < N; ++i) v[i] = yr[base + (size_t)i * KN];
+        side = (j == 0 || j == N - 1);
+        cm = 0;
+        if (side) {
+            d0 = eside[4 * e + (j == 0 ? 0 : 2)];
+            d1 = -1;
+            if (d0 >= 0) {
+#pragma unroll
+                for (int i = 1; i < N - 1; ++i) o[i - 1] = yr[side_pos(d0, i)];
+            }
+            const int4 q[2] = {ecq[4 * e + (j == 0 ? 0 : 3)], ecq[4 * e + (j == 0 ? 1 : 2)]};
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                if (q[h].x == -2) {
+                    // more than 4 copies: fold them from the corner's list
+                    T sum = 0;
+                    for (int t = c_off[q[h].y]; t < c_off[q[h].y + 1]; ++t) sum += yr[c_pos[t]];
+                    c[h][0] = sum;
+                    cm |= 1u << (4 * h);
+                } else {
+                    const int pp[4] = {q[h].x, q[h].y, q[h].z, q[h].w};
+#pragma unroll
+                    for (int t = 0; t < 4; ++t) {
+                        c[h][t] = pp[t] >= 0 ? yr[pp[t]] : T(0);
+                        if (pp[t] >= 0) cm |= 1u << (4 * h + t);
+                    }
+                }
+            }
+        } else {
+            d0 = eside[4 * e + 3];  // plane 0 is side 4
+            d1 = eside[4 * e + 1];  // plane N-1 is side 2
+            o[0] = d0 >= 0 ? yr[side_pos(d0, j)] : T(0);
+            o[1] = d1 >= 0 ? yr[side_pos(d1, j)] : T(0);
+        }
+    }
+
+    __device__ __forceinline__ void finish()
+    {
+        if (side) {
+#pragma unroll
+            for (int i = 1; i < N - 1; ++i) v[i] = (d0 >= 0) ? v[i] + o[i - 1] : T(0);
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                T sum = 0;
+#pragma unroll
+                for (int t = 0; t < 4; ++t)
+                    if ((cm >> (4 * h + t)) & 1u) sum += c[h][t];
+                v[h ? N - 1 : 0] = sum;
+            }
+        } else {
+            v[0] = (d0 >= 0) ? v[0] + o[0] : T(0);
+            v[N - 1] = (d1 >= 0) ? v[N - 1] + o[1] : T(0);
+        }
+    }
+};
 
 // Second kernel of a Lanczos step: work items of CPB chunks, one wave per
 // chunk and component. It finishes component w of y: interface copies gather
@@ -552,34 +595,36 @@ __global__ __launch_bounds__(128 * CPB) void k_minres_upd(MinresUpdArgs<T> a)
         const int4 ci = chunk_instance(a.it, c0);
         const int inst = ci.x;
         const KrylovScalars* sc = a.st + inst;
-        const int done = sc->done;
+        if (sc->done) return ci.z / CPB;  // stopped: its partials are never read again
+        const int chunk = c0 + sub;
+        const size_t base = (size_t)chunk * K * N * N + lane;
+        const int el = act ? lane / N : 0, j = act ? lane % N : 0;
+        const int e = chunk * K + el;
+        // loads first: they do not depend on alpha
+        ColumnGather<N, T> g;
+        T r[N];
+        if (act) {
+            g.issue(a.yraw + (size_t)wv * len, base, e, j, a.eside, a.ecq, a.c_off, a.c_pos);
+#pragma unroll
+            for (int i = 0; i < N; ++i) r[i] = a.r2[(size_t)wv * len + base + (size_t)i * KN];
+        }
         if (inst != cur_inst) {
             cur_inst = inst;
             // alpha = dot(v, y) from the operator kernel's partials (minres.hpp:54),
             // c2 = alpha / beta (minres.hpp:55); the instance's first workgroup keeps alpha
-            if (!done) {
-                const int p0 = ci.y;
-                const double alpha = group_sum(a.op_partial + p0, ci.z - p0, red, nc * CPB);
-                c2d = alpha / sc->beta;
-                if (c0 == p0 && threadIdx.x == 0) a.st[inst].alpha = alpha;
-            }
+            const int p0 = ci.y;
+            const double alpha = group_sum(a.op_partial + p0, ci.z - p0, red, nc * CPB);
+            c2d = alpha / sc->beta;
+            if (c0 == p0 && threadIdx.x == 0) a.st[inst].alpha = alpha;
         }
-        if (done) return ci.z / CPB;  // stopped: its partials are never read again
-        const int chunk = c0 + sub;
-        const size_t base = (size_t)chunk * K * N * N + lane;
         double part = 0;
         if (act) {
-            const int el = lane / N, j = lane % N;
-            const int e = chunk * K + el;
             const T c2 = (T)c2d;
             const unsigned bits = a.obits[e];
-            T v[N], r[N];
-            gather_column<N>(a.yraw + (size_t)wv * len, base, e, j, a.eside, a.ecorner, a.c_off, a.c_pos, v);
-#pragma unroll
-            for (int i = 0; i < N; ++i) r[i] = a.r2[(size_t)wv * len + base + (size_t)i * KN];
+            g.finish();
 #pragma unroll
             for (int i = 0; i < N; ++i) {
-                const T yv = v[i] - c2 * r[i];
+                const T yv = g.v[i] - c2 * r[i];
                 a.y[(size_t)wv * len + base + (size_t)i * KN] = yv;
                 if (owns<N>(i, j, bits)) part += (double)yv * (double)yv;
             }
@@ -636,12 +681,35 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 3 : 1, 8))) void k_minres_fu
         const int c0 = item * CPB;
         const int4 ci = chunk_instance(a.it, c0);
         const int inst = ci.x;
-        if (inst != cur_inst) {
+        const bool fresh = inst != cur_inst;
+        if (fresh) {
             cur_inst = inst;
             sc = *static_cast<const MinresState*>(a.st_in + inst);
+        }
+        const int chunk = c0 + sub;
+        const size_t base = (size_t)chunk * S::K * N * N + lane;
+        const Col<N, T> c{base, act};
+        // loads that do not depend on this step's recurrence scalars go out
+        // before the recurrence is evaluated (an instance that has stopped
+        // stays stopped; y_{i-3} is needed from step 3 on)
+        const bool early = !sc.done && a.step >= 2;
+        const bool early3 = early && (a.step >= 3 || sc.has_r1);
+        // the recurrence's sums are loaded first, so that its wait does not
+        // include the gather
+        const bool rec = fresh && !a.first && a.step >= 2 && !sc.done;
+        GroupSum3<NW, 8> gs;
+        if (rec) gs.issue(a.prev_sums + ci.y, nch, ci.z - ci.y);
+        ColumnGather<N, T> g;
+        T r2[N], r3[N];  // y_{i-2}, y_{i-3} of this component
+        if (early) {
+            if (act) g.issue(a.z_in + w * len, base, chunk * S::K + el, l, a.eside, a.ecq, a.c_off, a.c_pos);
+            c.ld(a.y2 + w * len, r2);
+            if (early3) c.ld(a.y3 + w * len, r3);
+        }
+        if (fresh) {
             if (!a.first && a.step >= 2) {
                 double P[3] = {0.0, 0.0, 0.0};  // not read once the instance has stopped
-                if (!sc.done) group_sum3(a.prev_sums + ci.y, nch, ci.z - ci.y, red, NW, P);
+                if (rec) gs.finish(a.prev_sums + ci.y, nch, ci.z - ci.y, red, P);
                 minres_fused_prologue(sc, a.step, P);
             }
             // this kernel applies the pending w/x update; the state it stores drops it
@@ -652,12 +720,8 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 3 : 1, 8))) void k_minres_fu
         // a stopped instance has no work left: skip the rest of its items
         // (its sums are never read again)
         if (sc.done && !pend) return ci.z / CPB;
-        const int chunk = c0 + sub;
-        const size_t base = (size_t)chunk * S::K * N * N + lane;
-        const Col<N, T> c{base, act};
         const unsigned bits = act ? a.obits[chunk * S::K + el] : 0u;
         double p0 = 0, p1 = 0, p2 = 0;
-        T r3[N];  // y_{i-3} of this component
         const bool has_r1 = sc.has_r1 != 0;
         if (!sc.done) {
             T yc[N];  // y_{i-1} of this component
@@ -665,14 +729,10 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 3 : 1, 8))) void k_minres_fu
                 c.ld(a.y_out + w * len, yc);
             } else {
                 const T ib = (T)sc.ib, cr1 = (T)sc.cr1, c2 = (T)(sc.alpha * sc.ib);
-                T v[N], r2[N];
-                if (act) gather_column<N>(a.z_in + w * len, base, chunk * S::K + el, l, a.eside, a.ecorner, a.c_off,
-                                          a.c_pos, v);
-                c.ld(a.y2 + w * len, r2);
-                if (has_r1) c.ld(a.y3 + w * len, r3);
+                if (act) g.finish();
 #pragma unroll
                 for (int i = 0; i < N; ++i) {
-                    T t = act ? v[i] * ib : T(0);
+                    T t = act ? g.v[i] * ib : T(0);
                     if (has_r1) t = t - cr1 * r3[i];
                     yc[i] = t - c2 * r2[i];
                     if (owns<N>(i, l, bits)) p2 += (double)yc[i] * (double)r2[i];
@@ -724,7 +784,7 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 3 : 1, 8))) void k_minres_fu
         if (pend) {
             // pending w/x update of iteration i-2 (minres.hpp:82-86) with
             // v = y_{i-3}/beta_{i-2}: w_new = ig*((v - eo*wa) - dl*wb), x += phi*w_new
-            if (sc.done || !has_r1) c.ld(a.y3 + w * len, r3);
+            if (!early3) c.ld(a.y3 + w * len, r3);
             const T ibp = (T)sc.p3_ibp, ig = (T)sc.p3_ig, eo = (T)sc.p3_eo, dl = (T)sc.p3_dl, phi = (T)sc.p3_phi;
             T wa[N], wb[N], x[N], wn[N];
             c.ld(a.wa + w * len, wa);

@@ -70,13 +70,16 @@ __device__ __forceinline__ double group_sum(const double* __restrict__ p, int n,
 
 // Three sums at once: p[a*stride .. a*stride + n) for a = 0, 1, 2, each with
 // the decomposition and order of group_sum (so each equals group_sum of its
-// array); the loads of the three arrays are issued together. sh: 12 doubles.
-template <int M>
+// array); the loads of the three arrays are issued together, LB per array
+// and thread at a time (the order of the additions does not depend on LB).
+// sh: 12 doubles.
+template <int M, int LB>
 __device__ __forceinline__ void group_sums3(const double* __restrict__ p, int stride, int n, int g0, int gstep,
                                             double* sh)
 {
     const int lane = threadIdx.x & 63;
-    constexpr int KB = 16 / M;
+    constexpr int KB = LB / M;
+    static_assert(KB >= 1, "load budget below the virtual groups per wave");
     double s[3][M];
 #pragma unroll
     for (int a = 0; a < 3; ++a)
@@ -112,19 +115,90 @@ __device__ __forceinline__ void group_sums3(const double* __restrict__ p, int st
         }
 }
 
+template <int LB = 16>
 __device__ __forceinline__ void group_sum3(const double* __restrict__ p, int stride, int n, double* sh, int nw,
                                            double (&out)[3])
 {
     const int wv = threadIdx.x >> 6;
-    if (nw == 1) group_sums3<4>(p, stride, n, 0, 1, sh);
-    else if (nw == 2) group_sums3<2>(p, stride, n, wv, 2, sh);
-    else if (nw == 3) { if (wv == 0) group_sums3<2>(p, stride, n, 0, 3, sh); else group_sums3<1>(p, stride, n, wv, 1, sh); }
-    else if (wv < 4) group_sums3<1>(p, stride, n, wv, 1, sh);
+    if (nw == 1) group_sums3<4, LB>(p, stride, n, 0, 1, sh);
+    else if (nw == 2) group_sums3<2, LB>(p, stride, n, wv, 2, sh);
+    else if (nw == 3) { if (wv == 0) group_sums3<2, LB>(p, stride, n, 0, 3, sh); else group_sums3<1, LB>(p, stride, n, wv, 1, sh); }
+    else if (wv < 4) group_sums3<1, LB>(p, stride, n, wv, 1, sh);
     __syncthreads();
 #pragma unroll
     for (int a = 0; a < 3; ++a) out[a] = ((sh[4 * a] + sh[4 * a + 1]) + sh[4 * a + 2]) + sh[4 * a + 3];
     __syncthreads();
 }
+
+// group_sum3 in two phases for a workgroup of NW waves (NW != 3): issue()
+// loads the first LB values per array and thread, finish() loads the rest,
+// adds in group_sum's order (so the results equal group_sum3's bit for bit)
+// and reduces. Loads issued between the two phases stay in flight while
+// finish() waits for its own, which were issued first.
+template <int NW, int LB>
+struct GroupSum3 {
+    static_assert(NW != 3, "three-wave workgroups use group_sum3");
+    static constexpr int G = NW < 4 ? NW : 4;  // waves that take part
+    static constexpr int M = 4 / G;            // virtual groups per wave
+    static constexpr int KB = LB / M;
+    static_assert(KB >= 1, "load budget below the virtual groups per wave");
+    double v[3][M][KB];
+
+    __device__ __forceinline__ void load(const double* __restrict__ p, int stride, int n, int kb)
+    {
+        const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int j = 0; j < M; ++j)
+#pragma unroll
+                for (int u = 0; u < KB; ++u) {
+                    const int k = (kb + u) * 256 + (wv + j * G) * 64 + lane;
+                    v[a][j][u] = k < n ? p[(size_t)a * stride + k] : 0.0;
+                }
+    }
+
+    __device__ __forceinline__ void issue(const double* __restrict__ p, int stride, int n)
+    {
+        if ((threadIdx.x >> 6) < G) load(p, stride, n, 0);
+    }
+
+    __device__ __forceinline__ void finish(const double* __restrict__ p, int stride, int n, double* sh,
+                                           double (&out)[3])
+    {
+        const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+        if (wv < G) {
+            double s[3][M];
+#pragma unroll
+            for (int a = 0; a < 3; ++a)
+#pragma unroll
+                for (int j = 0; j < M; ++j) s[a][j] = 0;
+            for (int kb = 0; kb * 256 < n; kb += KB) {
+                if (kb > 0) load(p, stride, n, kb);
+#pragma unroll
+                for (int a = 0; a < 3; ++a)
+#pragma unroll
+                    for (int j = 0; j < M; ++j)
+#pragma unroll
+                        for (int u = 0; u < KB; ++u) {
+                            const int k = (kb + u) * 256 + (wv + j * G) * 64 + lane;
+                            if (k < n) s[a][j] += v[a][j][u];
+                        }
+            }
+#pragma unroll
+            for (int a = 0; a < 3; ++a)
+#pragma unroll
+                for (int j = 0; j < M; ++j) {
+                    const double t = wave_sum(s[a][j]);
+                    if (lane == 0) sh[4 * a + wv + j * G] = t;
+                }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int a = 0; a < 3; ++a) out[a] = ((sh[4 * a] + sh[4 * a + 1]) + sh[4 * a + 2]) + sh[4 * a + 3];
+        __syncthreads();
+    }
+};
 
 // beta = sqrt(s), s = dot(y, y) of the step just finished, and the Givens
 // update of minres.hpp:62-99 (stop test minres.hpp:92 with atol = tol there).

@@ -192,7 +192,10 @@ struct MinresUpdArgs {
     int ncomp, nchunks;
     const uint8_t* obits;
     const int* eside;    // per element side: (neighbour << 3) | (neighbour side - 1) << 1 | reversed, or -1
-    const int* ecorner;  // per element corner: interior corner index, or -1 on the Dirichlet boundary
+    // per element corner: positions of the corner node's copies in fold order
+    // (-1 padded) when there are at most 4, {-2, corner index, 0, 0} when
+    // there are more (then c_off/c_pos list them), all -1 on the Dirichlet boundary
+    const int4* ecq;
     const int* c_off;
     const int* c_pos;
     double* partial;           // per-chunk beta^2 partial sums (out)
@@ -225,7 +228,7 @@ struct MinresFusedArgs {
     int nchunks;
     const uint8_t* obits;
     const int* eside;
-    const int* ecorner;
+    const int4* ecq;
     const int* c_off;
     const int* c_pos;
     double* sums;              // 3 * nchunks per-chunk sums of this step (out)
