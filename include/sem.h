@@ -245,6 +245,15 @@ int sem_get_step_shape(const sem_mesh* m, int precision, int* shape);
  * Defaults: FP64 0, FP32 1. */
 int sem_set_step_mode(sem_mesh* m, int precision, int mode);
 int sem_get_step_mode(const sem_mesh* m, int precision, int* mode);
+/* Diagonal scaling of the FP32 inner MINRES in mixed-precision Schrodinger
+ * stage solves (on = 1, the default): the inner solver works on S Res S with
+ * S = (m + alpha |k + c|)^(-1/2) from the assembled mass, stiffness and
+ * potential-mass diagonals, which takes fewer Lanczos steps; the outer FP64
+ * refinement stops on the true residual either way. It applies when the FP32
+ * step mode is 1 and graph mode is off. No counterpart in the reference
+ * (its MINRES is unpreconditioned, minres.hpp:19). */
+int sem_set_inner_scaling(sem_mesh* m, int on);
+int sem_get_inner_scaling(const sem_mesh* m, int* on);
 /* Graph mode: with on = 1 the MINRES solver captures `steps` Lanczos steps
  * (a multiple of 3) plus the batch-closing kernel as one HIP graph per solve
  * configuration and replays it between host checks of the stop flags

@@ -111,6 +111,8 @@ def lib():
             "sem_set_graph_mode": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
             "sem_set_step_mode": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
             "sem_get_step_mode": (C.c_int, [C.c_void_p, C.c_int, _ip]),
+            "sem_set_inner_scaling": (C.c_int, [C.c_void_p, C.c_int]),
+            "sem_get_inner_scaling": (C.c_int, [C.c_void_p, _ip]),
             "sem_batch_set_graph_mode": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
             "sem_autotune": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, _ip, _ip, _dp]),
             "sem_stepper_create": (C.c_int, [C.c_void_p, C.c_int, _dp, _dp, _dp, C.POINTER(_Opts),
@@ -585,6 +587,17 @@ def get_step_mode(mesh, precision=PREC_FP64):
     v = C.c_int()
     _check(lib().sem_get_step_mode(mesh._h, int(precision), C.byref(v)))
     return v.value
+
+
+def set_inner_scaling(mesh, on):
+    """Diagonal scaling of the FP32 inner MINRES of mixed-precision Schrodinger stage solves (default on)."""
+    _check(lib().sem_set_inner_scaling(mesh._h, int(bool(on))))
+
+
+def get_inner_scaling(mesh):
+    v = C.c_int()
+    _check(lib().sem_get_inner_scaling(mesh._h, C.byref(v)))
+    return bool(v.value)
 
 
 def set_graph_mode(mesh, on, steps=12):

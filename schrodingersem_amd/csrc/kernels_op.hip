@@ -744,11 +744,19 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 3 : 1, 8))) void k_minres_fu
                 if (owns<N>(i, l, bits)) p0 += (double)yc[i] * (double)yc[i];
             // stiffness input: the other component for Res, the only one otherwise
             T qs[N];
+            // Res with a diagonal scaling S: the operator acts on S y and its
+            // output is scaled by S again (S A S)
+            const bool scl = (KIND == K_RES) && a.dscale != nullptr;
+            T sv[N];
+            if (scl) {
+#pragma unroll
+                for (int i = 0; i < N; ++i) sv[i] = act ? (T)a.dscale[base + (size_t)i * S::KN] : T(0);
+            }
             if constexpr (NC == 2) {
                 T* mine = xch[wv];
                 T* other = xch[sub * NC + (1 - w)];
 #pragma unroll
-                for (int i = 0; i < N; ++i) mine[i * 64 + lane] = yc[i];
+                for (int i = 0; i < N; ++i) mine[i * 64 + lane] = scl ? yc[i] * sv[i] : yc[i];
                 __syncthreads();
 #pragma unroll
                 for (int i = 0; i < N; ++i) qs[i] = other[i * 64 + lane];
@@ -769,7 +777,9 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 3 : 1, 8))) void k_minres_fu
 #pragma unroll
                 for (int i = 0; i < N; ++i) {
                     const T t = a.alpha * (z[i] + cf[i] * qs[i]);
-                    z[i] = (w == 0) ? gj[i] * yc[i] + t : t - gj[i] * yc[i];
+                    const T yo = scl ? yc[i] * sv[i] : yc[i];
+                    z[i] = (w == 0) ? gj[i] * yo + t : t - gj[i] * yo;
+                    if (scl) z[i] = z[i] * sv[i];
                 }
             } else if constexpr (KIND == K_HELMHOLTZ) {
                 T cf[N];
@@ -847,6 +857,41 @@ void launch_minres_fused_close(const InstTable& it, const KrylovScalars* in, Kry
 {
     hipLaunchKernelGGL(k_minres_fused_close, dim3(it.n_inst), dim3(256), 0, s, it, in, out, sums, it.nchunks, step);
 }
+#endif
+
+#ifndef SEM_OP_FP32
+// Diagonal of the element stiffness (glaplace) at every element-local node:
+// the stiffness action on each unit vector e_(a,b) of the element, read at
+// (a, b). One wave per chunk, N*N actions; used once per scaled system.
+template <int N>
+__global__ __launch_bounds__(64) void k_stiff_diag(const double* __restrict__ geo, size_t len,
+                                                   double* __restrict__ out, Dm<N, double> D)
+{
+    using S = Shape<N>;
+    __shared__ double lds0[S::LDS], lds1[S::LDS];
+    const int lane = threadIdx.x;
+    const bool act = lane < S::KN;
+    const int el = act ? lane / N : 0;
+    const int l = act ? lane % N : 0;
+    const Col<N, double> c{(size_t)blockIdx.x * S::K * N * N + lane, act};
+    double ga[N], gb[N], gc[N], dg[N];
+    c.ld(geo, ga);
+    c.ld(geo + len, gb);
+    c.ld(geo + 2 * len, gc);
+#pragma unroll
+    for (int i = 0; i < N; ++i) dg[i] = 0.0;
+    for (int b = 0; b < N; ++b)
+        for (int a = 0; a < N; ++a) {
+            double q[N], y[N];
+#pragma unroll
+            for (int i = 0; i < N; ++i) q[i] = (l == b && i == a) ? 1.0 : 0.0;
+            stiffness<N>(D, lds0, lds1, el, l, act, q, ga, gb, gc, y);
+#pragma unroll
+            for (int i = 0; i < N; ++i) dg[i] = (l == b && i == a) ? y[i] : dg[i];
+        }
+    c.st(out, dg);
+}
+
 #endif
 
 // PCG step (pcg.hpp:70-80, identity preconditioner): p = r (first step) or
@@ -1089,6 +1134,19 @@ void launch_pcg_op_d(int N, const PcgOpArgs<T>& a, int nchunks, const T* d, hipS
 // The translation unit is compiled once per value type (SEM_OP_FP32 set or
 // not) so the two halves of the kernel set build in parallel.
 #ifndef SEM_OP_FP32
+template <int N>
+static void stiff_diag_N(const double* geo, size_t len, int nchunks, double* out, const double* d, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_stiff_diag<N>, dim3(nchunks), dim3(64), 0, s, geo, len, out, make_dm<N, double>(d));
+}
+
+void launch_stiff_diag(int N, const double* geo, size_t len, int nchunks, double* out, const double* d, hipStream_t s)
+{
+#define CALL_SD(n) stiff_diag_N<n>(geo, len, nchunks, out, d, s)
+    SEM_DISPATCH_N(N, CALL_SD)
+#undef CALL_SD
+}
+
 template void launch_element_op_d<double>(int, int, const OpArgs<double>&, const double*, hipStream_t);
 template void launch_minres_op_d<double>(int, int, const MinresOpArgs<double>&, const double*, int, int, hipStream_t,
                                           hipEvent_t, hipEvent_t);

@@ -184,6 +184,48 @@ template void launch_cast<double, float>(double*, const float*, size_t, hipStrea
 template void launch_cast<double, double>(double*, const double*, size_t, hipStream_t);
 template void launch_cast<float, float>(float*, const float*, size_t, hipStream_t);
 
+// out = in * S, component-wise with the per-node scale S (length len) on
+// each of the ncomp components; the product is formed in double.
+template <typename D, typename Src>
+__global__ __launch_bounds__(256) void k_scale_cast(D* out, const Src* in, const float* sc, size_t len, int ncomp)
+{
+    const size_t n = len * (size_t)ncomp;
+    for (size_t k = (size_t)blockIdx.x * 256 + threadIdx.x; k < n; k += (size_t)gridDim.x * 256)
+        out[k] = (D)((double)in[k] * (double)sc[k % len]);
+}
+
+template <typename D, typename Src>
+void launch_scale_cast(D* out, const Src* in, const float* sc, size_t len, int ncomp, hipStream_t s)
+{
+    const size_t n = len * (size_t)ncomp;
+    if (n == 0) return;
+    const int nb = (int)std::min<size_t>((n + 255) / 256, 8192);
+    hipLaunchKernelGGL((k_scale_cast<D, Src>), dim3(nb), dim3(256), 0, s, out, in, sc, len, ncomp);
+}
+template void launch_scale_cast<float, double>(float*, const double*, const float*, size_t, int, hipStream_t);
+template void launch_scale_cast<double, float>(double*, const float*, const float*, size_t, int, hipStream_t);
+
+// Diagonal scaling of the inner Schrodinger-stage system: with the assembled
+// mass diagonal m, stiffness diagonal k and potential-mass diagonal c,
+// S = 1 / sqrt(m + alpha |k + c|) (0 where the denominator is not positive:
+// padding entries).
+__global__ __launch_bounds__(256) void k_inner_scale(float* out, const double* m, const double* k, const double* c,
+                                                     double alpha, size_t n)
+{
+    for (size_t q = (size_t)blockIdx.x * 256 + threadIdx.x; q < n; q += (size_t)gridDim.x * 256) {
+        const double den = m[q] + alpha * fabs(k[q] + c[q]);
+        out[q] = den > 0.0 ? (float)(1.0 / sqrt(den)) : 0.0f;
+    }
+}
+
+void launch_inner_scale(float* out, const double* m, const double* k, const double* c, double alpha, size_t n,
+                        hipStream_t s)
+{
+    if (n == 0) return;
+    const int nb = (int)std::min<size_t>((n + 255) / 256, 8192);
+    hipLaunchKernelGGL(k_inner_scale, dim3(nb), dim3(256), 0, s, out, m, k, c, alpha, n);
+}
+
 // streaming copy: 16-byte accesses (native vector type), four in flight per thread
 typedef double dvec2 __attribute__((ext_vector_type(2)));
 __global__ __launch_bounds__(256) void k_copy16(dvec2* __restrict__ out, const dvec2* __restrict__ in, size_t n2)

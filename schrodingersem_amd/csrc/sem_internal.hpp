@@ -233,6 +233,9 @@ struct MinresFusedArgs {
     const int* c_pos;
     double* sums;              // 3 * nchunks per-chunk sums of this step (out)
     const double* prev_sums;   // the same of the previous step
+    // Schrodinger stage (Res) only: diagonal scaling S of the system, one
+    // value per node; the step applies S A S instead of A (nullptr: none)
+    const float* dscale;
     InstTable it;
     const KrylovScalars* st_in;
     KrylovScalars* st_out;
@@ -273,6 +276,8 @@ void launch_pcg_op_d(int N, const PcgOpArgs<T>& a, int nchunks, const T* d, hipS
 // (reads in[k], writes out[k])
 void launch_minres_close(int kind, const InstTable& it, const KrylovScalars* in, KrylovScalars* out,
                          const double* partial, hipStream_t s);
+// element stiffness diagonal at every element-local node (FP64 geometry)
+void launch_stiff_diag(int N, const double* geo, size_t len, int nchunks, double* out, const double* d, hipStream_t s);
 // fused Lanczos step (one launch per step)
 template <typename T>
 void launch_minres_fused(int N, int kind, const MinresFusedArgs<T>& a, const T* d, int cpb, int grid, hipStream_t s,

@@ -35,8 +35,13 @@ def test_fused_stage_solve_matches_oracle(sem, orc, precision):
         assert abs(r1.n_iter - it) <= 2 and abs(r1.n_iter - r0.n_iter) <= 2
         # phibar at the stop is a product of ~300 Givens sines; the two step
         # orders (and FMA contraction choices of the compiler) round
-        # differently, which moves it by about 1% on this problem
-        assert abs(r1.residual - r0.residual) <= 5e-2 * r0.residual, (r1.residual, r0.residual)
+        # differently, which moves it by several percent on this problem. Both
+        # stop below the threshold at the same step, so each lies within one
+        # step's reduction of it.
+        # stop test minres.hpp:92 with atol = tol: phibar < tol*||b|| + tol
+        thr = 1e-12 * np.sqrt(o.dot(b[:m.size], b[:m.size]) + o.dot(b[m.size:], b[m.size:])) + 1e-12
+        for r in (r0, r1):
+            assert 0.5 * thr < r.residual < thr, (r.residual, thr)
 
 
 def test_fused_helmholtz_and_laplace(sem, orc):
@@ -99,3 +104,34 @@ def test_fused_stepper_matches_two_launch(sem):
             st.close()
     for name in ("fp64", "mixed_learned"):
         assert rel(out[(1, name)], out[(0, name)]) < 1e-9, name
+
+
+def test_inner_scaling_mixed_stage_solve(sem, orc):
+    """Diagonal scaling of the FP32 inner solver: the same FP64 stop test and
+    the same solution to the oracle's tolerance, fewer inner Lanczos steps on
+    a patch with the mesh width (0.156) and degree of the 128 x 128 workload."""
+    xy, q = structured(16, (-2.5, 2.5, -2.5, 2.5), 0.0, 0)
+    p = 8
+    m = sem.Mesh.from_arrays(p, xy, q)
+    o = orc.Mesh.from_arrays(p, xy, q)
+    X, Y = m.coords()
+    V = X * X + Y * Y
+    rng = np.random.default_rng(5)
+    alpha = (np.cos(np.pi / 18.0) / np.sqrt(3.0) + 0.5) * 1e-3
+    b = 1e-3 * o.galerkin(orc.SCHRO_F, rng.standard_normal(2 * m.size), V)
+    zo, ok, it, _ = o.stage_solve(V, alpha, b, None, 5000, 1e-12)
+    zu = np.concatenate([o.unmask(zo[:m.size]), o.unmask(zo[m.size:])])
+    assert ok and sem.get_inner_scaling(m)
+    z1, r1 = sem.stage_solve(m, V, alpha, b, None, 5000, 1e-12, sem.PREC_MIXED)
+    sem.set_inner_scaling(m, False)
+    assert not sem.get_inner_scaling(m)
+    z0, r0 = sem.stage_solve(m, V, alpha, b, None, 5000, 1e-12, sem.PREC_MIXED)
+    sem.set_inner_scaling(m, True)
+    z2, r2 = sem.stage_solve(m, V, alpha, b, None, 5000, 1e-12, sem.PREC_MIXED)
+    assert r0.success and r1.success and r2.success
+    assert rel(z1, zu) < 1e-9 and rel(z0, zu) < 1e-9, (rel(z1, zu), rel(z0, zu))
+    assert np.array_equal(z1, z2) and r1.n_iter == r2.n_iter  # the cached scale gives the same bits
+    assert r1.n_iter < r0.n_iter, (r1.n_iter, r0.n_iter)
+    # FP64 solves do not use it
+    zf, rf = sem.stage_solve(m, V, alpha, b, None, 5000, 1e-12, sem.PREC_FP64)
+    assert rf.success and rel(zf, zu) < 1e-9
