@@ -543,6 +543,11 @@ static double wdot_host(sem_mesh* m, const T* a, const T* b, int ncomp)
 // recurrence as the two-launch path; the operator is applied before the 1/beta
 // scaling instead of after, so results agree to rounding. Buffers: y_k in
 // Y[k % 3], A y in Z[step % 2], w_k in W[k % 3], sums in S[step % 2].
+// Kernel timing mode 1 times one Lanczos step in four, picked by a hash of
+// the step index so that the sample does not follow the batch structure
+// (the first launch after a host check starts on an idle GPU)
+static bool sampled_step(int i) { return (((unsigned)i * 2654435761u) >> 20 & 3u) == 0; }
+
 // the one-launch Lanczos step runs for precision index ti (0 FP64, 1 FP32)
 static bool fused_step(const sem_mesh* m, int ti) { return m->step_mode[ti] == 1 && !m->use_graphs; }
 
@@ -649,7 +654,7 @@ static Results minres_fused1(sem_mesh* m, int kind, const T* cf, double alpha, c
             a.step = i;
             a.first = (b_ == 0);
             a.dscale = dscale;
-            const bool tim = m->timing == 2 || (m->timing == 1 && i % 4 == 1);
+            const bool tim = m->timing == 2 || (m->timing == 1 && sampled_step(i));
             launch_minres_fused<T>(m->L.N, kind, a, m->dmat<T>(), shp.op_cpb, shp.op_grid, s,
                                    tim ? m->ev[4 * b_] : nullptr, tim ? m->ev[4 * b_ + 1] : nullptr);
             ++launched;
@@ -660,7 +665,7 @@ static Results minres_fused1(sem_mesh* m, int kind, const T* cf, double alpha, c
         if (m->timing) {
             const int last = max_iter() + 2;
             for (int b_ = 0; b_ < launched && i0 + b_ <= last; ++b_) {
-                if (m->timing == 1 && (i0 + b_) % 4 != 1) continue;
+                if (m->timing == 1 && !sampled_step(i0 + b_)) continue;
                 float ms = 0;
                 SEM_HIP(hipEventElapsedTime(&ms, m->ev[4 * b_], m->ev[4 * b_ + 1]));
                 m->kt_us[ti] += 1000.0 * ms;
@@ -880,7 +885,7 @@ static Results minres_fused(sem_mesh* m, int kind, const T* cf, double alpha, co
         int cur = 0;
         const int nbatch = (i == 1) ? first_batch : m->batch;
         for (int b_ = 0; b_ < nbatch && i <= o.maxit + 1; ++b_, ++i) {
-            const bool tim = m->timing == 2 || (m->timing == 1 && i % 4 == 1);
+            const bool tim = m->timing == 2 || (m->timing == 1 && sampled_step(i));
             launch_step(i, b_, cur, tim);
             ++launched;
         }
@@ -890,7 +895,7 @@ static Results minres_fused(sem_mesh* m, int kind, const T* cf, double alpha, co
             // launches that ran a full Lanczos step (later ones in the batch exit early)
             const int last = max_iter();
             for (int b_ = 0; b_ < launched && i0 + b_ <= last; ++b_) {
-                if (m->timing == 1 && (i0 + b_) % 4 != 1) continue;
+                if (m->timing == 1 && !sampled_step(i0 + b_)) continue;
                 for (int q = 0; q < 2; ++q) {
                     float ms = 0;
                     SEM_HIP(hipEventElapsedTime(&ms, m->ev[4 * b_ + 2 * q], m->ev[4 * b_ + 2 * q + 1]));
