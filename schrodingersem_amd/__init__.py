@@ -681,15 +681,29 @@ class MlpNet:
         p = np.ascontiguousarray(p, dtype=np.float32)
         _check(lib().sem_mlp_set_params(self._h, p.ctypes.data_as(_fp)))
 
+    @staticmethod
+    def _padded(A, width, pad, what):
+        """rows x width or rows x pad float32 array as a contiguous rows x pad array (zero padding)."""
+        A = np.asarray(A, dtype=np.float32)
+        if A.ndim != 2 or A.shape[1] not in (width, pad):
+            raise ValueError("%s must have %d (or padded %d) columns, got shape %s" % (what, width, pad, A.shape))
+        if A.shape[1] == pad:
+            return np.ascontiguousarray(A)
+        out = np.zeros((A.shape[0], pad), dtype=np.float32)
+        out[:, :width] = A
+        return out
+
     def forward(self, X):
-        X = np.ascontiguousarray(X, dtype=np.float32)
+        X = self._padded(X, self.dims[0], self.in_pad, "X")
         Y = np.zeros((X.shape[0], self.out_pad), dtype=np.float32)
         _check(lib().sem_mlp_forward(self._h, X.shape[0], X.ctypes.data_as(_fp), Y.ctypes.data_as(_fp)))
         return Y
 
     def gradients(self, X, T):
-        X = np.ascontiguousarray(X, dtype=np.float32)
-        T = np.ascontiguousarray(T, dtype=np.float32)
+        X = self._padded(X, self.dims[0], self.in_pad, "X")
+        T = self._padded(T, self.dims[3], self.out_pad, "T")
+        if T.shape[0] != X.shape[0]:
+            raise ValueError("X and T must have the same number of rows")
         loss = C.c_double()
         _check(lib().sem_mlp_gradients(self._h, X.shape[0], X.ctypes.data_as(_fp), T.ctypes.data_as(_fp),
                                        C.byref(loss)))

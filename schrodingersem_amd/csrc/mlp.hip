@@ -38,61 +38,79 @@ __device__ __forceinline__ short f2bf(float f)
 // C[z] = op(A) op(B) (+ bias) over k in [z*kchunk, (z+1)*kchunk); op(A) is M x K,
 // op(B) is K x N. TA: A stored K x M (A(m,k) = A[k*lda + m]); TB: B stored
 // N x K (B(k,n) = B[n*ldb + k]). Split slice z is written at C + z*cstride.
-// 64 x 64 output tile per 256-thread workgroup, 32 x 32 per wavefront.
+// 64 x 64 output tile per 256-thread workgroup, 32 x 32 per wavefront, k
+// tiles of 32. Both operands are staged in LDS as bfloat16 with k
+// contiguous ([m][k] and [n][k]), so each MFMA operand is one 8-byte LDS read;
+// each thread loads 8 consecutive floats of a tile (two 16-byte loads for
+// full, aligned tiles), and the next tile's loads are in flight during the
+// current tile's MFMAs.
 template <bool TA, bool TB>
 __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, const float* __restrict__ A, int lda,
                                               const float* __restrict__ B, int ldb, float* __restrict__ C, int ldc,
                                               size_t cstride, const float* __restrict__ bias, int kchunk)
 {
-    __shared__ short As[64][36];
-    __shared__ short Bs[32][68];
+    constexpr int P = 40;  // LDS row pitch in bfloat16 (80 bytes: 16-byte aligned rows)
+    __shared__ __align__(16) short As[64][P];
+    __shared__ __align__(16) short Bt[64][P];
     const int m0 = blockIdx.y * 64, n0 = blockIdx.x * 64;
     const int kz0 = blockIdx.z * kchunk;
     const int kz1 = min(K, kz0 + kchunk);
     const int t = threadIdx.x, lane = t & 63, w = t >> 6, wm = w >> 1, wn = w & 1;
+    // this thread's 8 values of each tile: A(ma + v*dma, ka + v*dka), B(kb + v*dkb, nb + v*dnb)
+    const int ma = TA ? (t & 7) * 8 : t >> 2, ka = TA ? t >> 3 : (t & 3) * 8;
+    const int nb = TB ? t >> 2 : (t & 7) * 8, kb = TB ? (t & 3) * 8 : t >> 3;
+    const bool vec = (((reinterpret_cast<size_t>(A) | reinterpret_cast<size_t>(B)) & 15) == 0) && (lda & 3) == 0 &&
+                     (ldb & 3) == 0 && m0 + 64 <= M && n0 + 64 <= N;
+    float ra[8], rb[8];
+    auto load = [&](int k0) {
+        const bool full = vec && k0 + 32 <= kz1;
+        const float* pa = TA ? A + (size_t)(k0 + ka) * lda + (m0 + ma) : A + (size_t)(m0 + ma) * lda + (k0 + ka);
+        const float* pb = TB ? B + (size_t)(n0 + nb) * ldb + (k0 + kb) : B + (size_t)(k0 + kb) * ldb + (n0 + nb);
+        if (full) {
+            const float4 a0 = *reinterpret_cast<const float4*>(pa), a1 = *reinterpret_cast<const float4*>(pa + 4);
+            const float4 b0 = *reinterpret_cast<const float4*>(pb), b1 = *reinterpret_cast<const float4*>(pb + 4);
+            ra[0] = a0.x; ra[1] = a0.y; ra[2] = a0.z; ra[3] = a0.w; ra[4] = a1.x; ra[5] = a1.y; ra[6] = a1.z; ra[7] = a1.w;
+            rb[0] = b0.x; rb[1] = b0.y; rb[2] = b0.z; rb[3] = b0.w; rb[4] = b1.x; rb[5] = b1.y; rb[6] = b1.z; rb[7] = b1.w;
+        } else {
+#pragma unroll
+            for (int v = 0; v < 8; ++v) {
+                const int m = TA ? m0 + ma + v : m0 + ma, k = TA ? k0 + ka : k0 + ka + v;
+                ra[v] = (m < M && k < kz1) ? pa[v] : 0.f;
+                const int n = TB ? n0 + nb : n0 + nb + v, kq = TB ? k0 + kb + v : k0 + kb;
+                rb[v] = (n < N && kq < kz1) ? pb[v] : 0.f;
+            }
+        }
+    };
+    auto pack = [](const float* r, int j) {
+        return (unsigned)(unsigned short)f2bf(r[j]) | ((unsigned)(unsigned short)f2bf(r[j + 1]) << 16);
+    };
+    auto store = [&]() {
+        if (TA) {
+#pragma unroll
+            for (int v = 0; v < 8; ++v) As[ma + v][ka] = f2bf(ra[v]);
+        } else {
+            *reinterpret_cast<uint4*>(&As[ma][ka]) = make_uint4(pack(ra, 0), pack(ra, 2), pack(ra, 4), pack(ra, 6));
+        }
+        if (TB) {
+            *reinterpret_cast<uint4*>(&Bt[nb][kb]) = make_uint4(pack(rb, 0), pack(rb, 2), pack(rb, 4), pack(rb, 6));
+        } else {
+#pragma unroll
+            for (int v = 0; v < 8; ++v) Bt[nb + v][kb] = f2bf(rb[v]);
+        }
+    };
     f32x16 acc;
 #pragma unroll
     for (int r = 0; r < 16; ++r) acc[r] = 0.f;
+    if (kz0 < kz1) load(kz0);
     for (int k0 = kz0; k0 < kz1; k0 += 32) {
-        if (!TA) {
-            const int r = t >> 2, kk = (t & 3) * 8, m = m0 + r;
-#pragma unroll
-            for (int v = 0; v < 8; ++v) {
-                const int k = k0 + kk + v;
-                As[r][kk + v] = (m < M && k < kz1) ? f2bf(A[(size_t)m * lda + k]) : (short)0;
-            }
-        } else {
-            const int kk = t >> 3, r = (t & 7) * 8, k = k0 + kk;
-#pragma unroll
-            for (int v = 0; v < 8; ++v) {
-                const int m = m0 + r + v;
-                As[r + v][kk] = (m < M && k < kz1) ? f2bf(A[(size_t)k * lda + m]) : (short)0;
-            }
-        }
-        if (!TB) {
-            const int kk = t >> 3, c = (t & 7) * 8, k = k0 + kk;
-#pragma unroll
-            for (int v = 0; v < 8; ++v) {
-                const int n = n0 + c + v;
-                Bs[kk][c + v] = (k < kz1 && n < N) ? f2bf(B[(size_t)k * ldb + n]) : (short)0;
-            }
-        } else {
-            const int c = t >> 2, kk = (t & 3) * 8, n = n0 + c;
-#pragma unroll
-            for (int v = 0; v < 8; ++v) {
-                const int k = k0 + kk + v;
-                Bs[kk + v][c] = (k < kz1 && n < N) ? f2bf(B[(size_t)n * ldb + k]) : (short)0;
-            }
-        }
+        store();
         __syncthreads();
+        if (k0 + 32 < kz1) load(k0 + 32);
 #pragma unroll
         for (int ks = 0; ks < 32; ks += 8) {
-            bf16x4 a, b;
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                a[v] = As[wm * 32 + (lane & 31)][ks + 4 * (lane >> 5) + v];
-                b[v] = Bs[ks + 4 * (lane >> 5) + v][wn * 32 + (lane & 31)];
-            }
+            const int kq = ks + 4 * (lane >> 5);
+            const bf16x4 a = *reinterpret_cast<const bf16x4*>(&As[wm * 32 + (lane & 31)][kq]);
+            const bf16x4 b = *reinterpret_cast<const bf16x4*>(&Bt[wn * 32 + (lane & 31)][kq]);
             acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc, 0, 0, 0);
         }
         __syncthreads();

@@ -63,3 +63,16 @@ def test_gpu_entry_points_fail_loudly_without_gpu(sem):
         pytest.skip("a GPU is present")
     with pytest.raises(sem.SemError, match="HIP|device"):
         sem.load_mesh(3, SMALL_MESH)
+
+
+def test_mlp_host_arrays_are_padded_or_rejected(sem):
+    """The network binding takes rows x n_in (zero-padded here) or rows x in_pad
+    arrays and rejects any other width before the library reads them."""
+    P = sem.MlpNet._padded(np.ones((3, 5), np.float64), 5, 8, "X")
+    assert P.shape == (3, 8) and P.dtype == np.float32 and P.flags.c_contiguous
+    assert np.array_equal(P[:, :5], np.ones((3, 5))) and not P[:, 5:].any()
+    Q = sem.MlpNet._padded(np.arange(16, dtype=np.float32).reshape(2, 8), 5, 8, "X")
+    assert np.array_equal(Q, np.arange(16, dtype=np.float32).reshape(2, 8))
+    for bad in (np.ones((3, 6)), np.ones(5)):
+        with pytest.raises(ValueError):
+            sem.MlpNet._padded(bad, 5, 8, "X")
