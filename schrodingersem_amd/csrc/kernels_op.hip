@@ -697,7 +697,9 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 3 : 1, 8))) void k_minres_fu
         // the recurrence's sums are loaded first, so that its wait does not
         // include the gather
         const bool rec = fresh && !a.first && a.step >= 2 && !sc.done;
-        GroupSum3<NW, 8> gs;
+        // all of a 128 x 128 mesh's sums in one round of loads up to N = 10;
+        // fewer registers held across the gather for larger elements
+        GroupSum3<NW, (N <= 10 ? 12 : 4)> gs;
         if (rec) gs.issue(a.prev_sums + ci.y, nch, ci.z - ci.y);
         ColumnGather<N, T> g;
         T r2[N], r3[N];  // y_{i-2}, y_{i-3} of this component
