@@ -201,6 +201,8 @@ def bench_learned(sem, args):
         opts = sem.SolverOpts(max_iter=100000, tol=args.tol5, dt=dt_ref, precision=sem.PREC_MIXED, guess=guess,
                               learn_steps=args.learn_steps)
         st = sem.Stepper(m, 2, re, im, Vref, opts)
+        if args.learn_epochs and guess == sem.GUESS_LEARNED:
+            st.set_learning(args.learn_epochs, 3e-3)
         phases = {}
         for phase, nsteps in (("training", args.learn_steps), ("frozen", args.steps5 - args.learn_steps)):
             s0 = st.stats()
@@ -246,9 +248,11 @@ def bench_learned(sem, args):
     return 0
 
 
-def run_mode(sem, m, Vref, re, im, dt, tol, steps, warmup, precision, guess):
+def run_mode(sem, m, Vref, re, im, dt, tol, steps, warmup, precision, guess, epochs=0):
     opts = sem.SolverOpts(max_iter=100000, tol=tol, dt=dt, precision=precision, guess=guess, learn_steps=warmup)
     st = sem.Stepper(m, 1, re, im, Vref, opts)
+    if epochs and guess == sem.GUESS_LEARNED:
+        st.set_learning(epochs, 3e-3)
     tw = time.perf_counter()
     if warmup:
         assert st.advance(1e30, warmup)
@@ -288,6 +292,8 @@ def main():
     ap.add_argument("--p5", type=int, default=7)
     ap.add_argument("--steps5", type=int, default=100)
     ap.add_argument("--learn-steps", type=int, default=50)
+    ap.add_argument("--learn-epochs", type=int, default=0,
+                    help="Adam steps per training sample of the guess network (0: the library default)")
     ap.add_argument("--tol5", type=float, default=1e-10)
     ap.add_argument("--instances", type=int, default=64)
     ap.add_argument("--batch-steps", type=int, default=100)
@@ -325,7 +331,7 @@ def main():
                               ("mixed_ext", sem.PREC_MIXED, sem.GUESS_EXTRAPOLATED),
                               ("mixed_learned", sem.PREC_MIXED, sem.GUESS_LEARNED)):
         log("timing mode", name)
-        r = run_mode(sem, m, Vref, re, im, dt_ref, args.tol, args.steps, args.warmup, prec, guess)
+        r = run_mode(sem, m, Vref, re, im, dt_ref, args.tol, args.steps, args.warmup, prec, guess, args.learn_epochs)
         r["err"] = float(np.linalg.norm(r["psi"] - truth) / np.linalg.norm(truth))
         log("  %s: %.3f s for %d steps, %d stage solves, %d inner iterations, rel-L2 vs tol-1e-14 run %.2e, ok=%s" %
             (name, r["seconds"], args.steps, r["stage_solves"], r["iters"], r["err"], r["ok"]))

@@ -191,13 +191,20 @@ __global__ __launch_bounds__(256) void k_mse(const float* __restrict__ Y, const 
     if (threadIdx.x == 0) part[blockIdx.x] = ((sh[0] + sh[1]) + sh[2]) + sh[3];
 }
 
-__global__ void k_sum_double(const double* p, int n, double* out)
+// sum of n doubles in a fixed order: thread t adds p[t], p[t+256], ... in
+// turn, then a pairwise tree over the 256 thread sums (one workgroup of 256)
+__global__ __launch_bounds__(256) void k_sum_double(const double* p, int n, double* out)
 {
-    if (threadIdx.x == 0 && blockIdx.x == 0) {
-        double s = 0;
-        for (int i = 0; i < n; ++i) s += p[i];
-        *out = s;
+    __shared__ double sh[256];
+    double s = 0;
+    for (int i = threadIdx.x; i < n; i += 256) s += p[i];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+        __syncthreads();
     }
+    if (threadIdx.x == 0) *out = sh[0];
 }
 
 // Adam (Kingma & Ba 2015): m = b1 m + (1-b1) g; v = b2 v + (1-b2) g^2;
@@ -407,7 +414,7 @@ double Mlp::backward(int rows, bool host_loss)
     forward(rows);
     const int nb = std::min(grid_for((size_t)rows * out_pad), 4096);
     hipLaunchKernelGGL(k_mse, dim3(nb), dim3(256), 0, stream, Y, T, dY, rows, out_pad, out, red + 16);
-    hipLaunchKernelGGL(k_sum_double, dim3(1), dim3(64), 0, stream, red + 16, nb, red);
+    hipLaunchKernelGGL(k_sum_double, dim3(1), dim3(256), 0, stream, red + 16, nb, red);
     const float *W2 = params + off[2], *W3 = params + off[4];
     float *gW1 = grads + off[0], *gb1 = grads + off[1], *gW2 = grads + off[2], *gb2 = grads + off[3],
           *gW3 = grads + off[4], *gb3 = grads + off[5];
