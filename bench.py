@@ -14,8 +14,9 @@ and psi conjugated on input and output.
 Modes timed (same inputs, same warm-up window, state resident in HBM before
 the timed region):
   fp64           reference arithmetic (FP64 fused MINRES, zero stage guesses)
-  mixed_ext      FP32 inner MINRES inside FP64 iterative refinement,
-                 extrapolated stage guesses, true FP64 residual as the stop test
+  mixed_ext      FP32 inner MINRES (on the diagonally scaled system) inside
+                 FP64 iterative refinement, extrapolated stage guesses, true
+                 FP64 residual as the stop test
   mixed_learned  as mixed_ext, plus the per-element MLP correction of the
                  extrapolated guess; the network trains on the warm-up steps'
                  stage solutions (the warm-up is its training window) and is
@@ -29,9 +30,9 @@ kernels on this mesh for each precision (results do not depend on it).
 Also measured: live HIP-event timing of the dominant kernel (the MINRES
 Lanczos-step kernel of the reported mode: the fused one-launch step for the
 FP32 inner solver, the operator kernel of the two-launch step for FP64;
-sampled every fourth Lanczos step) over the timed region -> roofline; glaplace
-GB/s on a 512 x 512, degree 7 mesh (671 MB of algorithmic traffic, larger
-than the 256 MB Infinity Cache); the CPU oracle timed on one stage solve of
+one Lanczos step in four sampled) over the timed region -> roofline; glaplace
+GB/s (FP64 and FP32) on a 512 x 512, degree 7 mesh (671 MB of FP64
+algorithmic traffic, larger than the 256 MB Infinity Cache); the CPU oracle timed on one stage solve of
 the same workload (bounded sample) -> cpu_baseline.
 """
 import argparse

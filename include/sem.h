@@ -216,7 +216,9 @@ int sem_pcg_tridiag(long n, double lo, double di, double up, const double* b, do
  * per launch and the algorithmic bytes per launch. */
 int sem_time_element_op(sem_mesh* m, int kind, int precision, int reps, double* us_per_launch, double* bytes_per_launch);
 /* Live timing of the two MINRES Lanczos-step kernels: on = 1 times the
- * launches of every fourth Lanczos step with HIP events that take the
+ * launches of one Lanczos step in four (picked by a hash of the step index,
+ * so the sample does not follow the batches between host checks) with HIP
+ * events that take the
  * kernels' own start and end timestamps (hipExtLaunchKernel), on = 2 every
  * step, 0 turns it off. which = 0 / 1:
  * operator kernel (operator apply, y -= cr1*r1, alpha partial) FP64 / FP32;

@@ -150,8 +150,8 @@ struct sem_mesh {
     sem_run_stats stats{};
     int batch = 8;          // Lanczos steps launched between host checks of the stop flag
     int iter_hint[8] = {};  // iterations of the last MINRES solve per hint slot
-    // live timing of the two Lanczos-step kernels: 0 off, 1 every fourth
-    // Lanczos step, 2 every step
+    // live timing of the two Lanczos-step kernels: 0 off, 1 one Lanczos step
+    // in four (sampled_step), 2 every step
     int timing = 0;
     std::vector<hipEvent_t> ev;
     // slots: 0/1 operator kernel FP64/FP32, 2/3 update kernel FP64/FP32
