@@ -589,9 +589,11 @@ static Results minres_fused1(sem_mesh* m, int kind, const T* cf, double alpha, c
     const int ti = sizeof(T) == 8 ? 0 : 1;
     const StepShape shp = m->shape[ti];
     const int slot = o.hint_slot >= 0 ? std::min(o.hint_slot, 7) : ti;
-    // two more launches than iterations: the stop test of iteration k is
-    // evaluated at the start of step k + 2
-    const int first_batch = std::max(m->batch, (int)(0.85 * m->iter_hint[slot]) + 2);
+    // first batch: the previous solve's count, plus the two launches by which
+    // the stop test trails (iteration k is tested at the start of step k + 2),
+    // plus a margin of two: launches past an instance's stop exit after
+    // reading its state, which is cheaper than another host check
+    const int first_batch = std::max(m->batch, m->iter_hint[slot] + 4);
     if (m->timing) {
         while ((int)m->ev.size() < 4 * first_batch) {
             hipEvent_t e;
