@@ -180,6 +180,9 @@ int sem_batch_autotune(sem_batch* b, int precision, int steps, int* shapes);
 int sem_batch_set_step_shape(sem_batch* b, int group, int precision, const int* shape);
 /* sem_set_graph_mode on every group's mesh */
 int sem_batch_set_graph_mode(sem_batch* b, int on, int steps);
+/* sem_set_step_mode on every group of the batch (with graph mode on, the
+ * graph path's two-launch step is used whatever the mode). */
+int sem_batch_set_step_mode(sem_batch* b, int precision, int mode);
 /* schrodinger_bdf2 (schrodinger.hpp:22-128) on every instance to time T with
  * one set of options (FP64; reference or extrapolated guess). re[k], im[k]
  * (in/out) and V[k] are instance k's element-local arrays. ok[k] (optional):

@@ -114,6 +114,7 @@ def lib():
             "sem_set_inner_scaling": (C.c_int, [C.c_void_p, C.c_int]),
             "sem_get_inner_scaling": (C.c_int, [C.c_void_p, _ip]),
             "sem_batch_set_graph_mode": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
+            "sem_batch_set_step_mode": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
             "sem_autotune": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, _ip, _ip, _dp]),
             "sem_stepper_create": (C.c_int, [C.c_void_p, C.c_int, _dp, _dp, _dp, C.POINTER(_Opts),
                                              C.POINTER(C.c_void_p)]),
@@ -431,6 +432,10 @@ class Batch:
 
     def set_graph_mode(self, on, steps=12):
         _check(lib().sem_batch_set_graph_mode(self._h, int(bool(on)), int(steps)))
+
+    def set_step_mode(self, mode, precision=PREC_FP64):
+        """Lanczos step of every group: 0 two launches, 1 one fused launch (graph mode off)."""
+        _check(lib().sem_batch_set_step_mode(self._h, int(precision), int(mode)))
 
     def schrodinger_bdf2(self, psi_real, psi_imag, potentials, T, opts):
         """schrodinger_bdf2 on every instance; psi arrays updated in place; returns per-instance success."""

@@ -2350,6 +2350,16 @@ int sem_batch_set_graph_mode(sem_batch* b, int on, int steps)
     return SEM_OK;
 }
 
+int sem_batch_set_step_mode(sem_batch* b, int precision, int mode)
+{
+    if (!b) return SEM_EINVAL;
+    for (auto& g : b->groups) {
+        const int rc = sem_set_step_mode(g.m.get(), precision, mode);
+        if (rc != SEM_OK) return rc;
+    }
+    return SEM_OK;
+}
+
 int sem_batch_set_step_shape(sem_batch* b, int group, int precision, const int* shape)
 {
     if (!b || group < 0 || group >= (int)b->groups.size()) return SEM_EINVAL;

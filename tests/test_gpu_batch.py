@@ -111,3 +111,33 @@ def test_batch_session_matches_one_shot(sem):
     assert all(ok) and batch.last_run_stats()["steps"] == steps
     for k in range(len(inst)):
         assert np.array_equal(r2[k], re[k]) and np.array_equal(i2[k], im[k])
+
+
+def test_batch_fused_step_bitwise(sem):
+    """The one-launch Lanczos step over a batch: per-instance recurrences and
+    stop tests (instances stop at different steps; the kernel skips the items
+    of stopped ones), each instance equal bit for bit to its single run with
+    the same step, in one-item and persistent work shapes."""
+    dt, steps = 2e-3, 3
+    T = (steps - 0.5) * dt
+    inst = [instance(sem, n, p, s) for n, p, s in ((7, 4, 41), (3, 4, 42), (9, 4, 43), (5, 6, 44))]
+    opts = sem.SolverOpts(max_iter=5000, tol=1e-12, dt=dt)
+    single = []
+    for m, _, _, V, re, im in inst:
+        sem.set_step_mode(m, 1, sem.PREC_FP64)
+        sem.set_graph_mode(m, False)
+        a, b = re.copy(), im.copy()
+        assert sem.schrodinger_bdf2(a, b, V, m, T, opts)
+        single.append((a, b))
+    batch = sem.Batch([x[0] for x in inst])
+    batch.set_graph_mode(False)
+    batch.set_step_mode(1)
+    for shape in ((2, 0, 1, 0), (1, 3, 1, 0), (2, 64, 1, 0)):
+        for g in range(len(batch.info())):
+            batch.set_step_shape(g, shape)
+        re = [x[4].copy() for x in inst]
+        im = [x[5].copy() for x in inst]
+        assert all(batch.schrodinger_bdf2(re, im, [x[3] for x in inst], T, opts))
+        for k in range(len(inst)):
+            assert np.array_equal(re[k], single[k][0]) and np.array_equal(im[k], single[k][1]), (shape, k)
+    batch.close()
