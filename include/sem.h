@@ -252,7 +252,7 @@ int sem_set_step_mode(sem_mesh* m, int precision, int mode);
 int sem_get_step_mode(const sem_mesh* m, int precision, int* mode);
 /* Diagonal scaling of the FP32 inner MINRES in mixed-precision Schrodinger
  * stage solves (on = 1, the default): the inner solver works on S Res S with
- * S = (m + alpha |k + c|)^(-1/2) from the assembled mass, stiffness and
+ * S = (m + |alpha| |k + c|)^(-1/2) from the assembled mass, stiffness and
  * potential-mass diagonals, which takes fewer Lanczos steps; the outer FP64
  * refinement stops on the true residual either way. It applies when the FP32
  * step mode is 1 and graph mode is off. No counterpart in the reference

@@ -1080,7 +1080,7 @@ static Results pcg_fused(sem_mesh* m, const double* b, double* x, bool x_zero, i
 
 // ------------------------------------------------ mixed precision solve --
 // Diagonal scaling of the Schrodinger-stage system Res = [[M, aL], [aL, -M]]
-// (L = K + V M): S = 1 / sqrt(m + a |k + c|) from the assembled diagonals of
+// (L = K + V M): S = 1 / sqrt(m + |a| |k + c|) from the assembled diagonals of
 // the mass (m), stiffness (k) and potential mass (c) matrices. The inner
 // FP32 MINRES then solves (S Res S) w = S r and returns d = S w; on the
 // Schrodinger workloads this takes 15-20% fewer Lanczos steps per decade of

@@ -207,13 +207,13 @@ template void launch_scale_cast<double, float>(double*, const float*, const floa
 
 // Diagonal scaling of the inner Schrodinger-stage system: with the assembled
 // mass diagonal m, stiffness diagonal k and potential-mass diagonal c,
-// S = 1 / sqrt(m + alpha |k + c|) (0 where the denominator is not positive:
-// padding entries).
+// S = 1 / sqrt(m + |alpha| |k + c|) (any positive diagonal keeps S A S
+// symmetric; 0 where the denominator is not positive: padding entries).
 __global__ __launch_bounds__(256) void k_inner_scale(float* out, const double* m, const double* k, const double* c,
                                                      double alpha, size_t n)
 {
     for (size_t q = (size_t)blockIdx.x * 256 + threadIdx.x; q < n; q += (size_t)gridDim.x * 256) {
-        const double den = m[q] + alpha * fabs(k[q] + c[q]);
+        const double den = m[q] + fabs(alpha) * fabs(k[q] + c[q]);
         out[q] = den > 0.0 ? (float)(1.0 / sqrt(den)) : 0.0f;
     }
 }

@@ -59,7 +59,7 @@ void launch_cast(D* out, const S* in, size_t n, hipStream_t s);
 // out = in * S on each of ncomp components (S: one value per node, len nodes)
 template <typename D, typename S>
 void launch_scale_cast(D* out, const S* in, const float* sc, size_t len, int ncomp, hipStream_t s);
-// S = 1 / sqrt(m + alpha |k + c|) (0 where m + alpha |k + c| <= 0)
+// S = 1 / sqrt(m + |alpha| |k + c|) (0 where that is not positive)
 void launch_inner_scale(float* out, const double* m, const double* k, const double* c, double alpha, size_t n,
                         hipStream_t s);
 template <typename T>
