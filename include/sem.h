@@ -88,6 +88,11 @@ typedef struct sem_mesh sem_mesh;
 
 const char* sem_last_error(void);
 int sem_version(void);
+/* The progress line the steppers print with opts->verbose: VerboseTracker's
+ * operator() (verbose.hpp:33-44) with precision int(2 - log10 dt)
+ * (schrodinger.hpp:28); with_error adds the DIRK4S5 error estimate
+ * (schrodinger.hpp:271). Host only, no GPU. */
+int sem_progress_line(double t, double T, double dt, int with_error, double err, char* out, int out_len);
 int sem_device_info(char* name, int name_len, int* n_cu, long* hbm_bytes);
 
 /* ----- mesh setup: load_mesh<Mesh>(p, folder) + Mesh::compute_metrics()
@@ -161,6 +166,20 @@ int sem_stepper_set_learning(sem_stepper* s, int epochs, float lr);
  * only when its true FP64 residual is below the extrapolation's (two extra
  * operator applies per stage solve); off always starts from the network's guess */
 int sem_stepper_set_guard(sem_stepper* s, int on);
+/* Checkpoint and resume of a session (no counterpart in the reference, whose
+ * steppers keep their BDF2 history, schrodinger.hpp:85-102, in a local deque
+ * and restart with implicit Euler). A checkpoint is one host array of
+ * sem_stepper_checkpoint_size doubles: t, the step count, psi, the BDF2
+ * history, the previous two steps' stage solutions (extrapolated / learned
+ * guesses) and the guess network with its optimiser state, every vector in
+ * the element-local layout with the node's value on every copy. restore()
+ * takes a checkpoint of the same scheme and mesh size into a session created
+ * with any options: psi, t, step count and BDF2 history always, the stage
+ * history when the session extrapolates, the network when it has one of the
+ * same size. A run split by save/restore equals the one-shot run bit for bit. */
+int sem_stepper_checkpoint_size(const sem_stepper* s, long* n);
+int sem_stepper_save(sem_stepper* s, double* buf, long n);
+int sem_stepper_restore(sem_stepper* s, const double* buf, long n);
 void sem_stepper_destroy(sem_stepper* s);
 /* Batched instances (no counterpart in the reference, which runs one problem
  * per call): independent meshes advanced together. Instances of one degree
@@ -186,8 +205,10 @@ int sem_batch_set_step_mode(sem_batch* b, int precision, int mode);
 /* schrodinger_bdf2 (schrodinger.hpp:22-128) on every instance to time T with
  * one set of options (FP64; reference or extrapolated guess). re[k], im[k]
  * (in/out) and V[k] are instance k's element-local arrays. ok[k] (optional):
- * every solve of instance k converged. A group stops stepping when any of its
- * instances fails to converge, as the reference stops that instance. */
+ * every solve of instance k converged. An instance whose solve fails stops
+ * there, as the reference's stepper returns (BDF2 keeps the unconverged
+ * iterate and counts the step); the other instances go on to T
+ * (sem_batch_times gives each instance's time and step count). */
 int sem_batch_schrodinger_bdf2(sem_batch* b, double* const* re, double* const* im, const double* const* V, double T,
                                const sem_solver_opts* opts, int* ok);
 /* The same as a session kept on the GPU: start, advance (the while (t < T)
@@ -197,6 +218,9 @@ int sem_batch_start_bdf2(sem_batch* b, const double* const* re, const double* co
                          const sem_solver_opts* opts);
 int sem_batch_advance(sem_batch* b, double T, long max_steps, int* success);
 int sem_batch_read(sem_batch* b, double* const* re, double* const* im, int* ok, double* t);
+/* per-instance time and completed steps of the session (arrays of n entries;
+ * either may be null): an instance stopped by a failed solve keeps its own */
+int sem_batch_times(const sem_batch* b, double* t, int* steps);
 /* statistics of the last batch run (steps, stage solves and Lanczos steps summed
  * over groups) and its wall time in seconds */
 int sem_batch_last_run_stats(const sem_batch* b, sem_run_stats* out, double* seconds);

@@ -68,6 +68,9 @@ def lib():
         sig = {
             "sem_last_error": (C.c_char_p, []),
             "sem_version": (C.c_int, []),
+            "sem_progress_line": (C.c_int, [C.c_double, C.c_double, C.c_double, C.c_int, C.c_double, C.c_char_p,
+                                            C.c_int]),
+            "sem_batch_times": (C.c_int, [C.c_void_p, _dp, _ip]),
             "sem_device_info": (C.c_int, [C.c_char_p, C.c_int, _ip, _lp]),
             "sem_mesh_create": (C.c_int, [C.c_int, _dp, C.c_int, _ip, C.c_int, C.POINTER(C.c_void_p)]),
             "sem_mesh_load": (C.c_int, [C.c_int, C.c_char_p, C.POINTER(C.c_void_p)]),
@@ -124,6 +127,9 @@ def lib():
             "sem_stepper_destroy": (None, [C.c_void_p]),
             "sem_stepper_set_learning": (C.c_int, [C.c_void_p, C.c_int, C.c_float]),
             "sem_stepper_set_guard": (C.c_int, [C.c_void_p, C.c_int]),
+            "sem_stepper_checkpoint_size": (C.c_int, [C.c_void_p, _lp]),
+            "sem_stepper_save": (C.c_int, [C.c_void_p, _dp, C.c_long]),
+            "sem_stepper_restore": (C.c_int, [C.c_void_p, _dp, C.c_long]),
             "sem_batch_create": (C.c_int, [C.c_int, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)]),
             "sem_batch_destroy": (None, [C.c_void_p]),
             "sem_batch_info": (C.c_int, [C.c_void_p, _ip, _ip, _ip, _lp]),
@@ -478,6 +484,14 @@ class Batch:
                                     ok, C.byref(t)))
         return re, im, [bool(v) for v in ok], t.value
 
+    def times(self):
+        """per-instance (time, completed steps) of the session"""
+        n = len(self.meshes)
+        t = np.zeros(n)
+        st = (C.c_int * n)()
+        _check(lib().sem_batch_times(self._h, _ptr(t), st))
+        return t, [int(v) for v in st]
+
     def last_run_stats(self):
         s = _Stats()
         sec = C.c_double()
@@ -544,6 +558,19 @@ class Stepper:
         """Learned guess: keep the extrapolation when its true residual is smaller (default on)."""
         _check(lib().sem_stepper_set_guard(self._h, int(bool(on))))
 
+    def save(self):
+        """Checkpoint of the session (t, step count, psi, BDF2 history, stage history, guess network)."""
+        n = C.c_long()
+        _check(lib().sem_stepper_checkpoint_size(self._h, C.byref(n)))
+        buf = np.zeros(n.value)
+        _check(lib().sem_stepper_save(self._h, _ptr(buf), n.value))
+        return buf
+
+    def restore(self, checkpoint):
+        """Resume from a checkpoint of the same scheme and mesh size (see sem_stepper_restore)."""
+        buf = _f64(checkpoint)
+        _check(lib().sem_stepper_restore(self._h, _ptr(buf), buf.size))
+
     def close(self):
         if getattr(self, "_h", None) is not None and self._h.value and _lib is not None:
             _lib.sem_stepper_destroy(self._h)
@@ -551,6 +578,14 @@ class Stepper:
 
     def __del__(self):
         self.close()
+
+
+def progress_line(t, T, dt, err=None):
+    """The steppers' verbose progress line (VerboseTracker, verbose.hpp:33-44)."""
+    buf = C.create_string_buffer(256)
+    _check(lib().sem_progress_line(float(t), float(T), float(dt), int(err is not None),
+                                   0.0 if err is None else float(err), buf, 256))
+    return buf.value.decode()
 
 
 def time_copy(nbytes, reps=20, variant=1):

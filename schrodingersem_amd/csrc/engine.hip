@@ -56,7 +56,19 @@ struct KrylovOpts {
     double atol = 0;      // absolute term of the stop test (tol in the reference)
     bool rel_to_b = false; // stop relative to ||b|| instead of ||b - A x0||
     int hint_slot = -1;    // iteration-count history slot (default: by precision)
+    // batched meshes: instances (nonzero entries) that take no part in this solve;
+    // their Krylov state starts stopped, so x keeps its value there
+    const uint8_t* frozen = nullptr;
 };
+
+// marks instance k's Krylov state stopped before the first step (not converged)
+inline void freeze_instance(KrylovScalars& h, const uint8_t* frozen, int k)
+{
+    if (frozen && frozen[k]) {
+        h.done = 1;
+        h.success = 0;
+    }
+}
 
 }  // namespace sem
 
@@ -546,6 +558,10 @@ static double wdot_host(sem_mesh* m, const T* a, const T* b, int ncomp)
 // Kernel timing mode 1 times one Lanczos step in four, picked by a hash of
 // the step index so that the sample does not follow the batch structure
 // (the first launch after a host check starts on an idle GPU)
+// Upper bound of the first batch of Lanczos steps launched before a host
+// check of the stop flags (and of the timing events kept per batch).
+constexpr int kMaxFirstBatch = 2048;
+
 static bool sampled_step(int i) { return (((unsigned)i * 2654435761u) >> 20 & 3u) == 0; }
 
 // the one-launch Lanczos step runs for precision index ti (0 FP64, 1 FP32)
@@ -578,11 +594,12 @@ static Results minres_fused1(sem_mesh* m, int kind, const T* cf, double alpha, c
     wdot_dev<T>(m, Y[0], Y[0], nc, 0);
     launch_minres_init(m->sc, ni, 0, o.maxit, o.tol, s);
     if (o.rel_to_b) wdot_dev<T>(m, b, b, nc, 1);
-    if (o.rel_to_b || o.atol != o.tol) {
+    if (o.rel_to_b || o.atol != o.tol || o.frozen) {
         m->read_scalars();
         for (int k = 0; k < ni; ++k) {
             if (o.rel_to_b) m->h_sc[k].b_norm = std::sqrt(m->h_sc[k].red[1]);
             m->h_sc[k].atol = o.atol;
+            freeze_instance(m->h_sc[k], o.frozen, k);
         }
         SEM_HIP(hipMemcpyAsync(m->sc, m->h_sc, ni * sizeof(KrylovScalars), hipMemcpyHostToDevice, s));
     }
@@ -593,7 +610,7 @@ static Results minres_fused1(sem_mesh* m, int kind, const T* cf, double alpha, c
     // the stop test trails (iteration k is tested at the start of step k + 2),
     // plus a margin of two: launches past an instance's stop exit after
     // reading its state, which is cheaper than another host check
-    const int first_batch = std::max(m->batch, m->iter_hint[slot] + 4);
+    const int first_batch = std::max(m->batch, std::min(m->iter_hint[slot] + 4, kMaxFirstBatch));
     if (m->timing) {
         while ((int)m->ev.size() < 4 * first_batch) {
             hipEvent_t e;
@@ -617,6 +634,11 @@ static Results minres_fused1(sem_mesh* m, int kind, const T* cf, double alpha, c
     auto all_done = [&]() {
         for (int k = 0; k < ni; ++k)
             if (!m->h_sc[k].done) return false;
+        return true;
+    };
+    auto all_success = [&]() {
+        for (int k = 0; k < ni; ++k)
+            if (!m->h_sc[k].success) return false;
         return true;
     };
     int i = 1;
@@ -700,7 +722,9 @@ static Results minres_fused1(sem_mesh* m, int kind, const T* cf, double alpha, c
         if (o.maxit < 1) r.n_iter = 1;
         r.residual = h.phibar;
     }
-    m->iter_hint[slot] = max_iter();
+    // the next solve's first batch follows this one's count, if it converged
+    // (a solve that ran to maxit says nothing about the next one)
+    if (all_success()) m->iter_hint[slot] = max_iter();
     Results all = m->inst_results[0];
     for (int k = 1; k < ni; ++k) {
         all.success = all.success && m->inst_results[k].success;
@@ -749,12 +773,13 @@ static Results minres_fused(sem_mesh* m, int kind, const T* cf, double alpha, co
     if (o.rel_to_b) {  // b_norm = ||b|| (fair stop test for nonzero guesses)
         wdot_dev<T>(m, b, b, nc, 1);
     }
-    if (o.rel_to_b || o.atol != o.tol) {
+    if (o.rel_to_b || o.atol != o.tol || o.frozen) {
         // stop test phibar < tol * b_ref + atol (minres.hpp:92 has b_ref = ||r0||, atol = tol)
         m->read_scalars();
         for (int k = 0; k < ni; ++k) {
             if (o.rel_to_b) m->h_sc[k].b_norm = std::sqrt(m->h_sc[k].red[1]);
             m->h_sc[k].atol = o.atol;
+            freeze_instance(m->h_sc[k], o.frozen, k);
         }
         SEM_HIP(hipMemcpyAsync(m->sc, m->h_sc, ni * sizeof(KrylovScalars), hipMemcpyHostToDevice, s));
     }
@@ -765,7 +790,7 @@ static Results minres_fused(sem_mesh* m, int kind, const T* cf, double alpha, co
     // host round trip (the previous solve of this precision predicts it);
     // steps launched after the stop flag is set exit at once.
     const int slot = o.hint_slot >= 0 ? std::min(o.hint_slot, 7) : ti;
-    const int first_batch = std::max(m->batch, (int)(0.85 * m->iter_hint[slot]));
+    const int first_batch = std::max(m->batch, std::min((int)(0.85 * m->iter_hint[slot]), kMaxFirstBatch));
     if (m->timing) {
         while ((int)m->ev.size() < 4 * first_batch) {
             hipEvent_t e;
@@ -797,6 +822,11 @@ static Results minres_fused(sem_mesh* m, int kind, const T* cf, double alpha, co
     auto all_done = [&]() {
         for (int k = 0; k < ni; ++k)
             if (!m->h_sc[k].done) return false;
+        return true;
+    };
+    auto all_success = [&]() {
+        for (int k = 0; k < ni; ++k)
+            if (!m->h_sc[k].success) return false;
         return true;
     };
     // launches of Lanczos step i, the b_-th of its batch (state buffer cur)
@@ -930,7 +960,9 @@ static Results minres_fused(sem_mesh* m, int kind, const T* cf, double alpha, co
         if (o.maxit < 1) r.n_iter = 1;
         r.residual = h.phibar;
     }
-    m->iter_hint[slot] = max_iter();
+    // the next solve's first batch follows this one's count, if it converged
+    // (a solve that ran to maxit says nothing about the next one)
+    if (all_success()) m->iter_hint[slot] = max_iter();
     // aggregate: success of every instance, the longest iteration count, the largest residual
     Results all = m->inst_results[0];
     for (int k = 1; k < ni; ++k) {
@@ -1241,6 +1273,16 @@ struct Stepper {
     double alpha_bdf = 0;
     sem_run_stats stats{};
     std::vector<int> inst_ok;  // per instance: every solve so far succeeded
+    // batched meshes: an instance whose solve fails stops there, as the
+    // reference's stepper returns (BDF2 keeps the unconverged iterate and
+    // counts the step, schrodinger.hpp:96-114; the SDIRK schemes keep psi of
+    // the step's start, :222-226), while the other instances go on
+    std::vector<uint8_t> skip;       // instances left out of the solves (stopped, or failed in this step)
+    std::vector<uint8_t> frozen;     // stopped in an earlier step
+    std::vector<uint8_t> failed_now; // a solve of this step failed
+    std::vector<double> inst_t;      // per-instance time
+    std::vector<int> inst_steps;     // per-instance completed steps
+    bool multi() const { return m->n_inst > 1; }
     // learned guess (guess = 2): one network for all stages, told the stage
     // by one-hot input columns
     std::unique_ptr<Mlp> mlp;
@@ -1282,6 +1324,7 @@ static Results solve_res(Stepper& st, double alpha, const double* b, double* x, 
     ko.tol = st.opts.tol;
     ko.atol = st.opts.tol;
     ko.rel_to_b = rel_to_b;
+    if (st.multi()) ko.frozen = st.skip.data();
     Results r;
     const long ref0 = m->stats.refinements;
     if (st.opts.precision == SEM_PREC_MIXED) {
@@ -1290,7 +1333,18 @@ static Results solve_res(Stepper& st, double alpha, const double* b, double* x, 
     } else
         r = minres_fused<double>(m, K_RES, st.V.c64, alpha, b, x, zero, ko);
     if (st.inst_ok.size() != (size_t)m->n_inst) st.inst_ok.assign(m->n_inst, 1);
-    for (int k = 0; k < m->n_inst; ++k) st.inst_ok[k] = st.inst_ok[k] && m->inst_results[k].success;
+    for (int k = 0; k < m->n_inst; ++k) {
+        if (st.multi() && st.skip[k]) continue;
+        if (!m->inst_results[k].success) {
+            st.inst_ok[k] = 0;
+            if (st.multi()) st.failed_now[k] = st.skip[k] = 1;
+        }
+    }
+    // a batch goes on with the instances that converged
+    if (st.multi()) {
+        r.success = 0;
+        for (int k = 0; k < m->n_inst; ++k) r.success |= !st.skip[k];
+    }
     st.stats.refinements += m->stats.refinements - ref0;
     st.stats.stage_solves++;
     st.stats.total_iters += r.n_iter;
@@ -1445,8 +1499,10 @@ static bool do_callback(Stepper& st, sem_step_cb cb, void* user)
     return cb(h.data(), h.data() + rl, user) != 0;
 }
 
-// VerboseTracker (verbose.hpp:25-44)
-static void progress(double t, double T, double dt, const double* err)
+// One VerboseTracker line (verbose.hpp:9-44): t fixed and the error estimate
+// scientific, both with precision int(2 - log10 dt) (schrodinger.hpp:28),
+// and a 30-character progress bar (clamped to the bar when t passes T).
+static std::string progress_line(double t, double T, double dt, const double* err)
 {
     int prec = (int)(2 - std::log10(dt));
     if (prec < 0) prec = 0;
@@ -1459,10 +1515,17 @@ static void progress(double t, double T, double dt, const double* err)
     for (int i = 0; i < q; ++i) bar[i + 1] = '-';
     bar[q + 1] = '>';
     bar[w + 1] = ']';
+    char buf[256];
     if (err)
-        std::printf("%.*f :: %s :: error ~ %e\r", prec, t, bar.c_str(), *err);
+        std::snprintf(buf, sizeof buf, "%.*f :: %s :: error ~ %.*e\r", prec, t, bar.c_str(), prec, *err);
     else
-        std::printf("%.*f :: %s\r", prec, t, bar.c_str());
+        std::snprintf(buf, sizeof buf, "%.*f :: %s\r", prec, t, bar.c_str());
+    return buf;
+}
+
+static void progress(double t, double T, double dt, const double* err)
+{
+    std::fputs(progress_line(t, T, dt, err).c_str(), stdout);
     std::fflush(stdout);
 }
 
@@ -1527,6 +1590,24 @@ static Stepper* stepper_create(sem_mesh* m, int scheme, const double* re, const 
     return st.release();
 }
 
+// Batched meshes: the stage solutions of the instances left out of this step
+// (stopped earlier, or a stage solve failed) are zeroed before the update of
+// psi, so psi keeps its value there (schrodinger.hpp:222-226 returns before
+// the update). Instance k's entries are one contiguous chunk range per component.
+static void zero_skipped(Stepper& st, double* const* z, int nz)
+{
+    if (!st.multi()) return;
+    sem_mesh* m = st.m;
+    const size_t cl = (size_t)m->L.K * m->L.N * m->L.N;
+    for (int k = 0; k < m->n_inst; ++k) {
+        if (!st.skip[k]) continue;
+        const size_t off = (size_t)m->h_inst_off[k] * cl;
+        const size_t cnt = (size_t)(m->h_inst_off[k + 1] - m->h_inst_off[k]) * cl;
+        for (int j = 0; j < nz; ++j)
+            for (int c = 0; c < 2; ++c) launch_fill<double>(z[j] + c * m->len() + off, 0.0, cnt, m->stream);
+    }
+}
+
 // one DIRK4S3 step, schrodinger.hpp:376-406
 static bool step_dirk4s3(Stepper& st)
 {
@@ -1544,6 +1625,7 @@ static bool step_dirk4s3(Stepper& st)
     lincomb<double>(m, q, 0, {{1.0, p}, {2 * gamma, z[0]}, {1 - 4 * gamma, z[1]}}, n);
     stage_rhs(st, q, v);
     if (!stage_solve(st, alpha, v, z[2], 2).success) return false;
+    zero_skipped(st, z, 3);
     lincomb<double>(m, p, 2, {{delta, z[0]}, {1 - 2 * delta, z[1]}, {delta, z[2]}}, n);
     record_history(st, 3);
     return true;
@@ -1571,6 +1653,7 @@ static bool step_dirk4s5(Stepper& st)
                     {{1.0, p}, {25.0 / 24.0, z[0]}, {-49.0 / 48.0, z[1]}, {125.0 / 16.0, z[2]}, {-85.0 / 12.0, z[3]}}, n);
     stage_rhs(st, q, v);
     if (!stage_solve(st, alpha, v, z[4], 4).success) return false;
+    zero_skipped(st, z, 5);
     lincomb<double>(m, p, 2,
                     {{25.0 / 24.0, z[0]}, {-49.0 / 48.0, z[1]}, {125.0 / 16.0, z[2]}, {-85.0 / 12.0, z[3]}, {0.25, z[4]}},
                     n);
@@ -1610,11 +1693,44 @@ static bool stepper_advance(Stepper& st, double T, long max_steps, sem_step_cb c
     bool success = true;
     long k = 0;
     const double dt = st.opts.dt;
+    const int ni = st.m->n_inst;
+    if (st.multi() && st.skip.size() != (size_t)ni) {
+        st.skip.assign(ni, 0);
+        st.frozen.assign(ni, 0);
+        st.failed_now.assign(ni, 0);
+        st.inst_t.assign(ni, st.t);
+        st.inst_steps.assign(ni, st.stats.steps);
+    }
     while (st.t < T && (max_steps <= 0 || k < max_steps)) {
         bool ok;
+        if (st.multi()) {
+            bool active = false;
+            for (int q = 0; q < ni; ++q) active |= !st.frozen[q];
+            if (!active) break;
+        }
         if (st.scheme == 0) ok = step_bdf2(st);
         else if (st.scheme == 1) ok = step_dirk4s3(st);
         else ok = step_dirk4s5(st);
+        if (st.multi()) {
+            // per-instance bookkeeping of the reference's loop; the group's clock
+            // runs while any instance is still stepping
+            for (int q = 0; q < ni; ++q) {
+                if (st.frozen[q]) continue;
+                if (st.scheme == 0 || !st.failed_now[q]) {
+                    st.inst_t[q] += dt;
+                    st.inst_steps[q]++;
+                }
+                if (st.failed_now[q]) st.frozen[q] = 1;
+                st.failed_now[q] = 0;
+                st.skip[q] = st.frozen[q];
+            }
+            st.t += dt;
+            st.stats.steps++;
+            ++k;
+            if (st.opts.verbose) progress(st.t, T, dt, st.scheme == 2 ? &st.stats.last_error_estimate : nullptr);
+            if (do_callback(st, cb, user)) break;
+            continue;
+        }
         if (st.scheme == 0) {
             // bdf2 counts the step before testing success (schrodinger.hpp:107-114)
             st.t += dt;
@@ -1633,6 +1749,8 @@ static bool stepper_advance(Stepper& st, double T, long max_steps, sem_step_cb c
     }
     if (st.opts.verbose) std::printf("\n");
     st.m->sync();
+    if (st.multi())
+        for (int q = 0; q < ni; ++q) success = success && !st.frozen[q];
     return success;
 }
 
@@ -1644,6 +1762,141 @@ static void stepper_read(Stepper& st, double* re, double* im)
     m->download(h.data(), st.p, 2);
     if (re) std::copy(h.begin(), h.begin() + rl, re);
     if (im) std::copy(h.begin() + rl, h.end(), im);
+}
+
+// ------------------------------------------------- checkpoint and resume --
+// A session's checkpoint is one host array of doubles: a header, the learned
+// guess's per-stage scales, psi, the BDF2 history (the two projected states
+// of schrodinger.hpp:85-102), the stage solutions of the previous two steps
+// (extrapolated and learned guesses) and the guess network with its Adam
+// moments. Every vector is a two-component array in the reference's
+// element-local layout holding the node's value on every copy, so a
+// save/restore round trip is a permutation of the device arrays, bit for bit.
+constexpr double kCkptMagic = 5.0293e6;
+constexpr int kCkptHeader = 16;
+constexpr int kCkptScales = 5 * 6;
+
+static size_t ckpt_size(const Stepper& st)
+{
+    const size_t v = 2 * (size_t)st.m->L.E * st.m->L.N * st.m->L.N;
+    size_t n = kCkptHeader + kCkptScales + v;
+    if (st.scheme == 0) n += 2 * v;
+    n += 2 * st.zprev.size() * v;
+    if (st.mlp) n += 3 * st.mlp->nparams;
+    return n;
+}
+
+static void stepper_save(Stepper& st, double* out)
+{
+    sem_mesh* m = st.m;
+    const size_t v = 2 * (size_t)m->L.E * m->L.N * m->L.N;
+    double* h = out;
+    std::fill(h, h + kCkptHeader, 0.0);
+    h[0] = kCkptMagic;
+    h[1] = 1;
+    h[2] = st.scheme;
+    h[3] = m->L.E;
+    h[4] = m->L.N;
+    h[5] = st.t;
+    h[6] = st.stats.steps;
+    h[7] = st.nps;
+    h[8] = st.alpha_bdf;
+    h[9] = st.pnorm;
+    h[10] = st.have_prev;
+    h[11] = st.have_prev2;
+    h[12] = (double)st.zprev.size();
+    h[13] = st.mlp ? 1 : 0;
+    h[14] = st.mlp ? (double)st.mlp->nparams : 0;
+    h[15] = st.mlp ? (double)st.mlp->t : 0;
+    double* s = h + kCkptHeader;
+    for (int k = 0; k < 5; ++k) {
+        for (int q = 0; q < 3; ++q) s[6 * k + q] = st.s_feat[k][q];
+        s[6 * k + 3] = st.s_out[k];
+        s[6 * k + 4] = st.have_scale[k];
+        s[6 * k + 5] = st.have_out[k];
+    }
+    double* d = s + kCkptScales;
+    m->download(d, st.p, 2);
+    d += v;
+    if (st.scheme == 0) {
+        for (int q = 0; q < 2; ++q, d += v) {
+            if (q < st.nps) m->download(d, st.ps[q], 2);
+            else std::fill(d, d + v, 0.0);
+        }
+    }
+    for (size_t k = 0; k < st.zprev.size(); ++k) {
+        m->download(d, st.zprev[k], 2);
+        d += v;
+        m->download(d, st.zprev2[k], 2);
+        d += v;
+    }
+    if (st.mlp) {
+        const size_t np = st.mlp->nparams;
+        std::vector<float> f(np);
+        const float* src[3] = {st.mlp->params, st.mlp->adam_m, st.mlp->adam_v};
+        for (int q = 0; q < 3; ++q, d += np) {
+            SEM_HIP(hipMemcpy(f.data(), src[q], np * sizeof(float), hipMemcpyDeviceToHost));
+            std::copy(f.begin(), f.end(), d);
+        }
+    }
+}
+
+// Restores what this session keeps from a checkpoint of the same scheme and
+// mesh size: time, step count, psi and the BDF2 history always; the stage
+// history when this session extrapolates its guesses; the network when this
+// session has one of the same size.
+static void stepper_restore(Stepper& st, const double* in, size_t n)
+{
+    sem_mesh* m = st.m;
+    const size_t v = 2 * (size_t)m->L.E * m->L.N * m->L.N;
+    const double* h = in;
+    if (n < (size_t)kCkptHeader + kCkptScales + v || h[0] != kCkptMagic || h[1] != 1)
+        throw std::invalid_argument("sem_stepper_restore: not a stepper checkpoint");
+    if ((int)h[2] != st.scheme) throw std::invalid_argument("sem_stepper_restore: checkpoint of another scheme");
+    if ((int)h[3] != m->L.E || (int)h[4] != m->L.N)
+        throw std::invalid_argument("sem_stepper_restore: checkpoint of another mesh size");
+    const size_t nz = (size_t)h[12], np = (size_t)h[14];
+    size_t need = kCkptHeader + kCkptScales + v + (st.scheme == 0 ? 2 * v : 0) + 2 * nz * v + (h[13] != 0 ? 3 * np : 0);
+    if (n < need) throw std::invalid_argument("sem_stepper_restore: checkpoint array too short");
+    st.t = h[5];
+    st.stats.steps = (int)h[6];
+    st.nps = (int)h[7];
+    st.alpha_bdf = h[8];
+    st.pnorm = h[9];
+    const double* s = h + kCkptHeader;
+    const double* d = s + kCkptScales;
+    m->upload(st.p, d, 2);
+    d += v;
+    if (st.scheme == 0) {
+        for (int q = 0; q < 2; ++q, d += v)
+            if (q < st.nps) m->upload(st.ps[q], d, 2);
+    }
+    const bool hist = !st.zprev.empty() && st.zprev.size() == nz;
+    for (size_t k = 0; k < nz; ++k) {
+        if (hist) {
+            m->upload(st.zprev[k], d, 2);
+            m->upload(st.zprev2[k], d + v, 2);
+        }
+        d += 2 * v;
+    }
+    st.have_prev = hist && h[10] != 0;
+    st.have_prev2 = hist && h[11] != 0;
+    if (st.mlp && h[13] != 0 && np == st.mlp->nparams) {
+        for (int k = 0; k < 5; ++k) {
+            for (int q = 0; q < 3; ++q) st.s_feat[k][q] = s[6 * k + q];
+            st.s_out[k] = s[6 * k + 3];
+            st.have_scale[k] = s[6 * k + 4] != 0;
+            st.have_out[k] = s[6 * k + 5] != 0;
+        }
+        std::vector<float> f(np);
+        float* dst[3] = {st.mlp->params, st.mlp->adam_m, st.mlp->adam_v};
+        for (int q = 0; q < 3; ++q, d += np) {
+            for (size_t i = 0; i < np; ++i) f[i] = (float)d[i];
+            SEM_HIP(hipMemcpy(dst[q], f.data(), np * sizeof(float), hipMemcpyHostToDevice));
+        }
+        st.mlp->t = (long)h[15];
+    }
+    m->sync();
 }
 
 // -------------------------------------------------------------- batches --
@@ -1936,6 +2189,14 @@ extern "C" {
 
 const char* sem_last_error(void) { return g_error.c_str(); }
 int sem_version(void) { return 100; }
+
+int sem_progress_line(double t, double T, double dt, int with_error, double err, char* out, int out_len)
+{
+    if (!out || out_len < 1 || !(dt > 0) || !(T > 0)) return SEM_EINVAL;
+    const std::string s = progress_line(t, T, dt, with_error ? &err : nullptr);
+    std::snprintf(out, (size_t)out_len, "%s", s.c_str());
+    return (int)s.size() < out_len ? SEM_OK : SEM_EINVAL;
+}
 
 int sem_device_info(char* name, int name_len, int* n_cu, long* hbm_bytes)
 {
@@ -2275,6 +2536,32 @@ int sem_stepper_stats(const sem_stepper* h, sem_run_stats* out)
     return SEM_OK;
 }
 
+int sem_stepper_checkpoint_size(const sem_stepper* h, long* n)
+{
+    if (!h || !n) return SEM_EINVAL;
+    *n = (long)ckpt_size(*h->st);
+    return SEM_OK;
+}
+
+int sem_stepper_save(sem_stepper* h, double* buf, long n)
+{
+    return guarded([&] {
+        if (!h || !buf) throw std::invalid_argument("sem_stepper_save: null argument");
+        if (n < (long)ckpt_size(*h->st)) throw std::invalid_argument("sem_stepper_save: buffer too small");
+        stepper_save(*h->st, buf);
+        return (int)SEM_OK;
+    });
+}
+
+int sem_stepper_restore(sem_stepper* h, const double* buf, long n)
+{
+    return guarded([&] {
+        if (!h || !buf || n < 1) throw std::invalid_argument("sem_stepper_restore: bad argument");
+        stepper_restore(*h->st, buf, (size_t)n);
+        return (int)SEM_OK;
+    });
+}
+
 void sem_stepper_destroy(sem_stepper* h) { delete h; }
 
 int sem_batch_create(int n, sem_mesh* const* meshes, sem_batch** out)
@@ -2425,6 +2712,21 @@ int sem_batch_read(sem_batch* b, double* const* re, double* const* im, int* ok, 
         if (t) *t = b->session.empty() ? 0.0 : b->session[0]->t;
         return (int)SEM_OK;
     });
+}
+
+int sem_batch_times(const sem_batch* b, double* t, int* steps)
+{
+    if (!b || b->session.size() != b->groups.size()) return SEM_EINVAL;
+    for (size_t g = 0; g < b->groups.size(); ++g) {
+        const Stepper& st = *b->session[g];
+        const auto& mem = b->groups[g].members;
+        for (size_t q = 0; q < mem.size(); ++q) {
+            const bool per = st.m->n_inst > 1 && st.inst_t.size() == mem.size();
+            if (t) t[mem[q]] = per ? st.inst_t[q] : st.t;
+            if (steps) steps[mem[q]] = per ? st.inst_steps[q] : st.stats.steps;
+        }
+    }
+    return SEM_OK;
 }
 
 int sem_batch_schrodinger_bdf2(sem_batch* b, double* const* re, double* const* im, const double* const* V, double T,

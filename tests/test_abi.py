@@ -76,3 +76,18 @@ def test_mlp_host_arrays_are_padded_or_rejected(sem):
     for bad in (np.ones((3, 6)), np.ones(5)):
         with pytest.raises(ValueError):
             sem.MlpNet._padded(bad, 5, 8, "X")
+
+
+@pytest.mark.parametrize("t,T,dt,err", [(0.0123, 0.1, 1e-3, None), (0.05, 0.1, 5e-4, 3.25e-7), (0.1, 0.1, 1e-2, 1.0),
+                                        (0.11, 0.1, 1e-3, None)])
+def test_progress_line_is_verbose_tracker(sem, t, T, dt, err):
+    # VerboseTracker (verbose.hpp:9-44): std::fixed t and std::scientific error,
+    # both at setprecision(int(2 - log10 dt)) (schrodinger.hpp:28), then '\r'
+    prec = int(2 - np.log10(dt))
+    q = min(int(30 * (t / T)), 30)
+    # out[p+1] = '>' and then out[n+1] = ']': at the end of the run the bracket wins
+    bar = "[" + "-" * q + ">" + " " * (30 - q - 1) + "]" if q < 30 else "[" + "-" * 30 + "]"
+    want = "%.*f :: %s" % (prec, t, bar)
+    if err is not None:
+        want += " :: error ~ %.*e" % (prec, err)
+    assert sem.progress_line(t, T, dt, err) == want + "\r"

@@ -1,0 +1,199 @@
+"""The GPU path at the benchmark workloads' own sizes, against the oracle (live,
+or through the committed fixtures of tests/golden/make_config_fixtures.py),
+plus the convergence orders the discretisation must show: temporal order
+2 / 4 / 4 of BDF2 / DIRK4S3 / DIRK4S5 against the exact time evolution of the
+assembled operator, and spectral convergence in p of a manufactured Helmholtz
+solution with a varying coefficient."""
+import os
+
+import numpy as np
+import pytest
+import scipy.linalg as sla
+
+from conftest import GOLDEN, SMALL_MESH, rel, structured
+from test_oracle_dss import global_numbering, on_square
+from test_oracle_solvers import assembled
+
+pytestmark = pytest.mark.gpu
+
+
+def fixture(name):
+    return np.load(os.path.join(GOLDEN, name + ".npz"), allow_pickle=False)
+
+
+def check_against_fixture(psi, fx, bar=1e-9):
+    """rel-L2 distance to the oracle's psi, estimated from the 32 Gaussian
+    projections (relative standard deviation 0.25), plus exact agreement at
+    every stride-th entry and of the norm."""
+    from schrodingersem_amd import configs
+    sk = configs.sketch(psi)
+    est = np.linalg.norm(sk - fx["sketch"]) / np.sqrt(sk.size) / float(fx["norm"])
+    samp = psi[::int(fx["stride"])]
+    pointwise = np.max(np.abs(samp - fx["samples"])) / np.max(np.abs(fx["samples"]))
+    nrm = abs(np.linalg.norm(psi) / float(fx["norm"]) - 1.0)
+    assert est < bar and pointwise < bar and nrm < bar, (est, pointwise, nrm)
+    return est, pointwise, nrm
+
+
+# ---------------------------------------------------------------- config 2 --
+def test_config2_helmholtz_full_size(sem, orc):
+    # 64 x 64 jittered quads, p = 8 (263,169 DOF), k^2 = 10, 32 Gaussians, MINRES to 1e-10
+    from schrodingersem_amd import configs
+    m, xy, q, c, F = configs.config2(sem)
+    o = orc.Mesh.from_arrays(8, xy, q)
+    assert m.dof() == o.dof() == 263169
+    u = np.zeros(m.size)
+    r = sem.helmholtz(u, c, F, m, 100000, 1e-10)
+    uo, ok, it, res = o.helmholtz(np.zeros(m.size), c, F, 100000, 1e-10)
+    assert r.success and ok
+    assert abs(r.n_iter - it) <= max(3, it // 50), (r.n_iter, it)
+    err = rel(sem.unmask(u, m), o.unmask(uo))
+    print("config 2: GPU %d / oracle %d MINRES iterations, rel-L2 %.2e" % (r.n_iter, it, err))
+    assert err < 1e-9
+
+
+# ------------------------------------------------------------ configs 3, 5 --
+@pytest.mark.parametrize("precision", [0, 1])
+def test_config3_dirk4s3_step_full_size(sem, precision):
+    # one DIRK4S3 step on 128 x 128 quads of [-10,10]^2 at p = 8 (1,050,625 DOF),
+    # FP64 and mixed precision, against the oracle's step (fixture)
+    from schrodingersem_amd import configs
+    fx = fixture("config3_dirk4s3_step")
+    m, xy, q, V, re, im = configs.config3(sem, int(fx["n"]), int(fx["p"]))
+    dt = float(fx["dt"])
+    opts = sem.SolverOpts(max_iter=100000, tol=float(fx["tol"]), dt=dt, precision=precision)
+    assert sem.schrodinger_dirk4s3(re, im, V, m, dt, opts)
+    st = m.last_run_stats()
+    assert st["steps"] == 1 and st["stage_solves"] == 3
+    print("config 3 (precision %d): %s" % (precision, check_against_fixture(np.concatenate([re, im]), fx)))
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_config5_dirk4s5_step_full_size(sem, precision):
+    # one DIRK4S5 step on 256 x 256 quads of [-12,12]^2 at p = 7 (3,214,849 DOF),
+    # FP64 and mixed precision, against the oracle's step (fixture)
+    from schrodingersem_amd import configs
+    fx = fixture("config5_dirk4s5_step")
+    m, xy, q, V, re, im = configs.config5(sem, int(fx["n"]), int(fx["p"]))
+    dt = float(fx["dt"])
+    opts = sem.SolverOpts(max_iter=100000, tol=float(fx["tol"]), dt=dt, precision=precision)
+    assert sem.schrodinger_dirk4s5(re, im, V, m, dt, opts)
+    st = m.last_run_stats()
+    assert st["steps"] == 1 and st["stage_solves"] == 5
+    print("config 5 (precision %d): %s" % (precision, check_against_fixture(np.concatenate([re, im]), fx)))
+
+
+# ---------------------------------------------------------------- config 4 --
+def test_config4_all_instances(sem, orc):
+    # all 64 uneven instances (n in [16, 96], p in {4, ..., 12}) batched for 3 BDF2
+    # steps: a subset equals its single runs bit for bit, the three smallest
+    # instances match the oracle
+    from schrodingersem_amd import configs
+    inst = configs.config4(sem)
+    assert len(inst) == 64
+    dt, tol = 2e-3, 1e-10
+    T = 2.5 * dt
+    opts = sem.SolverOpts(max_iter=100000, tol=tol, dt=dt)
+    batch = sem.Batch([x["mesh"] for x in inst])
+    re = [x["re"].copy() for x in inst]
+    im = [x["im"].copy() for x in inst]
+    ok = batch.schrodinger_bdf2(re, im, [x["V"] for x in inst], T, opts)
+    assert all(ok)
+    assert batch.last_run_stats()["steps"] == 3
+    # single runs: one instance of each degree
+    seen = {}
+    for k, x in enumerate(inst):
+        seen.setdefault(x["p"], k)
+    for p, k in sorted(seen.items()):
+        x = inst[k]
+        a, b = x["re"].copy(), x["im"].copy()
+        assert sem.schrodinger_bdf2(a, b, x["V"], x["mesh"], T, opts)
+        assert np.array_equal(a, re[k]) and np.array_equal(b, im[k]), (p, k)
+    # the oracle on the three smallest instances
+    small = sorted(range(64), key=lambda k: inst[k]["mesh"].size)[:3]
+    for k in small:
+        x = inst[k]
+        o = orc.Mesh.from_arrays(x["p"], x["xy"], x["quads"])
+        ro, io, oko, _, _ = o.schrodinger(0, x["re"], x["im"], x["V"], T, 100000, tol, dt)
+        assert oko
+        err = rel(np.concatenate([re[k], im[k]]), np.concatenate([ro, io]))
+        print("config 4 instance %d (n=%d, p=%d): rel-L2 vs oracle %.2e" % (k, x["n"], x["p"], err))
+        assert err < 1e-9
+    batch.close()
+
+
+# ------------------------------------------------------- convergence orders --
+@pytest.fixture(scope="module")
+def exact_evolution(orc):
+    """small_mesh at p = 3 with V = 1 + 0.5 |x|^2: psi0 = four low eigenmodes of
+    (K + M_V, M) with complex weights, evolved exactly (psi' = i M^-1 L psi)."""
+    o = orc.Mesh.load(3, SMALL_MESH)
+    groups, bnd = global_numbering(o, on_square(-1, 1))
+    K, inner = assembled(o, groups, bnd, 0)
+    M, _ = assembled(o, groups, bnd, 2)
+    X, Y = o.coords()
+    V = 1.0 + 0.5 * (X * X + Y * Y)
+    own = [g[0] for g in inner]
+    L = K + np.diag(np.diag(M) * V[own])
+    mu, phi = sla.eigh(L, M)
+    w = np.array([1.0, 0.6 - 0.3j, 0.4j, -0.25])
+    c0 = phi[:, :4] @ w
+    T = 0.5
+    cT = phi[:, :4] @ (w * np.exp(1j * mu[:4] * T))
+
+    def scatter(c):
+        a = np.zeros(o.size, complex)
+        for i, g in enumerate(inner):
+            a[g] = c[i]
+        return a
+
+    return V, scatter(c0), cT, own, T
+
+
+@pytest.mark.parametrize("scheme,order,steps", [(0, 2, (64, 128, 256)), (1, 4, (32, 64, 128)), (2, 4, (16, 32, 64))])
+def test_temporal_order(sem, exact_evolution, scheme, order, steps):
+    V, psi0, cT, own, T = exact_evolution
+    m = sem.load_mesh(3, SMALL_MESH)
+    errs = []
+    for n in steps:
+        dt = T / n
+        re, im = psi0.real.copy(), psi0.imag.copy()
+        opts = sem.SolverOpts(max_iter=100000, tol=1e-14, dt=dt)
+        fn = (sem.schrodinger_bdf2, sem.schrodinger_dirk4s3, sem.schrodinger_dirk4s5)[scheme]
+        assert fn(re, im, V, m, T - 0.5 * dt, opts)
+        assert m.last_run_stats()["steps"] == n
+        errs.append(np.linalg.norm((re + 1j * im)[own] - cT) / np.linalg.norm(cT))
+    orders = [np.log2(a / b) for a, b in zip(errs, errs[1:])]
+    print("scheme %d: errors %s, observed orders %s" % (scheme, errs, orders))
+    assert all(order - 0.25 < q < order + 0.5 for q in orders), (errs, orders)
+
+
+@pytest.mark.parametrize("shear", [0.0, 0.4])
+def test_spatial_convergence(sem, shear):
+    # -Lap u + c u = f, c = 10 + 5x - 3y^2, u = sin(pi x) sin(pi y) on a 4 x 4 mesh of
+    # the unit square (sheared parallelograms for shear != 0, affine elements):
+    # the GPU solution's error falls spectrally with p
+    errs = []
+    for p in (2, 4, 6, 8):
+        xy, q = structured(4, (0, 1, 0, 1), 0.0, 0)
+        xy = np.asarray(xy, dtype=np.float64).reshape(-1, 2).copy()
+        X0 = xy[:, 0] + shear * xy[:, 1]
+        m = sem.Mesh.from_arrays(p, np.stack([X0, xy[:, 1]], 1).ravel(), q)
+        X, Y = m.coords()
+        # the exact solution of the sheared domain: sin(pi (x - shear y)) sin(pi y)
+        xs = X - shear * Y
+        ex = np.sin(np.pi * xs) * np.sin(np.pi * Y)
+        # u(x, y) = sin(pi (x - s y)) sin(pi y): u_xx = -pi^2 u,
+        # u_yy = -pi^2 (1 + s^2) u - 2 s pi^2 cos(pi (x - s y)) cos(pi y)
+        uxx = -np.pi**2 * ex
+        uyy = -np.pi**2 * (1 + shear**2) * ex - 2 * shear * np.pi**2 * np.cos(np.pi * xs) * np.cos(np.pi * Y)
+        c = 10.0 + 5.0 * X - 3.0 * Y * Y
+        f = -(uxx + uyy) + c * ex
+        u = np.zeros(m.size)
+        r = sem.helmholtz(u, c, f, m, 100000, 1e-14)
+        assert r.success
+        e = sem.unmask(u, m) - ex
+        errs.append(np.sqrt(sem.dot(m, e, e) / sem.dot(m, ex, ex)))
+    print("shear %.1f: errors %s" % (shear, errs))
+    assert all(b < a / 10 for a, b in zip(errs, errs[1:])), errs
+    assert errs[-1] < 1e-6
