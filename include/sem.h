@@ -237,6 +237,80 @@ int sem_minres_tridiag(long n, double lo, double di, double up, const double* b,
 int sem_pcg_tridiag(long n, double lo, double di, double up, const double* b, double* x, int maxit, double tol,
                     sem_solver_results* out);
 
+/* ----- generic Krylov solvers over the caller's operator (minres.hpp:19-107,
+ * pcg.hpp:12-97): A, dot and precond play the reference's LinOp, Dot and
+ * Precond template arguments. host = 1: b, x and every callback argument are
+ * host arrays of n doubles (the library stages its device vectors through
+ * them). host = 0: b, x and the callback arguments are device pointers, and a
+ * callback enqueues its work on `stream` (the library's HIP stream, as a
+ * void*) or synchronises before returning. A callback returns 0, or nonzero
+ * to abort the solve (SEM_EINVAL). dot = NULL: Euclidean dot on the device;
+ * precond = NULL: IdentityPreconditioner (solver_base.hpp:14-22).
+ * consistent = 0 computes MINRES's beta as minres.hpp:63 does,
+ * sqrt(dot(precond(r2), precond(r2))); consistent = 1 uses
+ * sqrt(dot(r2, precond(r2))), the normalisation that matches minres.hpp:31
+ * for a symmetric positive definite preconditioner. Both are the same with the
+ * identity. ----- */
+typedef int (*sem_linop_fn)(void* user, const double* x, double* y, void* stream);
+typedef int (*sem_dot_fn)(void* user, const double* a, const double* b, double* result, void* stream);
+typedef int (*sem_precond_fn)(void* user, const double* r, double* z, void* stream);
+typedef struct sem_krylov {
+    long n;
+    int host;
+    int consistent;
+    sem_linop_fn A;
+    void* A_user;
+    sem_dot_fn dot;
+    void* dot_user;
+    sem_precond_fn precond;
+    void* precond_user;
+} sem_krylov;
+int sem_minres(const sem_krylov* k, const double* b, double* x, int maxit, double tol, sem_solver_results* out);
+int sem_pcg(const sem_krylov* k, const double* b, double* x, int maxit, double tol, sem_solver_results* out);
+
+/* Preconditioners of the mesh solvers (no counterpart in the reference beyond
+ * IdentityPreconditioner, solver_base.hpp:14-22). */
+enum sem_precond_kind {
+    SEM_PRECOND_IDENTITY = 0, /* the reference's arithmetic (generic recurrences) */
+    SEM_PRECOND_JACOBI = 1    /* 1/|diag| of the assembled operator, zero on Dirichlet entries */
+};
+/* poisson / helmholtz (openmp_impl/poisson.hpp:17-59, helmholtz.hpp:18-64)
+ * with a preconditioner: PCG on K u = M f, MINRES (consistent normalisation)
+ * on (K + M_c) u = M f; same arguments and outputs as sem_poisson /
+ * sem_helmholtz. */
+int sem_poisson_precond(sem_mesh* m, double* u, const double* f, int max_iter, double tol, int precond,
+                        sem_solver_results* out);
+int sem_helmholtz_precond(sem_mesh* m, double* u, const double* c, const double* f, int max_iter, double tol,
+                          int precond, sem_solver_results* out);
+
+/* galerkin_op(op, u, mesh) (openmp_impl/galerkin.hpp:24-35) with the caller's
+ * element operator, the reference's Op action: unmask -> op -> global_sum ->
+ * mask. in and out are device arrays of n_elements*N*N doubles in the
+ * element-local layout; op receives device arrays u (unmasked input) and y
+ * (its output, element by element) in the same layout and enqueues its work
+ * on `stream`, returning 0 (nonzero aborts with SEM_EINVAL). */
+typedef int (*sem_element_op_fn)(void* user, int n_elements, int N, const double* u, double* y, void* stream);
+int sem_galerkin_apply_op(sem_mesh* m, sem_element_op_fn op, void* user, const double* in, double* out);
+
+/* Device memory for the device-pointer entry points (no torch or HIP types
+ * needed by the caller): n doubles; sem_copy direction 0 host->device,
+ * 1 device->host, 2 device->device. */
+int sem_device_alloc(long n, double** ptr);
+int sem_device_free(double* ptr);
+int sem_copy(double* dst, const double* src, long n, int direction);
+/* waits for the work enqueued on a stream handed to a callback */
+int sem_stream_synchronize(void* stream);
+
+/* Host-only basis routines (no GPU): gauss_lobatto<double>(n) with n + 1
+ * nodes and weights (gauss_lobatto.hpp:43-126); derivative_matrix(x) for n
+ * points, D column-major (derivative_matrix.hpp:12-38); Quad::compute_metrics
+ * (Quad.hpp:27-55) of the quad with corners (xs, ys) on the n points z, J, A,
+ * B, C n x n column-major. */
+int sem_gauss_lobatto(int n, double* x, double* w);
+int sem_derivative_matrix(int n, const double* x, double* D);
+int sem_quad_metrics(const double* xs, const double* ys, int n, const double* z, double* J, double* A, double* B,
+                     double* C);
+
 /* ----- measurement hooks (device-resident data, HIP events on the handle's stream) ----- */
 /* Times `reps` launches of the element-operator kernel over device-resident
  * random data (kind, precision 0 = FP64, 1 = FP32); returns mean microseconds
@@ -270,8 +344,13 @@ int sem_get_step_shape(const sem_mesh* m, int precision, int* shape);
 /* MINRES step mode per precision (SEM_PREC_FP64, or SEM_PREC_MIXED for the
  * FP32 inner solver): 0 two launches per Lanczos step; 1 one fused launch per
  * step that applies the operator before the 1/beta scaling (same recurrence,
- * results equal to rounding; graph mode, when on, takes precedence).
- * Defaults: FP64 0, FP32 1. */
+ * results equal to rounding; graph mode, when on, takes precedence); 2 the
+ * fused step with the recurrence's reduction of the per-chunk sums moved out
+ * of every workgroup into one small kernel per step (same order: results
+ * equal mode 1's bit for bit; cheaper on large meshes). Modes 1 and 2 are
+ * FP32 only: FP64 returns SEM_EUNSUP for them.
+ * Defaults: FP64 0; FP32 2 when an instance of the mesh has 1024 chunks of
+ * elements or more, 1 otherwise. */
 int sem_set_step_mode(sem_mesh* m, int precision, int mode);
 int sem_get_step_mode(const sem_mesh* m, int precision, int* mode);
 /* Diagonal scaling of the FP32 inner MINRES in mixed-precision Schrodinger
@@ -279,7 +358,7 @@ int sem_get_step_mode(const sem_mesh* m, int precision, int* mode);
  * S = (m + |alpha| |k + c|)^(-1/2) from the assembled mass, stiffness and
  * potential-mass diagonals, which takes fewer Lanczos steps; the outer FP64
  * refinement stops on the true residual either way. It applies when the FP32
- * step mode is 1 and graph mode is off. No counterpart in the reference
+ * step mode is 1 or 2 and graph mode is off. No counterpart in the reference
  * (its MINRES is unpreconditioned, minres.hpp:19). */
 int sem_set_inner_scaling(sem_mesh* m, int on);
 int sem_get_inner_scaling(const sem_mesh* m, int* on);

@@ -59,6 +59,10 @@ def lib():
                                              C.c_double, _ip, _dp]),
             "orc_pcg_tridiag": (C.c_int, [C.c_int, C.c_double, C.c_double, C.c_double, _dp, _dp, C.c_int,
                                           C.c_double, _ip, _dp]),
+            "orc_minres_tridiag_pc": (C.c_int, [C.c_int, C.c_double, C.c_double, C.c_double, _dp, _dp, C.c_int,
+                                                C.c_double, _dp, C.c_int, _ip, _dp]),
+            "orc_pcg_tridiag_pc": (C.c_int, [C.c_int, C.c_double, C.c_double, C.c_double, _dp, _dp, C.c_int,
+                                             C.c_double, _dp, _ip, _dp]),
             "orc_poisson": (C.c_int, [C.c_void_p, _dp, _dp, C.c_int, C.c_double, _ip, _dp]),
             "orc_helmholtz": (C.c_int, [C.c_void_p, _dp, _dp, _dp, C.c_int, C.c_double, _ip, _dp]),
             "orc_schrodinger": (C.c_int, [C.c_void_p, C.c_int, _dp, _dp, _dp, C.c_double, C.c_int, C.c_double,
@@ -276,4 +280,28 @@ def pcg_tridiag(lo, di, up, b, x, maxit, tol):
     it = C.c_int()
     res = C.c_double()
     ok = lib().orc_pcg_tridiag(b.size, lo, di, up, _p(b), _p(x), int(maxit), float(tol), C.byref(it), C.byref(res))
+    return x, bool(ok), it.value, res.value
+
+
+def minres_tridiag_pc(lo, di, up, b, x, maxit, tol, pdiag=None, consistent=False):
+    """minres.hpp with a diagonal preconditioner z = pdiag * r (None: identity)."""
+    b = _f(b)
+    x = _f(x, b.size).copy()
+    pd = None if pdiag is None else _f(pdiag, b.size)
+    it = C.c_int()
+    res = C.c_double()
+    ok = lib().orc_minres_tridiag_pc(b.size, lo, di, up, _p(b), _p(x), int(maxit), float(tol),
+                                     None if pd is None else _p(pd), int(bool(consistent)), C.byref(it), C.byref(res))
+    return x, bool(ok), it.value, res.value
+
+
+def pcg_tridiag_pc(lo, di, up, b, x, maxit, tol, pdiag=None):
+    """pcg.hpp with a diagonal preconditioner z = pdiag * r (None: identity)."""
+    b = _f(b)
+    x = _f(x, b.size).copy()
+    pd = None if pdiag is None else _f(pdiag, b.size)
+    it = C.c_int()
+    res = C.c_double()
+    ok = lib().orc_pcg_tridiag_pc(b.size, lo, di, up, _p(b), _p(x), int(maxit), float(tol),
+                                  None if pd is None else _p(pd), C.byref(it), C.byref(res))
     return x, bool(ok), it.value, res.value

@@ -28,6 +28,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <functional>
 #include <cstdint>
 #include <cstring>
 #include <fstream>
@@ -691,6 +692,110 @@ static Results pcg(vec& x, LinOp A, const vec& b, Dot dotp, int max_iter, double
     return {success ? 1 : 0, i, std::sqrt(rho)};
 }
 
+// minres.hpp:19-107 and pcg.hpp:12-97 with a preconditioner z = P(r) (the
+// Precond argument; P empty means IdentityPreconditioner). MINRES follows
+// minres.hpp:24 (y = precond(r1)), :31 (beta1 = sqrt(dot(y, r1))) and :60-63
+// (y = precond(r2); beta = sqrt(dot(y, y))); consistent = true takes
+// beta = sqrt(dot(r2, y)) instead.
+typedef std::function<vec(const vec&)> Precond;
+
+template <class LinOp, class Dot>
+static Results minres_pc(vec& x, LinOp A, const vec& b, Dot dotp, const Precond& P, bool consistent, int maxit,
+                         double tol)
+{
+    const size_t n = x.size();
+    bool success = false;
+    vec Ax = A(x);
+    vec r1(n);
+    for (size_t k = 0; k < n; ++k) r1[k] = b[k] - Ax[k];
+    vec y = P ? P(r1) : r1;
+    vec r2 = r1;
+    double b_norm = std::sqrt(dotp(r1, r1));
+    const double machine_epsilon = std::numeric_limits<double>::epsilon();
+    double beta1 = std::sqrt(dotp(y, r1));
+    double beta = beta1, beta_old = 0, tnorm = 0, dbar = 0, eps = 0, eps_old = 0, phibar = beta1, cs = -1, sn = 0;
+    vec w(n);
+    for (size_t k = 0; k < n; ++k) w[k] = 0.0 * x[k];
+    vec w2 = w, w1;
+    vec v(n);
+    int i;
+    for (i = 1; i <= maxit; ++i) {
+        double ib = 1.0 / beta;
+        for (size_t k = 0; k < n; ++k) v[k] = ib * y[k];
+        y = A(v);
+        if (i >= 2) {
+            double c = beta / beta_old;
+            for (size_t k = 0; k < n; ++k) y[k] -= c * r1[k];
+        }
+        double alpha = dotp(v, y);
+        double c2 = alpha / beta;
+        for (size_t k = 0; k < n; ++k) y[k] -= c2 * r2[k];
+        r1.swap(r2);
+        r2 = y;
+        if (P) y = P(r2);
+        beta_old = beta;
+        beta = std::sqrt(consistent ? dotp(r2, y) : dotp(y, y));
+        tnorm += alpha * alpha + beta * beta + beta_old * beta_old;
+        eps_old = eps;
+        double delta = cs * dbar + sn * alpha;
+        double gbar = sn * dbar - cs * alpha;
+        eps = sn * beta;
+        dbar = -cs * beta;
+        double gamma = std::max(std::sqrt(gbar * gbar + beta * beta), machine_epsilon);
+        cs = gbar / gamma;
+        sn = beta / gamma;
+        double phi = cs * phibar;
+        phibar *= sn;
+        w1.swap(w2);
+        w2.swap(w);
+        double ig = 1.0 / gamma;
+        if (w.size() != n) w.resize(n);
+        for (size_t k = 0; k < n; ++k) w[k] = ig * ((v[k] - eps_old * w1[k]) - delta * w2[k]);
+        for (size_t k = 0; k < n; ++k) x[k] += phi * w[k];
+        if (phibar < tol * b_norm + tol) {
+            success = true;
+            break;
+        }
+    }
+    return {success ? 1 : 0, i, phibar};
+}
+
+// pcg.hpp:59-97 with the preconditioned conj_grad overloads (:12-17, :27-39)
+template <class LinOp, class Dot>
+static Results pcg_pc(vec& x, LinOp A, const vec& b, Dot dotp, const Precond& P, int max_iter, double tol)
+{
+    const size_t n = x.size();
+    vec Ax = A(x);
+    vec r(n);
+    for (size_t k = 0; k < n; ++k) r[k] = b[k] - Ax[k];
+    double bnorm = std::sqrt(dotp(b, b));
+    bool success = false;
+    vec p;
+    double rho_prev = 0, rho = 0;
+    int i;
+    for (i = 0; i < max_iter; ++i) {
+        if (i == 0) {
+            p = P ? P(r) : r;
+            rho = dotp(r, p);
+        } else {
+            vec z = P ? P(r) : r;
+            rho = dotp(r, z);
+            double beta = rho / rho_prev;
+            for (size_t k = 0; k < n; ++k) p[k] = z[k] + beta * p[k];
+        }
+        vec Ap = A(p);
+        double alpha = rho / dotp(p, Ap);
+        for (size_t k = 0; k < n; ++k) x[k] += alpha * p[k];
+        for (size_t k = 0; k < n; ++k) r[k] -= alpha * Ap[k];
+        rho_prev = rho;
+        if (std::sqrt(rho) < tol * bnorm) {
+            success = true;
+            break;
+        }
+    }
+    return {success ? 1 : 0, i, std::sqrt(rho)};
+}
+
 // ------------------------------------------------------ a22/a23 drivers ---
 // openmp_impl/poisson.hpp:17-59
 static Results poisson(const Mesh& m, double* u, const double* f, int max_iter, double tol)
@@ -1158,6 +1263,50 @@ int orc_pcg_tridiag(int n, double lo, double di, double up, const double* b, dou
 {
     vec xx(x, x + n), bb(b, b + n);
     Results r = pcg(xx, [&](const vec& v) { return tridiag_apply(v, lo, di, up); }, bb, edot, maxit, tol);
+    std::copy(xx.begin(), xx.end(), x);
+    *n_iter = r.n_iter;
+    *resid = r.residual;
+    return r.success;
+}
+
+// the same solvers with a diagonal preconditioner z = pdiag * r (pdiag NULL: none)
+int orc_minres_tridiag_pc(int n, double lo, double di, double up, const double* b, double* x, int maxit, double tol,
+                          const double* pdiag, int consistent, int* n_iter, double* resid)
+{
+    vec xx(x, x + n), bb(b, b + n);
+    Precond P;
+    vec pd;
+    if (pdiag) {
+        pd.assign(pdiag, pdiag + n);
+        P = [&pd](const vec& r) {
+            vec z(r.size());
+            for (size_t k = 0; k < r.size(); ++k) z[k] = pd[k] * r[k];
+            return z;
+        };
+    }
+    Results r = minres_pc(xx, [&](const vec& v) { return tridiag_apply(v, lo, di, up); }, bb, edot, P, consistent != 0,
+                          maxit, tol);
+    std::copy(xx.begin(), xx.end(), x);
+    *n_iter = r.n_iter;
+    *resid = r.residual;
+    return r.success;
+}
+
+int orc_pcg_tridiag_pc(int n, double lo, double di, double up, const double* b, double* x, int maxit, double tol,
+                       const double* pdiag, int* n_iter, double* resid)
+{
+    vec xx(x, x + n), bb(b, b + n);
+    Precond P;
+    vec pd;
+    if (pdiag) {
+        pd.assign(pdiag, pdiag + n);
+        P = [&pd](const vec& r) {
+            vec z(r.size());
+            for (size_t k = 0; k < r.size(); ++k) z[k] = pd[k] * r[k];
+            return z;
+        };
+    }
+    Results r = pcg_pc(xx, [&](const vec& v) { return tridiag_apply(v, lo, di, up); }, bb, edot, P, maxit, tol);
     std::copy(xx.begin(), xx.end(), x);
     *n_iter = r.n_iter;
     *resid = r.residual;

@@ -127,6 +127,20 @@ def lib():
             "sem_stepper_destroy": (None, [C.c_void_p]),
             "sem_stepper_set_learning": (C.c_int, [C.c_void_p, C.c_int, C.c_float]),
             "sem_stepper_set_guard": (C.c_int, [C.c_void_p, C.c_int]),
+            "sem_minres": (C.c_int, [C.POINTER(_Krylov), _dp, _dp, C.c_int, C.c_double, C.POINTER(_Results)]),
+            "sem_pcg": (C.c_int, [C.POINTER(_Krylov), _dp, _dp, C.c_int, C.c_double, C.POINTER(_Results)]),
+            "sem_poisson_precond": (C.c_int, [C.c_void_p, _dp, _dp, C.c_int, C.c_double, C.c_int,
+                                              C.POINTER(_Results)]),
+            "sem_helmholtz_precond": (C.c_int, [C.c_void_p, _dp, _dp, _dp, C.c_int, C.c_double, C.c_int,
+                                                C.POINTER(_Results)]),
+            "sem_galerkin_apply_op": (C.c_int, [C.c_void_p, ELEMENT_OP_CB, C.c_void_p, _dp, _dp]),
+            "sem_device_alloc": (C.c_int, [C.c_long, C.POINTER(_dp)]),
+            "sem_device_free": (C.c_int, [_dp]),
+            "sem_copy": (C.c_int, [_dp, _dp, C.c_long, C.c_int]),
+            "sem_stream_synchronize": (C.c_int, [C.c_void_p]),
+            "sem_gauss_lobatto": (C.c_int, [C.c_int, _dp, _dp]),
+            "sem_derivative_matrix": (C.c_int, [C.c_int, _dp, _dp]),
+            "sem_quad_metrics": (C.c_int, [_dp, _dp, C.c_int, _dp, _dp, _dp, _dp, _dp]),
             "sem_stepper_checkpoint_size": (C.c_int, [C.c_void_p, _lp]),
             "sem_stepper_save": (C.c_int, [C.c_void_p, _dp, C.c_long]),
             "sem_stepper_restore": (C.c_int, [C.c_void_p, _dp, C.c_long]),
@@ -173,6 +187,18 @@ def _f64(a, n=None):
 
 def _ptr(a):
     return a.ctypes.data_as(_dp)
+
+
+LINOP_CB = C.CFUNCTYPE(C.c_int, C.c_void_p, _dp, _dp, C.c_void_p)
+DOT_CB = C.CFUNCTYPE(C.c_int, C.c_void_p, _dp, _dp, _dp, C.c_void_p)
+PRECOND_CB = C.CFUNCTYPE(C.c_int, C.c_void_p, _dp, _dp, C.c_void_p)
+ELEMENT_OP_CB = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_int, C.c_int, _dp, _dp, C.c_void_p)
+PRECOND_IDENTITY, PRECOND_JACOBI = 0, 1
+
+
+class _Krylov(C.Structure):
+    _fields_ = [("n", C.c_long), ("host", C.c_int), ("consistent", C.c_int), ("A", LINOP_CB), ("A_user", C.c_void_p),
+                ("dot", DOT_CB), ("dot_user", C.c_void_p), ("precond", PRECOND_CB), ("precond_user", C.c_void_p)]
 
 
 class SolverResults:
@@ -617,9 +643,15 @@ def set_step_shape(mesh, shape, precision=PREC_FP64):
 
 
 def set_step_mode(mesh, mode, precision=None):
-    """MINRES Lanczos step: 0 two launches per step, 1 one fused launch per step
-    (precision PREC_FP64 or PREC_MIXED; None sets both)."""
-    for p in ((PREC_FP64, PREC_MIXED) if precision is None else (precision,)):
+    """MINRES Lanczos step: 0 two launches per step, 1 one fused launch per step,
+    2 the fused step with its recurrence reduction in a separate small kernel
+    (precision PREC_FP64 or PREC_MIXED; None sets PREC_MIXED, and PREC_FP64 too
+    for mode 0: the fused step runs in FP32 only)."""
+    if precision is None:
+        precs = (PREC_FP64, PREC_MIXED) if mode == 0 else (PREC_MIXED,)
+    else:
+        precs = (precision,)
+    for p in precs:
         _check(lib().sem_set_step_mode(mesh._h, int(p), int(mode)))
 
 
@@ -767,3 +799,184 @@ def device_info():
     hbm = C.c_long()
     _check(lib().sem_device_info(name, 256, C.byref(ncu), C.byref(hbm)))
     return name.value.decode(), ncu.value, hbm.value
+
+
+# ------------------------------------------------------------ generic solvers --
+def _krylov(n, A, dot, precond, host, consistent):
+    """sem_krylov with Python callables; host=True: they take and fill numpy arrays,
+    host=False: they receive DevicePtr views (device pointers, library stream)."""
+    keep = []
+
+    def wrap(fn, kind):
+        if fn is None:
+            return kind()
+        if kind is LINOP_CB or kind is PRECOND_CB:
+            def cb(_u, x, y, stream):
+                try:
+                    if host:
+                        fn(np.ctypeslib.as_array(x, (n,)), np.ctypeslib.as_array(y, (n,)))
+                    else:
+                        fn(x, y, stream)
+                    return 0
+                except Exception:  # noqa: BLE001 - reported to the library as a failed callback
+                    return 1
+        else:
+            def cb(_u, a, b, out, stream):
+                try:
+                    out[0] = fn(np.ctypeslib.as_array(a, (n,)), np.ctypeslib.as_array(b, (n,))) if host \
+                        else fn(a, b, stream)
+                    return 0
+                except Exception:  # noqa: BLE001
+                    return 1
+        f = kind(cb)
+        keep.append(f)
+        return f
+
+    k = _Krylov(n, int(bool(host)), int(bool(consistent)), wrap(A, LINOP_CB), None, wrap(dot, DOT_CB), None,
+                wrap(precond, PRECOND_CB), None)
+    return k, keep
+
+
+def minres(x, A, b, maxit, tol, dot=None, precond=None, consistent=False):
+    """minres.hpp:19-107 on the GPU over host arrays: A(x, out) fills out = A x,
+    dot(a, b) returns a float, precond(r, out) fills out = P r (None: Euclidean
+    dot, IdentityPreconditioner). x is updated in place."""
+    if not (isinstance(x, np.ndarray) and x.dtype == np.float64 and x.flags.c_contiguous):
+        raise ValueError("x must be a contiguous float64 array")
+    k, keep = _krylov(x.size, A, dot, precond, True, consistent)
+    r = _Results()
+    _check(lib().sem_minres(C.byref(k), _ptr(_f64(b, x.size)), _ptr(x), int(maxit), float(tol), C.byref(r)))
+    del keep
+    return SolverResults(r)
+
+
+def pcg(x, A, b, maxit, tol, dot=None, precond=None):
+    """pcg.hpp:59-97 on the GPU over host arrays (see minres)."""
+    if not (isinstance(x, np.ndarray) and x.dtype == np.float64 and x.flags.c_contiguous):
+        raise ValueError("x must be a contiguous float64 array")
+    k, keep = _krylov(x.size, A, dot, precond, True, False)
+    r = _Results()
+    _check(lib().sem_pcg(C.byref(k), _ptr(_f64(b, x.size)), _ptr(x), int(maxit), float(tol), C.byref(r)))
+    del keep
+    return SolverResults(r)
+
+
+def minres_device(n, A, b_dev, x_dev, maxit, tol, dot=None, precond=None, consistent=False, solver="minres"):
+    """The same on device vectors (pointers from DeviceArray): A(x_ptr, y_ptr, stream)."""
+    k, keep = _krylov(n, A, dot, precond, False, consistent)
+    r = _Results()
+    fn = lib().sem_minres if solver == "minres" else lib().sem_pcg
+    _check(fn(C.byref(k), b_dev, x_dev, int(maxit), float(tol), C.byref(r)))
+    del keep
+    return SolverResults(r)
+
+
+class DeviceArray:
+    """n doubles of device memory (sem_device_alloc), with host copies in and out."""
+
+    def __init__(self, n):
+        self.n = int(n)
+        p = _dp()
+        _check(lib().sem_device_alloc(self.n, C.byref(p)))
+        self.ptr = p
+
+    @classmethod
+    def from_host(cls, a):
+        a = _f64(a)
+        d = cls(a.size)
+        _check(lib().sem_copy(d.ptr, _ptr(a), a.size, 0))
+        return d
+
+    def to_host(self):
+        a = np.zeros(self.n)
+        _check(lib().sem_copy(_ptr(a), self.ptr, self.n, 1))
+        return a
+
+    def close(self):
+        if getattr(self, "ptr", None) is not None and _lib is not None:
+            _lib.sem_device_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        self.close()
+
+
+def copy_to_host(ptr, n):
+    a = np.zeros(n)
+    _check(lib().sem_copy(_ptr(a), ptr, n, 1))
+    return a
+
+
+def copy_to_device(ptr, a):
+    a = _f64(a)
+    _check(lib().sem_copy(ptr, _ptr(a), a.size, 0))
+
+
+def stream_synchronize(stream):
+    _check(lib().sem_stream_synchronize(stream))
+
+
+def poisson_precond(u, f, mesh, max_iter, tol, precond=PRECOND_JACOBI):
+    """poisson (openmp_impl/poisson.hpp:17-59) with a preconditioner; u in/out."""
+    r = _Results()
+    _check(lib().sem_poisson_precond(mesh._h, _ptr(u), _ptr(_f64(f, mesh.size)), int(max_iter), float(tol),
+                                     int(precond), C.byref(r)))
+    return SolverResults(r)
+
+
+def helmholtz_precond(u, c, f, mesh, max_iter, tol, precond=PRECOND_JACOBI):
+    """helmholtz (openmp_impl/helmholtz.hpp:18-64) with a preconditioner; u in/out."""
+    r = _Results()
+    _check(lib().sem_helmholtz_precond(mesh._h, _ptr(u), _ptr(_f64(c, mesh.size)), _ptr(_f64(f, mesh.size)),
+                                       int(max_iter), float(tol), int(precond), C.byref(r)))
+    return SolverResults(r)
+
+
+def galerkin_apply_op(mesh, op, u):
+    """galerkin_op (galerkin.hpp:24-35) with a host element operator op(u, y)
+    on (n_elements, N*N) arrays in the element-local layout; the library runs
+    unmask, global_sum and mask on the GPU and calls op between them."""
+    n = mesh.size
+
+    def cb(_u, ne, N, ud, yd, stream):
+        try:
+            stream_synchronize(stream)
+            uu = copy_to_host(ud, ne * N * N).reshape(ne, N * N)
+            yy = np.zeros_like(uu)
+            op(uu, yy)
+            copy_to_device(yd, yy.ravel())
+            return 0
+        except Exception:  # noqa: BLE001
+            return 1
+
+    f = ELEMENT_OP_CB(cb)
+    din = DeviceArray.from_host(_f64(u, n))
+    dout = DeviceArray(n)
+    _check(lib().sem_galerkin_apply_op(mesh._h, f, None, din.ptr, dout.ptr))
+    out = dout.to_host()
+    din.close()
+    dout.close()
+    return out
+
+
+def gauss_lobatto(n):
+    """gauss_lobatto<double>(n): n + 1 GLL nodes and weights (host only)."""
+    x = np.zeros(n + 1)
+    w = np.zeros(n + 1)
+    _check(lib().sem_gauss_lobatto(int(n), _ptr(x), _ptr(w)))
+    return x, w
+
+
+def derivative_matrix(x):
+    x = _f64(x)
+    D = np.zeros(x.size * x.size)
+    _check(lib().sem_derivative_matrix(x.size, _ptr(x), _ptr(D)))
+    return D.reshape(x.size, x.size, order="F")
+
+
+def quad_metrics(xs, ys, z):
+    z = _f64(z)
+    n = z.size
+    out = [np.zeros(n * n) for _ in range(4)]
+    _check(lib().sem_quad_metrics(_ptr(_f64(xs, 4)), _ptr(_f64(ys, 4)), n, _ptr(z), *[_ptr(a) for a in out]))
+    return [a.reshape(n, n, order="F") for a in out]

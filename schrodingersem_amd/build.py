@@ -36,21 +36,35 @@ UNITS = [
 ]
 
 
-def _compile(unit):
+def _deps_mtime():
+    """newest modification time of the headers every unit may include"""
+    t = 0.0
+    for d in (CSRC, os.path.join(ROOT, "include")):
+        for f in os.listdir(d):
+            if f.endswith((".hpp", ".h")):
+                t = max(t, os.path.getmtime(os.path.join(d, f)))
+    return t
+
+
+def _compile(unit, force=True):
     src, obj, extra = unit
     out = os.path.join(OBJ, obj)
-    cmd = [HIPCC] + COMMON + extra + ["-c", os.path.join(CSRC, src), "-o", out]
+    srcp = os.path.join(CSRC, src)
+    if not force and os.path.exists(out) and os.path.getmtime(out) > max(os.path.getmtime(srcp), _deps_mtime()):
+        return out
+    cmd = [HIPCC] + COMMON + extra + ["-c", srcp, "-o", out]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("hipcc failed for %s:\n%s\n%s" % (src, " ".join(cmd), r.stderr[-6000:]))
     return out
 
 
-def build_lib(jobs=None):
+def build_lib(jobs=None, force=True):
+    """Compile every unit (force) or only those older than their source or a header."""
     os.makedirs(OBJ, exist_ok=True)
     jobs = jobs or min(len(UNITS), os.cpu_count() or 4)
     with ThreadPoolExecutor(max_workers=jobs) as ex:
-        objs = list(ex.map(_compile, UNITS))
+        objs = list(ex.map(lambda u: _compile(u, force), UNITS))
     tmp = LIB + ".tmp"
     cmd = [HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -85,8 +99,9 @@ def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--jobs", type=int, default=None)
     ap.add_argument("--skip-oracle", action="store_true")
+    ap.add_argument("--incremental", action="store_true", help="recompile only out-of-date units")
     a = ap.parse_args(argv)
-    print(build_lib(a.jobs))
+    print(build_lib(a.jobs, force=not a.incremental))
     print(build_cpp_tests())
     if not a.skip_oracle:
         print(build_oracle())

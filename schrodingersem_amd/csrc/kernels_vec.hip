@@ -542,6 +542,44 @@ void launch_pcg_fin_rho(KrylovScalars* sc, const double* partial, int n, hipStre
     hipLaunchKernelGGL(k_pcg_fin_rho, dim3(1), dim3(256), 0, s, sc, partial, n);
 }
 
+// ------------------------------------------------- layouts and diagonals --
+// Element-local arrays between the reference layout (e*N*N + i + j*N) and the
+// chunked device layout (Layout::pos); one thread per entry of the E elements.
+__global__ __launch_bounds__(256) void k_layout(const double* __restrict__ in, double* __restrict__ out, long E, int N,
+                                                int K, int to_dev)
+{
+    const long total = E * N * N;
+    for (long r = (long)blockIdx.x * 256 + threadIdx.x; r < total; r += (long)gridDim.x * 256) {
+        const long e = r / (N * N);
+        const int ij = (int)(r - e * N * N), i = ij % N, j = ij / N;
+        const long d = (e / K) * K * N * N + (long)i * K * N + (e % K) * N + j;
+        if (to_dev) out[d] = in[r];
+        else out[r] = in[d];
+    }
+}
+
+void launch_layout(const double* in, double* out, long E, int N, int K, int to_dev, hipStream_t s)
+{
+    const long total = E * N * N;
+    const int nb = (int)std::max<long>(1, std::min<long>((total + 255) / 256, 65536));
+    hipLaunchKernelGGL(k_layout, dim3(nb), dim3(256), 0, s, in, out, E, N, K, to_dev);
+}
+
+// out = 1 / |d| (0 where d is 0): a symmetric positive Jacobi preconditioner
+__global__ __launch_bounds__(256) void k_abs_recip(double* out, const double* d, size_t n)
+{
+    for (size_t k = (size_t)blockIdx.x * 256 + threadIdx.x; k < n; k += (size_t)gridDim.x * 256) {
+        const double a = fabs(d[k]);
+        out[k] = a > 0.0 ? 1.0 / a : 0.0;
+    }
+}
+
+void launch_abs_recip(double* out, const double* d, size_t n, hipStream_t s)
+{
+    const int nb = (int)std::max<size_t>(1, std::min<size_t>((n + 255) / 256, 65536));
+    hipLaunchKernelGGL(k_abs_recip, dim3(nb), dim3(256), 0, s, out, d, n);
+}
+
 // ---------------------------------------------------- tridiagonal operator --
 __global__ __launch_bounds__(256) void k_tridiag(const double* x, double* y, long n, double lo, double di, double up)
 {

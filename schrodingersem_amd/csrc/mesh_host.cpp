@@ -99,7 +99,7 @@ std::vector<double> diff_matrix(const std::vector<double>& x)
 // Bilinear-quad metrics (Quad.hpp:27-55). X_xi, Y_xi vary with eta (index j),
 // X_eta, Y_eta with xi (index i); the C entry keeps the reference's Y_xi[i]
 // (Quad.hpp:53), which equals Y_xi[j] on every parallelogram.
-static void quad_metrics(const double* xs, const double* ys, const std::vector<double>& z, double* J, double* A,
+void quad_metrics(const double* xs, const double* ys, const std::vector<double>& z, double* J, double* A,
                          double* B, double* C)
 {
     const size_t n = z.size();
@@ -326,5 +326,50 @@ extern "C" int sem_mesh_write(const char* folder, const double* xy, int n_nodes,
     std::fclose(fi);
     std::fclose(fc);
     std::fclose(fe);
+    return SEM_OK;
+}
+
+// ------------------------------------------------ host-only basis entry points --
+// gauss_lobatto<double>(n) (gauss_lobatto.hpp:43-126): n + 1 nodes and weights
+extern "C" int sem_gauss_lobatto(int n, double* x, double* w)
+{
+    if (!x || !w || n < 1 || n > sem::kMaxN - 1) {
+        sem::set_error("sem_gauss_lobatto: need 1 <= n <= " + std::to_string(sem::kMaxN - 1) + " and output arrays");
+        return SEM_EINVAL;
+    }
+    try {
+        std::vector<double> xx, ww;
+        sem::gll_rule(n, xx, ww);
+        std::copy(xx.begin(), xx.end(), x);
+        std::copy(ww.begin(), ww.end(), w);
+    } catch (const std::exception& e) {
+        sem::set_error(e.what());
+        return SEM_EINVAL;
+    }
+    return SEM_OK;
+}
+
+// derivative_matrix(x) (derivative_matrix.hpp:12-38): D (n x n, column-major)
+extern "C" int sem_derivative_matrix(int n, const double* x, double* D)
+{
+    if (!x || !D || n < 1) {
+        sem::set_error("sem_derivative_matrix: bad argument");
+        return SEM_EINVAL;
+    }
+    const std::vector<double> d = sem::diff_matrix(std::vector<double>(x, x + n));
+    std::copy(d.begin(), d.end(), D);
+    return SEM_OK;
+}
+
+// Quad::compute_metrics(z) (Quad.hpp:27-55) for corners (xs, ys): J, A, B, C
+// (n x n, column-major, the reference's C index included)
+extern "C" int sem_quad_metrics(const double* xs, const double* ys, int n, const double* z, double* J, double* A,
+                                double* B, double* C)
+{
+    if (!xs || !ys || !z || !J || !A || !B || !C || n < 1) {
+        sem::set_error("sem_quad_metrics: bad argument");
+        return SEM_EINVAL;
+    }
+    sem::quad_metrics(xs, ys, std::vector<double>(z, z + n), J, A, B, C);
     return SEM_OK;
 }

@@ -70,6 +70,8 @@ struct HostMesh {
 
 void gll_rule(int p, std::vector<double>& x, std::vector<double>& w);
 std::vector<double> diff_matrix(const std::vector<double>& x);
+void quad_metrics(const double* xs, const double* ys, const std::vector<double>& z, double* J, double* A,
+                  double* B, double* C);
 void build_host_mesh(HostMesh& hm, int p, const double* xy, int n_nodes, const int* quads, int n_el);
 void read_mesh_folder(const std::string& folder, std::vector<double>& xy, std::vector<int>& quads);
 

@@ -99,5 +99,9 @@ void launch_pcg_upd(double* x, double* r, const double* p, const double* ap, siz
 void launch_pcg_fin_rho(KrylovScalars* sc, const double* partial, int n, hipStream_t s);
 
 void launch_tridiag(const double* x, double* y, long n, double lo, double di, double up, hipStream_t s);
+// element-local arrays of E elements: reference layout -> device layout (to_dev = 1) or back
+void launch_layout(const double* in, double* out, long E, int N, int K, int to_dev, hipStream_t s);
+// out = 1 / |d| (0 where d = 0)
+void launch_abs_recip(double* out, const double* d, size_t n, hipStream_t s);
 
 }  // namespace sem

@@ -91,3 +91,20 @@ def test_progress_line_is_verbose_tracker(sem, t, T, dt, err):
     if err is not None:
         want += " :: error ~ %.*e" % (prec, err)
     assert sem.progress_line(t, T, dt, err) == want + "\r"
+
+
+@pytest.mark.parametrize("p", [1, 4, 7, 8, 10, 13])
+def test_host_basis_routines_match_oracle(sem, orc, p):
+    # gauss_lobatto / derivative_matrix / Quad::compute_metrics are host-only entry points
+    x, w = sem.gauss_lobatto(p)
+    xo, wo = orc.gll(p)
+    assert np.array_equal(x, xo) if p <= 8 else np.max(np.abs(x - xo)) < 5e-15
+    assert np.max(np.abs(w - wo)) < 1e-14
+    D = sem.derivative_matrix(x)
+    assert np.max(np.abs(D - orc.dmat(x))) < 1e-12 * (p + 1) ** 2
+    xs = np.array([0.0, 1.2, 1.0, -0.1])
+    ys = np.array([0.0, 0.1, 0.9, 1.1])
+    got = sem.quad_metrics(xs, ys, x)
+    want = orc.quad_metrics(xs, ys, x)
+    for a, b in zip(got, want):
+        assert np.max(np.abs(a - b)) <= 1e-14 * np.max(np.abs(b))

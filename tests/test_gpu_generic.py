@@ -1,0 +1,166 @@
+"""The generic boundary: minres / pcg over a caller's operator, dot product and
+preconditioner (minres.hpp:19-20, pcg.hpp:59-60, solver_base.hpp:14-22), the
+Op action with a caller's element operator (galerkin.hpp:24-35), and the mesh
+solvers with a Jacobi preconditioner, against the oracle."""
+import numpy as np
+import pytest
+
+from conftest import SMALL_MESH, rel, structured
+
+pytestmark = pytest.mark.gpu
+
+
+def fd(n=20, k=0.0):
+    x = np.linspace(-1, 1, n)
+    h = x[1] - x[0]
+    lo, di, up = -1 / h**2, 2 / h**2 - k, -1 / h**2
+    A = np.diag(np.full(n, di)) + np.diag(np.full(n - 1, lo), -1) + np.diag(np.full(n - 1, up), 1)
+    return lo, di, up, x * x, A
+
+
+def host_op(A):
+    def op(x, y):
+        y[:] = A @ x
+    return op
+
+
+@pytest.mark.parametrize("k", [0.0, 2.0])
+def test_generic_identity_matches_oracle(sem, orc, k):
+    # tests/minres.cpp and tests/pcg.cpp through the caller-operator entry points
+    lo, di, up, f, A = fd(20, k)
+    n = f.size
+    x = np.zeros(n)
+    r = sem.minres(x, host_op(A), f, n, 1e-5, dot=lambda a, b: float(a @ b))
+    xo, ok, it, res = orc.minres_tridiag(lo, di, up, f, np.zeros(n), n, 1e-5)
+    assert r.success == ok and r.n_iter == it and rel(x, xo) < 1e-10
+    assert np.linalg.norm(A @ x - f) <= 1e-5 * np.linalg.norm(f)
+    # the library's own (device) dot gives the same
+    x2 = np.zeros(n)
+    r2 = sem.minres(x2, host_op(A), f, n, 1e-5)
+    assert r2.n_iter == it and rel(x2, xo) < 1e-10
+    if k == 0.0:
+        x = np.zeros(n)
+        r = sem.pcg(x, host_op(A), f, n, 1e-5)
+        xo, ok, it, res = orc.pcg_tridiag(lo, di, up, f, np.zeros(n), n, 1e-5)
+        assert r.success == ok and r.n_iter == it and rel(x, xo) < 1e-10
+
+
+@pytest.mark.parametrize("consistent", [False, True])
+def test_generic_minres_preconditioned_matches_oracle(sem, orc, consistent):
+    lo, di, up, f, A = fd(40, 2.0)
+    pd = 0.5 + 1.5 * np.random.default_rng(4).random(40)
+    x = np.zeros(40)
+    r = sem.minres(x, host_op(A), f, 400, 1e-12, precond=lambda a, z: np.multiply(pd, a, out=z),
+                   consistent=consistent)
+    xo, ok, it, res = orc.minres_tridiag_pc(lo, di, up, f, np.zeros(40), 400, 1e-12, pd, consistent)
+    assert r.success == ok and abs(r.n_iter - it) <= 1
+    assert rel(x, xo) < 1e-9
+    if consistent:  # a symmetric positive preconditioner: the solution of A x = f
+        assert rel(x, np.linalg.solve(A, f)) < 1e-9
+
+
+def test_generic_pcg_preconditioned_matches_oracle(sem, orc):
+    lo, di, up, f, A = fd(40)
+    pd = 0.5 + 1.5 * np.random.default_rng(3).random(40)
+    x = np.zeros(40)
+    r = sem.pcg(x, host_op(A), f, 400, 1e-14, precond=lambda a, z: np.multiply(pd, a, out=z))
+    xo, ok, it, res = orc.pcg_tridiag_pc(lo, di, up, f, np.zeros(40), 400, 1e-14, pd)
+    assert r.success and ok and r.n_iter == it
+    assert rel(x, xo) < 1e-10 and rel(x, np.linalg.solve(A, f)) < 1e-10
+
+
+def test_generic_device_vectors(sem):
+    # device mode: the callbacks receive device pointers and the library's stream
+    lo, di, up, f, A = fd(30, 2.0)
+    n = f.size
+
+    def op(xp, yp, stream):
+        sem.stream_synchronize(stream)
+        sem.copy_to_device(yp, A @ sem.copy_to_host(xp, n))
+
+    b = sem.DeviceArray.from_host(f)
+    x = sem.DeviceArray.from_host(np.zeros(n))
+    r = sem.minres_device(n, op, b.ptr, x.ptr, 200, 1e-12)
+    assert r.success and rel(x.to_host(), np.linalg.solve(A, f)) < 1e-9
+    x2 = sem.DeviceArray.from_host(np.zeros(n))
+    lo0, di0, up0, f0, A0 = fd(30)
+
+    def op0(xp, yp, stream):
+        sem.stream_synchronize(stream)
+        sem.copy_to_device(yp, A0 @ sem.copy_to_host(xp, n))
+
+    b0 = sem.DeviceArray.from_host(f0)
+    r = sem.minres_device(n, op0, b0.ptr, x2.ptr, 200, 1e-12, solver="pcg")
+    assert r.success and rel(x2.to_host(), np.linalg.solve(A0, f0)) < 1e-9
+    for d in (b, x, x2, b0):
+        d.close()
+
+
+def test_callback_failure_is_an_error(sem):
+    def bad(x, y):
+        raise RuntimeError("operator failed")
+    with pytest.raises(sem.SemError):
+        sem.minres(np.zeros(5), bad, np.ones(5), 10, 1e-8)
+
+
+def test_galerkin_apply_op_with_caller_operator(sem, orc):
+    # the Op action with a host element operator (the oracle's glaplace plus a
+    # coefficient mass term) equals the library's own Helmholtz galerkin apply
+    m = sem.load_mesh(5, SMALL_MESH)
+    o = orc.Mesh.load(5, SMALL_MESH)
+    X, Y = m.coords()
+    c = 2.0 + X * Y
+    N = m.N
+
+    def op(u, y):
+        y[:] = o.element_op(orc.HELMHOLTZ, u.ravel(), c).reshape(u.shape)
+
+    rng = np.random.default_rng(6)
+    u = rng.standard_normal(m.size)
+    got = sem.galerkin_apply_op(m, op, u)
+    want = sem.galerkin_op(sem.OP_HELMHOLTZ, u, m, c)
+    assert rel(got, want) < 1e-13
+    assert rel(got, o.galerkin(orc.HELMHOLTZ, u, c)) < 1e-12
+    del N
+
+
+def graded_mesh(sem, n, p):
+    # a structured mesh of [0,1]^2 with element widths growing 30x across the square
+    xy, q = structured(n, (0, 1, 0, 1), 0.0, 0)
+    xy = np.asarray(xy, dtype=np.float64).reshape(-1, 2).copy()
+    g = lambda t: (np.exp(3.4 * t) - 1) / (np.exp(3.4) - 1)  # noqa: E731
+    xy = np.stack([g(xy[:, 0]), g(xy[:, 1])], 1)
+    return xy.ravel(), q
+
+
+def test_preconditioned_mesh_solvers(sem, orc):
+    xy, q = graded_mesh(sem, 10, 6)
+    m = sem.Mesh.from_arrays(6, xy, q)
+    o = orc.Mesh.from_arrays(6, xy, q)
+    X, Y = m.coords()
+    F = np.sin(3 * X) * np.cos(2 * Y) + X
+    uo, ok, ito, _ = o.poisson(np.zeros(m.size), F, 10 * m.dof(), 1e-13)
+    assert ok
+    its = {}
+    for pc in (sem.PRECOND_IDENTITY, sem.PRECOND_JACOBI):
+        u = np.zeros(m.size)
+        r = sem.poisson_precond(u, F, m, 10 * m.dof(), 1e-13, pc)
+        assert r.success
+        assert rel(sem.unmask(u, m), o.unmask(uo)) < 1e-9
+        its[pc] = r.n_iter
+    assert abs(its[sem.PRECOND_IDENTITY] - ito) <= 2
+    assert its[sem.PRECOND_JACOBI] < its[sem.PRECOND_IDENTITY], its
+    # indefinite Helmholtz (tests/helmholtz.cpp's c) with |diag| Jacobi
+    c = np.full(m.size, -40.0 - np.sqrt(2.0))
+    uo, ok, ito, _ = o.helmholtz(np.zeros(m.size), c, F, 20 * m.dof(), 1e-12)
+    assert ok
+    hits = {}
+    for pc in (sem.PRECOND_IDENTITY, sem.PRECOND_JACOBI):
+        u = np.zeros(m.size)
+        r = sem.helmholtz_precond(u, c, F, m, 20 * m.dof(), 1e-12, pc)
+        assert r.success
+        assert rel(sem.unmask(u, m), o.unmask(uo)) < 1e-8
+        hits[pc] = r.n_iter
+    print("graded mesh iterations: PCG identity %d, Jacobi %d; MINRES identity %d, Jacobi %d (oracle %d)"
+          % (its[0], its[1], hits[0], hits[1], ito))
+    assert abs(hits[sem.PRECOND_IDENTITY] - ito) <= max(2, ito // 50)

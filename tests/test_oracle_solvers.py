@@ -167,3 +167,48 @@ def test_reference_stepper_tests(orc, scheme):
     re, im, ok, stats, _ = o.schrodinger(scheme, np.sin(np.pi * X) * np.sin(np.pi * Y), np.zeros(o.size),
                                          np.zeros(o.size), 0.01, 20 * o.dof(), 1e-8, 0.001)
     assert ok and stats[0] == 10
+
+
+# ----- preconditioned solvers (Precond argument of minres.hpp:19-20, pcg.hpp:59-60) -----
+def test_preconditioned_identity_is_the_identity_path(orc):
+    lo, di, up, f, A = fd(30, 2.0)
+    x0, ok0, it0, r0 = orc.minres_tridiag(lo, di, up, f, np.zeros(30), 200, 1e-12)
+    for consistent in (False, True):
+        x1, ok1, it1, r1 = orc.minres_tridiag_pc(lo, di, up, f, np.zeros(30), 200, 1e-12, np.ones(30), consistent)
+        assert (ok1, it1, r1) == (ok0, it0, r0) and np.array_equal(x1, x0)
+    lo, di, up, f, A = fd(30)
+    x0, ok0, it0, r0 = orc.pcg_tridiag(lo, di, up, f, np.zeros(30), 200, 1e-12)
+    x1, ok1, it1, r1 = orc.pcg_tridiag_pc(lo, di, up, f, np.zeros(30), 200, 1e-12, np.ones(30))
+    assert (ok1, it1, r1) == (ok0, it0, r0) and np.array_equal(x1, x0)
+
+
+def test_diagonal_preconditioned_pcg(orc):
+    # any symmetric positive definite preconditioner keeps the solution: a
+    # varying positive diagonal converges to the direct solve, along other
+    # iterates than the identity (so the preconditioner is applied every step)
+    lo, di, up, f, A = fd(40)
+    pd = 0.5 + 1.5 * np.random.default_rng(3).random(40)
+    x, ok, it, res = orc.pcg_tridiag_pc(lo, di, up, f, np.zeros(40), 400, 1e-14, pd)
+    assert ok and np.linalg.norm(x - np.linalg.solve(A, f)) < 1e-10 * np.linalg.norm(x)
+    x4, _, _, _ = orc.pcg_tridiag_pc(lo, di, up, f, np.zeros(40), 4, 1e-14, pd)
+    x4i, _, _, _ = orc.pcg_tridiag(lo, di, up, f, np.zeros(40), 4, 1e-14)
+    assert np.linalg.norm(x4 - x4i) > 1e-3 * np.linalg.norm(x4i)
+
+
+@pytest.mark.parametrize("c", [0.37, 5.0])
+def test_minres_consistent_scalar_preconditioner(orc, c):
+    # P = c I changes nothing in exact arithmetic when beta is normalised consistently
+    lo, di, up, f, A = fd(30, 2.0)
+    x0, ok0, it0, _ = orc.minres_tridiag(lo, di, up, f, np.zeros(30), 200, 1e-12)
+    x1, ok1, it1, _ = orc.minres_tridiag_pc(lo, di, up, f, np.zeros(30), 200, 1e-12, np.full(30, c), True)
+    assert ok1 and abs(it1 - it0) <= 1 and np.linalg.norm(x1 - x0) < 1e-9 * np.linalg.norm(x0)
+    # the reference's normalisation (minres.hpp:63) scales beta by sqrt(c) and does not give that solution
+    x2, ok2, it2, _ = orc.minres_tridiag_pc(lo, di, up, f, np.zeros(30), 30, 1e-12, np.full(30, c), False)
+    assert np.linalg.norm(x2 - x0) > 1e-6 * np.linalg.norm(x0)
+
+
+def test_minres_consistent_jacobi_indefinite(orc):
+    lo, di, up, f, A = fd(40, 2.0)
+    pd = 1.0 / np.abs(np.diag(A))
+    x, ok, it, res = orc.minres_tridiag_pc(lo, di, up, f, np.zeros(40), 400, 1e-13, pd, True)
+    assert ok and np.linalg.norm(x - np.linalg.solve(A, f)) < 1e-9 * np.linalg.norm(x)
