@@ -267,6 +267,16 @@ typedef struct sem_krylov {
 } sem_krylov;
 int sem_minres(const sem_krylov* k, const double* b, double* x, int maxit, double tol, sem_solver_results* out);
 int sem_pcg(const sem_krylov* k, const double* b, double* x, int maxit, double tol, sem_solver_results* out);
+/* sem_minres without any allocation, for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc). No counterpart in the reference. Device vectors only
+ * (k->host = 0) and n <= 262144: the workspace (8n doubles of vectors plus the
+ * dot partials, about 16 MB) is a static device array that every GPU the
+ * library runs on holds until the process ends; larger n returns SEM_EINVAL.
+ * The work runs on hipStreamPerThread, which the callbacks receive as
+ * `stream`. Results equal sem_minres's bit for bit. Every call on one GPU
+ * shares that GPU's array: calls that overlap on one GPU race. */
+int sem_minres_noalloc(const sem_krylov* k, const double* b, double* x, int maxit, double tol, sem_solver_results* out);
 
 /* Preconditioners of the mesh solvers (no counterpart in the reference beyond
  * IdentityPreconditioner, solver_base.hpp:14-22). */

@@ -129,6 +129,7 @@ def lib():
             "sem_stepper_set_guard": (C.c_int, [C.c_void_p, C.c_int]),
             "sem_minres": (C.c_int, [C.POINTER(_Krylov), _dp, _dp, C.c_int, C.c_double, C.POINTER(_Results)]),
             "sem_pcg": (C.c_int, [C.POINTER(_Krylov), _dp, _dp, C.c_int, C.c_double, C.POINTER(_Results)]),
+            "sem_minres_noalloc": (C.c_int, [C.POINTER(_Krylov), _dp, _dp, C.c_int, C.c_double, C.POINTER(_Results)]),
             "sem_poisson_precond": (C.c_int, [C.c_void_p, _dp, _dp, C.c_int, C.c_double, C.c_int,
                                               C.POINTER(_Results)]),
             "sem_helmholtz_precond": (C.c_int, [C.c_void_p, _dp, _dp, _dp, C.c_int, C.c_double, C.c_int,
@@ -862,10 +863,11 @@ def pcg(x, A, b, maxit, tol, dot=None, precond=None):
 
 
 def minres_device(n, A, b_dev, x_dev, maxit, tol, dot=None, precond=None, consistent=False, solver="minres"):
-    """The same on device vectors (pointers from DeviceArray): A(x_ptr, y_ptr, stream)."""
+    """The same on device vectors (pointers from DeviceArray): A(x_ptr, y_ptr, stream);
+    solver "minres", "pcg" or "minres_noalloc" (the entry point that allocates nothing)."""
     k, keep = _krylov(n, A, dot, precond, False, consistent)
     r = _Results()
-    fn = lib().sem_minres if solver == "minres" else lib().sem_pcg
+    fn = {"minres": lib().sem_minres, "pcg": lib().sem_pcg, "minres_noalloc": lib().sem_minres_noalloc}[solver]
     _check(fn(C.byref(k), b_dev, x_dev, int(maxit), float(tol), C.byref(r)))
     del keep
     return SolverResults(r)

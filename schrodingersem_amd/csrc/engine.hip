@@ -11,6 +11,7 @@
 #include <functional>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <thread>
 #include <unordered_map>
@@ -61,6 +62,13 @@ struct KrylovOpts {
     // their Krylov state starts stopped, so x keeps its value there
     const uint8_t* frozen = nullptr;
 };
+
+// Workspace of sem_minres_noalloc: one array per GPU, held until the process
+// ends. Host code must not pass `scratch` itself to a kernel (it names the
+// host-side shadow); it uses the device address noalloc_scratch() returns.
+constexpr long kNoAllocMaxN = 262144;                      // largest n of a solve
+constexpr size_t kNoAllocDoubles = 8 * (size_t)kNoAllocMaxN + 1024 + 8;  // 8 vectors, partials, result
+static __device__ __align__(256) double scratch[kNoAllocDoubles];
 
 // marks instance k's Krylov state stopped before the first step (not converged)
 inline void freeze_instance(KrylovScalars& h, const uint8_t* frozen, int k)
@@ -2969,6 +2977,66 @@ static int krylov_solve(bool use_minres, const sem_krylov* k, const double* b, d
         Results r = use_minres ? generic_minres(g, dx, db, maxit, tol, buf) : generic_pcg(g, dx, db, maxit, tol, buf);
         SEM_HIP(hipStreamSynchronize(g.s));
         if (k->host) SEM_HIP(hipMemcpy(x, dx, n * sizeof(double), hipMemcpyDeviceToHost));
+        fill_results(out, r);
+        return (int)SEM_OK;
+    });
+}
+
+// The device address of `scratch` on the current device, looked up once per
+// device (each GPU holds its own copy of the array at its own address).
+static double* noalloc_scratch()
+{
+    static std::mutex mu;
+    static std::map<int, double*> addr;
+    int dev = 0;
+    SEM_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = addr.find(dev);
+    if (it != addr.end()) return it->second;
+    void* p = nullptr;
+    SEM_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) throw HipError{hipErrorInvalidSymbol, "hipGetSymbolAddress(scratch)"};
+    addr[dev] = (double*)p;
+    return (double*)p;
+}
+
+int sem_minres_noalloc(const sem_krylov* k, const double* b, double* x, int maxit, double tol, sem_solver_results* out)
+{
+    return guarded([&] {
+        if (!k || !k->A || k->n < 1 || !b || !x) throw std::invalid_argument("sem_minres_noalloc: bad argument");
+        if (k->host) throw std::invalid_argument("sem_minres_noalloc: device vectors only (host = 0)");
+        if (k->n > kNoAllocMaxN)
+            throw std::invalid_argument("sem_minres_noalloc: n = " + std::to_string(k->n) + " exceeds the workspace (n <= " +
+                                        std::to_string(kNoAllocMaxN) + ")");
+        if (!check_gpu()) return (int)SEM_EHIP;
+        const long n = k->n;
+        double* ws = noalloc_scratch();
+        double h_result = 0;  // pageable host word for the dot results
+        GenericCtx g{};
+        g.s = hipStreamPerThread;
+        g.n = n;
+        g.consistent = k->consistent != 0;
+        g.part = ws + 8 * (size_t)kNoAllocMaxN;
+        g.red = g.part + 1024;
+        g.h_red = &h_result;
+        void* vs = (void*)g.s;
+        auto fail = [](const char* what) { throw std::invalid_argument(std::string(what) + " callback failed"); };
+        g.A = [&](const double* in, double* o) {
+            if (k->A(k->A_user, in, o, vs) != 0) fail("operator");
+        };
+        if (k->dot)
+            g.dotf = [&](const double* a, const double* c) {
+                double r = 0;
+                if (k->dot(k->dot_user, a, c, &r, vs) != 0) fail("dot");
+                return r;
+            };
+        if (k->precond)
+            g.P = [&](const double* in, double* o) {
+                if (k->precond(k->precond_user, in, o, vs) != 0) fail("preconditioner");
+            };
+        // the eight vectors of generic_minres are written before they are read
+        Results r = generic_minres(g, x, b, maxit, tol, ws);
+        SEM_HIP(hipStreamSynchronize(g.s));
         fill_results(out, r);
         return (int)SEM_OK;
     });
