@@ -297,7 +297,7 @@ def bench_sdirk(sem, args, workload):
     avg_us = total_us / max(launches, 1)
     achieved = bpl / (avg_us * 1e-6) / 1e9 if launches else 0.0
     shape = tuned["fp32"]["shape"]
-    fused = sem.get_step_mode(m, sem.PREC_MIXED) == 1
+    fused = sem.get_step_mode(m, sem.PREC_MIXED) >= 1
     kbase = "k_minres_fused" if fused else "k_minres_op"
     traffic = None
     prof = os.path.join(ROOT, "profiles", "hbm_traffic_%s.json" % wname)

@@ -652,7 +652,9 @@ __global__ __launch_bounds__(128 * CPB) void k_minres_upd(MinresUpdArgs<T> a)
 // component. Every workgroup first evaluates the recurrence of this step
 // (minres_fused_prologue) from the previous step's per-chunk sums; the
 // workgroup holding an instance's first chunk stores the state in the other
-// buffer.
+// buffer. In step mode 2 the closing kernel evaluates the recurrence after
+// every step and each launch starts from that state (first = 1); a variant
+// compiled without the in-kernel reduction (fewer registers) was slower.
 template <int N, typename T, int KIND, int CPB, bool PERSIST>
 __global__ __launch_bounds__(64 * CPB * ((KIND == K_RES) ? 2 : 1))
 __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 3 : 1, 8))) void k_minres_fused(MinresFusedArgs<T> a, Dm<N, T> D)
@@ -1038,19 +1040,17 @@ static void minres_upd_N(const MinresUpdArgs<T>& a, int cpb, int grid, hipStream
     }
 }
 
+// The fused step runs the FP32 inner solver of the mixed-precision stage
+// solves (operator Res) only.
 template <int N, typename T, int CPB, bool P>
 static void minres_fused_NC(int kind, const MinresFusedArgs<T>& a, const Dm<N, T>& D, int grid, hipStream_t s,
                             hipEvent_t e0, hipEvent_t e1)
 {
     const int items = a.nchunks / CPB;
     dim3 g(P ? std::min(grid, items) : items);
-    dim3 b1(64 * CPB), b2(128 * CPB);
-    switch (kind) {
-    case K_LAPLACE: SEM_LAUNCH_TIMED((k_minres_fused<N, T, K_LAPLACE, CPB, P>), g, b1, s, e0, e1, a, D); break;
-    case K_HELMHOLTZ: SEM_LAUNCH_TIMED((k_minres_fused<N, T, K_HELMHOLTZ, CPB, P>), g, b1, s, e0, e1, a, D); break;
-    case K_RES: SEM_LAUNCH_TIMED((k_minres_fused<N, T, K_RES, CPB, P>), g, b2, s, e0, e1, a, D); break;
-    default: throw std::runtime_error("minres: unsupported operator kind");
-    }
+    dim3 b2(128 * CPB);
+    if (kind != K_RES) throw std::runtime_error("minres: the one-launch step serves the Schrodinger stage operator");
+    SEM_LAUNCH_TIMED((k_minres_fused<N, T, K_RES, CPB, P>), g, b2, s, e0, e1, a, D);
 }
 
 template <int N, typename T>
@@ -1153,9 +1153,14 @@ template void launch_element_op_d<double>(int, int, const OpArgs<double>&, const
 template void launch_minres_op_d<double>(int, int, const MinresOpArgs<double>&, const double*, int, int, hipStream_t,
                                           hipEvent_t, hipEvent_t);
 template void launch_minres_upd<double>(int, const MinresUpdArgs<double>&, int, int, hipStream_t, hipEvent_t, hipEvent_t);
-template void launch_minres_fused<double>(int, int, const MinresFusedArgs<double>&, const double*, int, int, hipStream_t,
-                                           hipEvent_t, hipEvent_t);
 template void launch_pcg_op_d<double>(int, const PcgOpArgs<double>&, int, const double*, hipStream_t);
+// the one-launch step is an FP32 kernel only (FP64 solves use the two-launch step)
+template <>
+void launch_minres_fused<double>(int, int, const MinresFusedArgs<double>&, const double*, int, int, hipStream_t,
+                                 hipEvent_t, hipEvent_t)
+{
+    throw std::runtime_error("the one-launch Lanczos step runs in FP32 only");
+}
 #else
 template void launch_element_op_d<float>(int, int, const OpArgs<float>&, const float*, hipStream_t);
 template void launch_minres_op_d<float>(int, int, const MinresOpArgs<float>&, const float*, int, int, hipStream_t,
