@@ -192,6 +192,13 @@ def measure_glaplace(sem, copy_gbs, extra_meshes):
                                          "frac": round(gbs / HBM_PEAK_GBS, 4),
                                          "frac_of_copy": round(gbs / copy_gbs, 4)}
     del mb
+    # the FP32 stage-operator apply (the inner solver's operator) per contraction, on the last mesh
+    mres = extra_meshes[-1][1]
+    for mode, tag in ((0, "valu"), (3, "mfma_bf16x3"), (1, "mfma_bf16")):
+        sem.set_contraction(mres, mode)
+        us, by = sem.time_element_op(mres, sem.OP_SCHRO_RES, sem.PREC_MIXED, 50)
+        out["res_apply_fp32_" + tag] = {"us": round(us, 2), "GB/s": round(by / us / 1e3, 1)}
+    sem.set_contraction(mres, 0)
     for name, mm in extra_meshes:
         us, by = sem.time_element_op(mm, sem.OP_LAPLACE, sem.PREC_FP64, 50)
         out[name] = {"us": round(us, 2), "GB/s": round(by / us / 1e3, 1),
@@ -278,11 +285,19 @@ def bench_sdirk(sem, args, workload):
             ("fp64", sem.PREC_FP64, sem.GUESS_REFERENCE, args.tol),
             ("mixed_ext", sem.PREC_MIXED, sem.GUESS_EXTRAPOLATED, args.tol),
             ("mixed_learned", sem.PREC_MIXED, sem.GUESS_LEARNED, args.tol)]
+    if not args.skip_mfma:
+        # the FP32 inner operator's stiffness contractions on MFMA tiles
+        # (split-bf16 and bf16 operands, FP32 accumulation) against the VALU form
+        arms += [("mixed_ext_mfma_bf16x3", sem.PREC_MIXED, sem.GUESS_EXTRAPOLATED, args.tol, 3),
+                 ("mixed_ext_mfma_bf16", sem.PREC_MIXED, sem.GUESS_EXTRAPOLATED, args.tol, 1)]
     res = {}
-    for name, prec, guess, tol in arms:
+    for arm in arms:
+        name, prec, guess, tol = arm[:4]
+        sem.set_contraction(m, arm[4] if len(arm) > 4 else 0)
         log("arm %s (tol %g)" % (name, tol))
         res[name] = run_arm(sem, m, scheme, V, ck, name, prec, guess, tol, args.warmup, args.steps,
                             args.learn_steps, args.learn_epochs, timing=(name != "truth"))
+    sem.set_contraction(m, 0)
     truth = res.pop("truth")
     for name, r in res.items():
         r["err"] = float(np.linalg.norm(r["psi"] - truth["psi"]) / np.linalg.norm(truth["psi"]))
@@ -396,6 +411,7 @@ def main():
     ap.add_argument("--p3", type=int, default=8)
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-glaplace", action="store_true")
+    ap.add_argument("--skip-mfma", action="store_true", help="leave out the MFMA-contraction arms")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--instances", type=int, default=64)
     ap.add_argument("--batch-steps", type=int, default=100)

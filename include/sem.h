@@ -121,6 +121,20 @@ int sem_mesh_write(const char* folder, const double* xy, int n_nodes, const int*
 int sem_glaplace_apply(sem_mesh* m, const double* u, double* y);
 /* Element operator only (no unmask / assembly), any kind; coef/alpha as in sem_op_kind. */
 int sem_element_apply(sem_mesh* m, int kind, const double* coef, double alpha, const double* in, double* out);
+/* The element operator in FP32 (the inner solver's precision; host arrays in
+ * and out in FP64): with SEM_OP_SCHRO_RES the stiffness contractions follow
+ * sem_set_contraction. No counterpart in the reference. */
+int sem_element_apply_fp32(sem_mesh* m, int kind, const double* coef, double alpha, const double* in, double* out);
+/* Contractions of the stiffness action in the FP32 inner solver's stage
+ * operator (fused Lanczos step and sem_element_apply_fp32 with Res): 0 VALU
+ * FMAs (default); 1 the stacked-element form on MFMA, v_mfma_f32_16x16x16bf16
+ * tiles with bf16 operands and FP32 accumulation (about 4e-3 relative error;
+ * the FP64 refinement still stops on the true residual, the inner solves ask
+ * for a 5e-3 reduction); 3 the same with each operand split into hi + lo bf16
+ * (three MFMAs per tile, about 2^-16 relative error). No counterpart in the
+ * reference. */
+int sem_set_contraction(sem_mesh* m, int mode);
+int sem_get_contraction(const sem_mesh* m, int* mode);
 /* galerkin_op: unmask -> element op -> global_sum -> mask (galerkin.hpp:24-35);
  * 2-component kinds take and return [re | im]. */
 int sem_galerkin_apply(sem_mesh* m, int kind, const double* coef, double alpha, const double* in, double* out);

@@ -127,6 +127,9 @@ def lib():
             "sem_stepper_destroy": (None, [C.c_void_p]),
             "sem_stepper_set_learning": (C.c_int, [C.c_void_p, C.c_int, C.c_float]),
             "sem_stepper_set_guard": (C.c_int, [C.c_void_p, C.c_int]),
+            "sem_element_apply_fp32": (C.c_int, [C.c_void_p, C.c_int, _dp, C.c_double, _dp, _dp]),
+            "sem_set_contraction": (C.c_int, [C.c_void_p, C.c_int]),
+            "sem_get_contraction": (C.c_int, [C.c_void_p, _ip]),
             "sem_minres": (C.c_int, [C.POINTER(_Krylov), _dp, _dp, C.c_int, C.c_double, C.POINTER(_Results)]),
             "sem_pcg": (C.c_int, [C.POINTER(_Krylov), _dp, _dp, C.c_int, C.c_double, C.POINTER(_Results)]),
             "sem_minres_noalloc": (C.c_int, [C.POINTER(_Krylov), _dp, _dp, C.c_int, C.c_double, C.POINTER(_Results)]),
@@ -982,3 +985,28 @@ def quad_metrics(xs, ys, z):
     out = [np.zeros(n * n) for _ in range(4)]
     _check(lib().sem_quad_metrics(_ptr(_f64(xs, 4)), _ptr(_f64(ys, 4)), n, _ptr(z), *[_ptr(a) for a in out]))
     return [a.reshape(n, n, order="F") for a in out]
+
+
+CONTRACT_VALU, CONTRACT_BF16, CONTRACT_BF16X3 = 0, 1, 3
+
+
+def element_op_fp32(kind, u, mesh, coef=None, alpha=0.0):
+    """The element operator in FP32 (no assembly); Res follows set_contraction."""
+    nc = _ncomp(kind)
+    u = _f64(u, nc * mesh.size)
+    out = np.zeros_like(u)
+    c = None if coef is None else _f64(coef, mesh.size)
+    _check(lib().sem_element_apply_fp32(mesh._h, int(kind), None if c is None else _ptr(c), float(alpha), _ptr(u),
+                                        _ptr(out)))
+    return out
+
+
+def set_contraction(mesh, mode):
+    """Stiffness contractions of the FP32 inner solver: 0 VALU, 1 bf16 MFMA, 3 split-bf16 MFMA."""
+    _check(lib().sem_set_contraction(mesh._h, int(mode)))
+
+
+def get_contraction(mesh):
+    v = C.c_int()
+    _check(lib().sem_get_contraction(mesh._h, C.byref(v)))
+    return v.value

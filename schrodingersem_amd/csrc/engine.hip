@@ -128,6 +128,9 @@ struct sem_mesh {
     // default); the assembled stiffness and mass diagonals are kept once
     // computed, the scale itself for the last (field, generation, alpha)
     int inner_scale = 1;
+    // stiffness contractions of the FP32 inner solver's operator: 0 VALU FMAs,
+    // 1 bf16 MFMA tiles, 3 split-bf16 MFMA tiles (FP32 accumulation)
+    int contraction = 0;
     bool is_geo = false;
     const double* is_cf = nullptr;
     long is_gen = -1;
@@ -703,6 +706,7 @@ static Results minres_fused1(sem_mesh* m, int kind, const T* cf, double alpha, c
             }
             a.step = i;
             a.first = ext || (b_ == 0);
+            a.mf = sizeof(T) == 4 ? m->contraction : 0;
             a.dscale = dscale;
             const bool tim = m->timing == 2 || (m->timing == 1 && sampled_step(i));
             launch_minres_fused<T>(m->L.N, kind, a, m->dmat<T>(), shp.op_cpb, shp.op_grid, s,
@@ -1170,6 +1174,11 @@ static const float* inner_scale(sem_mesh* m, const double* cf64, double alpha)
     return S;
 }
 
+// Smallest relative reduction asked of one FP32 inner solve: what FP32 (or the
+// split-bf16 contractions) can deliver; bf16 contractions carry about 4e-3
+// relative error, so asking more of one sweep only adds Lanczos steps.
+static double inner_tol_floor(const sem_mesh* m) { return m->contraction == 1 ? 5e-3 : 2e-5; }
+
 // FP64 iterative refinement around an FP32 MINRES: r = b - A x in FP64, the
 // correction A d = r is solved in FP32 (FP64 recurrence scalars and dots),
 // x += d in FP64. Stops on the true FP64 residual, ||r|| < tol*||r0|| + atol
@@ -1355,7 +1364,7 @@ static Results solve_res(Stepper& st, double alpha, const double* b, double* x, 
     Results r;
     const long ref0 = m->stats.refinements;
     if (st.opts.precision == SEM_PREC_MIXED) {
-        r = solve_mixed(m, K_RES, st.V.c64, st.V.c32, alpha, b, x, zero, ko, 2e-5);
+        r = solve_mixed(m, K_RES, st.V.c64, st.V.c32, alpha, b, x, zero, ko, inner_tol_floor(m));
         st.stats.max_true_rel_residual = std::max(st.stats.max_true_rel_residual, m->last_true_rel);
     } else
         r = minres_fused<double>(m, K_RES, st.V.c64, alpha, b, x, zero, ko);
@@ -2400,6 +2409,56 @@ int sem_element_apply(sem_mesh* m, int kind, const double* coef, double alpha, c
     });
 }
 
+int sem_element_apply_fp32(sem_mesh* m, int kind, const double* coef, double alpha, const double* in, double* out)
+{
+    return guarded([&] {
+        if (!m || !in || !out || kind < 0 || kind > 5) throw std::invalid_argument("sem_element_apply_fp32: bad argument");
+        if ((kind == K_HELMHOLTZ || kind == K_RES || kind == K_F || kind == K_L) && !coef)
+            throw std::invalid_argument("sem_element_apply_fp32: this operator needs a coefficient field");
+        const int nc = op_ncomp(kind);
+        const size_t n = (size_t)nc * m->len();
+        double* din = m->buf<double>("api_in", n);
+        double* dout = m->buf<double>("api_out", n);
+        float* fin = m->buf<float>("api_in32", n);
+        float* fout = m->buf<float>("api_out32", n);
+        m->upload(din, in, nc);
+        launch_cast<float, double>(fin, din, n, m->stream);
+        Field f;
+        if (coef) f = make_field(m, coef, "api_cf");
+        OpArgs<float> a{};
+        a.in = fin;
+        a.out = fout;
+        a.geo = m->geo32;
+        a.cf = f.c32;
+        a.alpha = (float)alpha;
+        a.post = 1.0f;
+        a.len = m->len();
+        a.nchunks = m->L.nchunks;
+        a.mf = kind == K_RES ? m->contraction : 0;
+        launch_element_op_d<float>(m->L.N, kind, a, m->D32.data(), m->stream);
+        launch_cast<double, float>(dout, fout, n, m->stream);
+        m->download(out, dout, nc);
+        return (int)SEM_OK;
+    });
+}
+
+int sem_set_contraction(sem_mesh* m, int mode)
+{
+    if (!m || (mode != 0 && mode != 1 && mode != 3)) {
+        set_error("sem_set_contraction: mode is 0 (VALU), 1 (bf16 MFMA) or 3 (split-bf16 MFMA)");
+        return SEM_EINVAL;
+    }
+    m->contraction = mode;
+    return SEM_OK;
+}
+
+int sem_get_contraction(const sem_mesh* m, int* mode)
+{
+    if (!m || !mode) return SEM_EINVAL;
+    *mode = m->contraction;
+    return SEM_OK;
+}
+
 int sem_glaplace_apply(sem_mesh* m, const double* u, double* y)
 {
     return sem_element_apply(m, SEM_OP_LAPLACE, nullptr, 0.0, u, y);
@@ -2841,7 +2900,7 @@ int sem_stage_solve(sem_mesh* m, const double* V, double alpha, const double* b,
         o.tol = tol;
         o.atol = tol;
         Results r;
-        if (precision == SEM_PREC_MIXED) r = solve_mixed(m, K_RES, f.c64, f.c32, alpha, db, dz, false, o, 2e-5);
+        if (precision == SEM_PREC_MIXED) r = solve_mixed(m, K_RES, f.c64, f.c32, alpha, db, dz, false, o, inner_tol_floor(m));
         else r = minres_fused<double>(m, K_RES, f.c64, alpha, db, dz, false, o);
         // unmasked (consistent) solution
         m->download(z, dz, 2);
@@ -3385,7 +3444,7 @@ int sem_time_element_op(sem_mesh* m, int kind, int precision, int reps, double* 
             float* cf = m->buf<float>("t_cf32", m->len());
             launch_fill<float>(in, 0.5f, n, m->stream);
             launch_fill<float>(cf, 0.25f, m->len(), m->stream);
-            OpArgs<float> a{in, out, m->geo32, cf, 0.1f, 1.0f, m->len(), m->L.nchunks};
+            OpArgs<float> a{in, out, m->geo32, cf, 0.1f, 1.0f, m->len(), m->L.nchunks, kind == K_RES ? m->contraction : 0};
             launch_element_op_d<float>(N, kind, a, m->D32.data(), m->stream);
             SEM_HIP(hipEventRecord(e0, m->stream));
             for (int r = 0; r < reps; ++r) launch_element_op_d<float>(N, kind, a, m->D32.data(), m->stream);

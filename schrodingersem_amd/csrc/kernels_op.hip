@@ -124,6 +124,162 @@ __device__ __forceinline__ void stiffness(const Dm<N, T>& D, T* lds0, T* lds1, i
     for (int i = 0; i < N; ++i) out[i] += qe[i];
 }
 
+// ----------------------------------------------- MFMA stacked contractions --
+// The D-contractions of the stiffness action as dense 16 x 16 x 16 bf16 MFMA
+// tiles with FP32 accumulation (v_mfma_f32_16x16x16bf16_1k). Lane l holds
+// A[l%16][4(l/16)+r] and B[4(l/16)+r][l%16] of the operands and receives
+// C[4(l/16)+r][l%16], r = 0..3. A wave's chunk is stacked two ways:
+//   xi direction:  C = M X with the 64 lanes (element columns) as the columns of X;
+//   eta direction: C = X M with the K*N (plane, element) pairs of the chunk as rows
+//                  and the element's column index as the contracted index.
+// Both go through one LDS scratch of 64 columns x 17 (odd pitch) per wave.
+// SPLIT = 1 rounds operands to bf16; SPLIT = 3 represents each operand as
+// hi + lo bf16 and sums hi*hi + hi*lo + lo*hi (about 2^-16 relative error).
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+constexpr int kMfPitch = 17;
+
+__device__ __forceinline__ short f2bf(float f)  // round to nearest even (finite values)
+{
+    unsigned u = __float_as_uint(f);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (short)(u >> 16);
+}
+__device__ __forceinline__ float bf2f(short h) { return __uint_as_float(((unsigned)(unsigned short)h) << 16); }
+
+__device__ __forceinline__ void split_bf(const float (&v)[4], bf16x4& hi, bf16x4& lo)
+{
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const short h = f2bf(v[r]);
+        hi[r] = h;
+        lo[r] = f2bf(v[r] - bf2f(h));
+    }
+}
+
+template <int SPLIT>
+__device__ __forceinline__ f32x4 mm16(bf16x4 ahi, bf16x4 alo, bf16x4 bhi, bf16x4 blo)
+{
+    f32x4 c = {0.f, 0.f, 0.f, 0.f};
+    if constexpr (SPLIT == 3) {  // small terms first
+        c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ahi, blo, c, 0, 0, 0);
+        c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(alo, bhi, c, 0, 0, 0);
+    }
+    return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ahi, bhi, c, 0, 0, 0);
+}
+
+// D in the two operand layouts: P[r] = D(l%16, 4(l/16)+r), Q[r] = D(4(l/16)+r, l%16)
+// (zero outside N x N), hi and lo parts
+struct MfD {
+    bf16x4 p_hi, p_lo, q_hi, q_lo;
+};
+
+template <int N>
+__device__ __forceinline__ MfD mf_dmat(const Dm<N, float>& D, int lane)
+{
+    const int m = lane % 16, kb = 4 * (lane / 16);
+    float p[4], q[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int k = kb + r;
+        const bool in = m < N && k < N;
+        p[r] = in ? D.v[m + k * N] : 0.f;
+        q[r] = in ? D.v[k + m * N] : 0.f;
+    }
+    MfD d;
+    split_bf(p, d.p_hi, d.p_lo);
+    split_bf(q, d.q_hi, d.q_lo);
+    return d;
+}
+
+// y = M x, column per lane (xi direction); A operand (ahi, alo) holds M
+template <int N, int SPLIT>
+__device__ __forceinline__ void mf_xi(float* S, int lane, const float (&x)[N], bf16x4 ahi, bf16x4 alo, float (&y)[N])
+{
+    wave_lds_sync();
+#pragma unroll
+    for (int i = 0; i < N; ++i) S[lane * kMfPitch + i] = x[i];
+    wave_lds_sync();
+    const int c16 = lane % 16, kb = 4 * (lane / 16);
+    f32x4 c[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        const int col = 16 * t + c16;
+        float b[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) b[r] = (kb + r < N) ? S[col * kMfPitch + kb + r] : 0.f;
+        bf16x4 bhi, blo;
+        split_bf(b, bhi, blo);
+        c[t] = mm16<SPLIT>(ahi, alo, bhi, blo);
+    }
+    wave_lds_sync();
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+            if (kb + r < N) S[(16 * t + c16) * kMfPitch + kb + r] = c[t][r];
+    wave_lds_sync();
+#pragma unroll
+    for (int i = 0; i < N; ++i) y[i] = S[lane * kMfPitch + i];
+}
+
+// y(i, el, j) = sum_k x(i, el, k) M(k, j), column per lane (eta direction);
+// B operand (bhi, blo) holds M
+template <int N, int SPLIT>
+__device__ __forceinline__ void mf_eta(float* S, int lane, const float (&x)[N], bf16x4 bhi, bf16x4 blo, float (&y)[N])
+{
+    constexpr int K = 64 / N, R = K * N;
+    wave_lds_sync();
+#pragma unroll
+    for (int i = 0; i < N; ++i) S[lane * kMfPitch + i] = x[i];
+    wave_lds_sync();
+    const int c16 = lane % 16, kb = 4 * (lane / 16);
+    f32x4 c[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        const int rho = 16 * t + c16, i = rho / K, el = rho % K;
+        float a[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) a[r] = (rho < R && kb + r < N) ? S[(el * N + kb + r) * kMfPitch + i] : 0.f;
+        bf16x4 ahi, alo;
+        split_bf(a, ahi, alo);
+        c[t] = mm16<SPLIT>(ahi, alo, bhi, blo);
+    }
+    wave_lds_sync();
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int rho = 16 * t + kb + r;
+            if (rho < R && c16 < N) S[((rho % K) * N + c16) * kMfPitch + rho / K] = c[t][r];
+        }
+    wave_lds_sync();
+#pragma unroll
+    for (int i = 0; i < N; ++i) y[i] = S[lane * kMfPitch + i];
+}
+
+// the stiffness action of `stiffness` (same inputs and outputs per lane) with
+// the contractions on MFMA; q, ga, gb, gc are zero on inactive lanes
+template <int N, int SPLIT>
+__device__ __forceinline__ void stiffness_mf(const MfD& d, float* S, int lane, const float (&q)[N], const float (&ga)[N],
+                                             const float (&gb)[N], const float (&gc)[N], float (&out)[N])
+{
+    float qxi[N], qe[N];
+    mf_xi<N, SPLIT>(S, lane, q, d.p_hi, d.p_lo, qxi);   // D u
+    mf_eta<N, SPLIT>(S, lane, q, d.p_hi, d.p_lo, qe);   // u D^T
+    float g1[N], g2[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        g1[i] = ga[i] * qxi[i] - gb[i] * qe[i];
+        g2[i] = gc[i] * qe[i] - gb[i] * qxi[i];
+    }
+    float o2[N];
+    mf_xi<N, SPLIT>(S, lane, g1, d.q_hi, d.q_lo, out);  // D^T G1
+    mf_eta<N, SPLIT>(S, lane, g2, d.q_hi, d.q_lo, o2);  // G2 D
+#pragma unroll
+    for (int i = 0; i < N; ++i) out[i] += o2[i];
+}
+
 template <int N>
 __device__ __forceinline__ bool is_interior(int i, int l)
 {
@@ -160,11 +316,13 @@ struct Col {
 
 // Element operator of every sem_op_kind; ring entries are written raw (the DSS
 // kernel assembles them), interior entries are final and scaled by `post`.
-template <int N, typename T, int KIND, bool NT>
+template <int N, typename T, int KIND, bool NT, int MF = 0>
 __global__ __launch_bounds__(64) void k_element_op(OpArgs<T> a, Dm<N, T> D)
 {
     using S = Shape<N>;
+    static_assert(MF == 0 || sizeof(T) == 4, "MFMA contractions serve the FP32 apply");
     __shared__ T lds0[S::LDS], lds1[S::LDS];
+    __shared__ float mfs[MF ? 64 * kMfPitch : 1];
     const int lane = threadIdx.x;
     const bool act = lane < S::KN;
     const int el = act ? lane / N : 0;
@@ -186,8 +344,12 @@ __global__ __launch_bounds__(64) void k_element_op(OpArgs<T> a, Dm<N, T> D)
         c.ld(a.in, q0);
         c.ld(a.in + len, q1);
         c.ld(a.cf, cf);
+        auto stiff = [&](const T (&q)[N], T (&o)[N]) {
+            if constexpr (MF != 0) stiffness_mf<N, MF>(mf_dmat<N>(D, lane), mfs, lane, q, ga, gb, gc, o);
+            else stiffness<N>(D, lds0, lds1, el, l, act, q, ga, gb, gc, o);
+        };
         // component 0 output from K q1
-        stiffness<N>(D, lds0, lds1, el, l, act, q1, ga, gb, gc, ky);
+        stiff(q1, ky);
         if constexpr (KIND == K_RES) {
             T gj[N];
             c.ld(a.geo + 3 * len, gj);
@@ -198,7 +360,7 @@ __global__ __launch_bounds__(64) void k_element_op(OpArgs<T> a, Dm<N, T> D)
             for (int i = 0; i < N; ++i) y[i] = -(ky[i] + cf[i] * q1[i]);
         }
         finish(a.out, y);
-        stiffness<N>(D, lds0, lds1, el, l, act, q0, ga, gb, gc, ky);
+        stiff(q0, ky);
         if constexpr (KIND == K_RES) {
             T gj[N];
             c.ld(a.geo + 3 * len, gj);
@@ -655,14 +817,17 @@ __global__ __launch_bounds__(128 * CPB) void k_minres_upd(MinresUpdArgs<T> a)
 // buffer. In step mode 2 the closing kernel evaluates the recurrence after
 // every step and each launch starts from that state (first = 1); a variant
 // compiled without the in-kernel reduction (fewer registers) was slower.
-template <int N, typename T, int KIND, int CPB, bool PERSIST>
+template <int N, typename T, int KIND, int CPB, bool PERSIST, int MF = 0>
 __global__ __launch_bounds__(64 * CPB * ((KIND == K_RES) ? 2 : 1))
 __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 3 : 1, 8))) void k_minres_fused(MinresFusedArgs<T> a, Dm<N, T> D)
 {
     using S = Shape<N>;
     constexpr int NC = (KIND == K_RES) ? 2 : 1;
     constexpr int NW = NC * CPB;
-    __shared__ T lds[NW][2][S::LDS];
+    // MF: the stiffness contractions on MFMA (1 bf16, 3 split bf16), FP32 only
+    static_assert(MF == 0 || sizeof(T) == 4, "MFMA contractions serve the FP32 step");
+    __shared__ T lds[MF ? 1 : NW][2][MF ? 1 : S::LDS];
+    __shared__ float mfs[MF ? NW : 1][MF ? 64 * kMfPitch : 1];
     __shared__ T xch[NC == 2 ? NW : 1][64 * N];  // component exchange (Res)
     __shared__ double wpart[3][NW];
     __shared__ double red[12];
@@ -773,7 +938,8 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 3 : 1, 8))) void k_minres_fu
             c.ld(a.geo, ga);
             c.ld(a.geo + len, gb);
             c.ld(a.geo + 2 * len, gc);
-            stiffness<N>(D, lds[wv][0], lds[wv][1], el, l, act, qs, ga, gb, gc, z);
+            if constexpr (MF != 0) stiffness_mf<N, MF>(mf_dmat<N>(D, lane), mfs[wv], lane, qs, ga, gb, gc, z);
+            else stiffness<N>(D, lds[wv][0], lds[wv][1], el, l, act, qs, ga, gb, gc, z);
             if constexpr (KIND == K_RES) {
                 T cf[N], gj[N];
                 c.ld(a.cf, cf);
@@ -848,7 +1014,9 @@ __global__ __launch_bounds__(256) void k_minres_fused_close(InstTable it, const 
     const int k = blockIdx.x;
     const int p0 = it.inst_off[k];
     double P[3];
-    group_sum3(sums + p0, nch, it.inst_off[k + 1] - p0, red, 4, P);
+    // every load of an instance of up to 8192 chunks in one round (the order of
+    // the additions does not depend on the load budget)
+    group_sum3<32>(sums + p0, nch, it.inst_off[k + 1] - p0, red, 4, P);
     if (threadIdx.x == 0) {
         MinresState sc = *static_cast<const MinresState*>(in + k);
         minres_fused_prologue(sc, step, P);
@@ -956,6 +1124,16 @@ template <int N, typename T, bool NT>
 static void element_op_NT(int kind, const OpArgs<T>& a, const Dm<N, T>& D, hipStream_t s)
 {
     dim3 g(a.nchunks), b(64);
+    if constexpr (sizeof(T) == 4) {
+        if (a.mf != 0) {
+            if (kind != K_RES) throw std::runtime_error("MFMA contractions serve the stage operator Res");
+            if (a.mf == 1) hipLaunchKernelGGL((k_element_op<N, T, K_RES, NT, 1>), g, b, 0, s, a, D);
+            else hipLaunchKernelGGL((k_element_op<N, T, K_RES, NT, 3>), g, b, 0, s, a, D);
+            return;
+        }
+    } else {
+        if (a.mf != 0) throw std::runtime_error("MFMA contractions serve the FP32 apply");
+    }
     switch (kind) {
     case K_LAPLACE: hipLaunchKernelGGL((k_element_op<N, T, K_LAPLACE, NT>), g, b, 0, s, a, D); break;
     case K_HELMHOLTZ: hipLaunchKernelGGL((k_element_op<N, T, K_HELMHOLTZ, NT>), g, b, 0, s, a, D); break;
@@ -1050,7 +1228,17 @@ static void minres_fused_NC(int kind, const MinresFusedArgs<T>& a, const Dm<N, T
     dim3 g(P ? std::min(grid, items) : items);
     dim3 b2(128 * CPB);
     if (kind != K_RES) throw std::runtime_error("minres: the one-launch step serves the Schrodinger stage operator");
-    SEM_LAUNCH_TIMED((k_minres_fused<N, T, K_RES, CPB, P>), g, b2, s, e0, e1, a, D);
+    if (a.mf == 0) {
+        SEM_LAUNCH_TIMED((k_minres_fused<N, T, K_RES, CPB, P>), g, b2, s, e0, e1, a, D);
+    } else if constexpr (!P) {
+        // the MFMA variants are built for the one-item-per-workgroup shapes
+        if (a.mf == 1) SEM_LAUNCH_TIMED((k_minres_fused<N, T, K_RES, CPB, false, 1>), g, b2, s, e0, e1, a, D);
+        else SEM_LAUNCH_TIMED((k_minres_fused<N, T, K_RES, CPB, false, 3>), g, b2, s, e0, e1, a, D);
+    } else {
+        const dim3 gi(items);
+        if (a.mf == 1) SEM_LAUNCH_TIMED((k_minres_fused<N, T, K_RES, CPB, false, 1>), gi, b2, s, e0, e1, a, D);
+        else SEM_LAUNCH_TIMED((k_minres_fused<N, T, K_RES, CPB, false, 3>), gi, b2, s, e0, e1, a, D);
+    }
 }
 
 template <int N, typename T>

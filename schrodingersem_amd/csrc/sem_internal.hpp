@@ -115,6 +115,7 @@ struct OpArgs {
     T post;        // scale applied to element-interior outputs (ring entries stay raw)
     size_t len;
     int nchunks;
+    int mf;        // FP32 Res only: stiffness contractions on MFMA (0 VALU, 1 bf16, 3 split bf16)
 };
 
 // Device-resident Krylov scalars (FP64), one record per instance.
@@ -243,6 +244,7 @@ struct MinresFusedArgs {
     KrylovScalars* st_out;
     int step;   // i (1-based)
     int first;  // first launch of a batch: the state already holds this step's recurrence
+    int mf;     // stiffness contractions on MFMA (0 VALU, 1 bf16, 3 split bf16)
 };
 
 // Work-item shape of the two Lanczos-step kernels: CPB chunks per workgroup
