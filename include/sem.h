@@ -81,7 +81,7 @@ typedef struct sem_run_stats {
     double max_true_rel_residual; /* max over FP64-refined stage solves of ||b - A x|| / ||b|| */
     long mlp_train_steps;    /* Adam steps taken by the guess networks */
     double mlp_last_loss;    /* last training loss (mean squared error, scaled targets) */
-    long learned_guess_chosen; /* stage solves that started from the network's guess (not the fallback) */
+    long learned_guess_chosen; /* stage solves (of each instance, in a batch) that started from the network's guess */
 } sem_run_stats;
 
 typedef struct sem_mesh sem_mesh;
@@ -213,21 +213,34 @@ int sem_batch_autotune(sem_batch* b, int precision, int steps, int* shapes);
 int sem_batch_set_step_shape(sem_batch* b, int group, int precision, const int* shape);
 /* sem_set_graph_mode on every group's mesh */
 int sem_batch_set_graph_mode(sem_batch* b, int on, int steps);
-/* sem_set_step_mode on every group of the batch (with graph mode on, the
- * graph path's two-launch step is used whatever the mode). */
+/* sem_set_step_mode on every group of the batch */
 int sem_batch_set_step_mode(sem_batch* b, int precision, int mode);
-/* schrodinger_bdf2 (schrodinger.hpp:22-128) on every instance to time T with
- * one set of options (FP64; reference or extrapolated guess). re[k], im[k]
- * (in/out) and V[k] are instance k's element-local arrays. ok[k] (optional):
- * every solve of instance k converged. An instance whose solve fails stops
- * there, as the reference's stepper returns (BDF2 keeps the unconverged
- * iterate and counts the step); the other instances go on to T
- * (sem_batch_times gives each instance's time and step count). */
+/* schrodinger_bdf2 (schrodinger.hpp:22-128), schrodinger_dirk4s3 (:294-423)
+ * and schrodinger_dirk4s5 (:138-285) on every instance to time T with one set
+ * of options (FP64 or mixed precision; reference, extrapolated or learned
+ * guess). re[k], im[k] (in/out) and V[k] are instance k's element-local
+ * arrays. ok[k] (optional): every solve of instance k converged. An instance
+ * whose solve fails stops there, as the reference's stepper returns (BDF2
+ * keeps the unconverged iterate and counts the step; the SDIRK schemes keep
+ * psi of the step's start); the other instances go on to T (sem_batch_times
+ * gives each instance's time and step count). With the reference or
+ * extrapolated guess, in either precision, every instance computes bit for
+ * bit what it computes alone (mixed precision: each instance refines to its
+ * own target). With the learned guess the instances of one degree share one
+ * network, trained on the stages every instance solved: each instance meets
+ * its own stop test, so results agree with single runs to the tolerance. */
 int sem_batch_schrodinger_bdf2(sem_batch* b, double* const* re, double* const* im, const double* const* V, double T,
                                const sem_solver_opts* opts, int* ok);
-/* The same as a session kept on the GPU: start, advance (the while (t < T)
- * loop, also stopping after max_steps steps when max_steps > 0; success: every
- * group converged), read (psi, per-instance success, time). */
+int sem_batch_schrodinger_dirk4s3(sem_batch* b, double* const* re, double* const* im, const double* const* V,
+                                  double T, const sem_solver_opts* opts, int* ok);
+int sem_batch_schrodinger_dirk4s5(sem_batch* b, double* const* re, double* const* im, const double* const* V,
+                                  double T, const sem_solver_opts* opts, int* ok);
+/* The same as a session kept on the GPU: start (scheme 0 bdf2, 1 dirk4s3,
+ * 2 dirk4s5), advance (the while (t < T) loop, also stopping after max_steps
+ * steps when max_steps > 0; success: every group converged), read (psi,
+ * per-instance success, time). */
+int sem_batch_start(sem_batch* b, int scheme, const double* const* re, const double* const* im,
+                    const double* const* V, const sem_solver_opts* opts);
 int sem_batch_start_bdf2(sem_batch* b, const double* const* re, const double* const* im, const double* const* V,
                          const sem_solver_opts* opts);
 int sem_batch_advance(sem_batch* b, double T, long max_steps, int* success);
@@ -368,7 +381,7 @@ int sem_get_step_shape(const sem_mesh* m, int precision, int* shape);
 /* MINRES step mode per precision (SEM_PREC_FP64, or SEM_PREC_MIXED for the
  * FP32 inner solver): 0 two launches per Lanczos step; 1 one fused launch per
  * step that applies the operator before the 1/beta scaling (same recurrence,
- * results equal to rounding; graph mode, when on, takes precedence); 2 the
+ * results equal to rounding); 2 the
  * fused step with the recurrence's reduction of the per-chunk sums moved out
  * of every workgroup into one small kernel per step (same order: results
  * equal mode 1's bit for bit; cheaper on large meshes). Modes 1 and 2 are
@@ -382,15 +395,17 @@ int sem_get_step_mode(const sem_mesh* m, int precision, int* mode);
  * S = (m + |alpha| |k + c|)^(-1/2) from the assembled mass, stiffness and
  * potential-mass diagonals, which takes fewer Lanczos steps; the outer FP64
  * refinement stops on the true residual either way. It applies when the FP32
- * step mode is 1 or 2 and graph mode is off. No counterpart in the reference
+ * step mode is 1 or 2. No counterpart in the reference
  * (its MINRES is unpreconditioned, minres.hpp:19). */
 int sem_set_inner_scaling(sem_mesh* m, int on);
 int sem_get_inner_scaling(const sem_mesh* m, int* on);
 /* Graph mode: with on = 1 the MINRES solver captures `steps` Lanczos steps
- * (a multiple of 3) plus the batch-closing kernel as one HIP graph per solve
- * configuration and replays it between host checks of the stop flags
- * (results are unchanged; kernel timing, when on, uses the direct launches).
- * Off by default for a plain mesh, on for the meshes of a batch. */
+ * (a multiple of 3; the one-launch step of modes 1 and 2 rounds it up to a
+ * multiple of 6 and replays from its second host check on) plus the
+ * batch-closing kernel as one HIP graph per solve configuration and replays
+ * it between host checks of the stop flags (results are unchanged bit for
+ * bit; kernel timing, when on, uses the direct launches). Off by default for
+ * a plain mesh, on for the meshes of a batch. */
 int sem_set_graph_mode(sem_mesh* m, int on, int steps);
 /* Autotuner: times each candidate shape on this mesh over `steps` Lanczos
  * steps of a synthetic solve of operator `kind` (SEM_OP_LAPLACE, _HELMHOLTZ or

@@ -155,9 +155,15 @@ def lib():
             "sem_batch_set_step_shape": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _ip]),
             "sem_batch_schrodinger_bdf2": (C.c_int, [C.c_void_p, C.POINTER(_dp), C.POINTER(_dp), C.POINTER(_dp),
                                                      C.c_double, C.POINTER(_Opts), _ip]),
+            "sem_batch_schrodinger_dirk4s3": (C.c_int, [C.c_void_p, C.POINTER(_dp), C.POINTER(_dp), C.POINTER(_dp),
+                                                     C.c_double, C.POINTER(_Opts), _ip]),
+            "sem_batch_schrodinger_dirk4s5": (C.c_int, [C.c_void_p, C.POINTER(_dp), C.POINTER(_dp), C.POINTER(_dp),
+                                                     C.c_double, C.POINTER(_Opts), _ip]),
             "sem_batch_last_run_stats": (C.c_int, [C.c_void_p, C.POINTER(_Stats), _dp]),
             "sem_batch_start_bdf2": (C.c_int, [C.c_void_p, C.POINTER(_dp), C.POINTER(_dp), C.POINTER(_dp),
                                                C.POINTER(_Opts)]),
+            "sem_batch_start": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(_dp), C.POINTER(_dp), C.POINTER(_dp),
+                                          C.POINTER(_Opts)]),
             "sem_batch_advance": (C.c_int, [C.c_void_p, C.c_double, C.c_long, _ip]),
             "sem_batch_read": (C.c_int, [C.c_void_p, C.POINTER(_dp), C.POINTER(_dp), _ip, _dp]),
             "sem_mlp_create": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_ulonglong, C.POINTER(C.c_void_p)]),
@@ -470,11 +476,10 @@ class Batch:
         _check(lib().sem_batch_set_graph_mode(self._h, int(bool(on)), int(steps)))
 
     def set_step_mode(self, mode, precision=PREC_FP64):
-        """Lanczos step of every group: 0 two launches, 1 one fused launch (graph mode off)."""
+        """Lanczos step of every group: 0 two launches, 1 one fused launch, 2 fused launch plus a closing kernel."""
         _check(lib().sem_batch_set_step_mode(self._h, int(precision), int(mode)))
 
-    def schrodinger_bdf2(self, psi_real, psi_imag, potentials, T, opts):
-        """schrodinger_bdf2 on every instance; psi arrays updated in place; returns per-instance success."""
+    def _run(self, fn, psi_real, psi_imag, potentials, T, opts):
         n = len(self.meshes)
         for k in range(n):
             for a in (psi_real[k], psi_imag[k]):
@@ -487,16 +492,32 @@ class Batch:
         vv = (_dp * n)(*[_ptr(a) for a in V])
         ok = (C.c_int * n)()
         o = opts._c()
-        _check(lib().sem_batch_schrodinger_bdf2(self._h, re, im, vv, float(T), C.byref(o), ok))
+        _check(getattr(lib(), fn)(self._h, re, im, vv, float(T), C.byref(o), ok))
         return [bool(v) for v in ok]
 
-    def start_bdf2(self, psi_real, psi_imag, potentials, opts):
-        """Start a BDF2 session on every instance (state stays on the GPU)."""
+    def schrodinger_bdf2(self, psi_real, psi_imag, potentials, T, opts):
+        """schrodinger_bdf2 on every instance; psi arrays updated in place; returns per-instance success."""
+        return self._run("sem_batch_schrodinger_bdf2", psi_real, psi_imag, potentials, T, opts)
+
+    def schrodinger_dirk4s3(self, psi_real, psi_imag, potentials, T, opts):
+        """schrodinger_dirk4s3 on every instance; psi arrays updated in place; returns per-instance success."""
+        return self._run("sem_batch_schrodinger_dirk4s3", psi_real, psi_imag, potentials, T, opts)
+
+    def schrodinger_dirk4s5(self, psi_real, psi_imag, potentials, T, opts):
+        """schrodinger_dirk4s5 on every instance; psi arrays updated in place; returns per-instance success."""
+        return self._run("sem_batch_schrodinger_dirk4s5", psi_real, psi_imag, potentials, T, opts)
+
+    def start(self, scheme, psi_real, psi_imag, potentials, opts):
+        """Start a session (scheme 0 bdf2, 1 dirk4s3, 2 dirk4s5) on every instance (state stays on the GPU)."""
         n = len(self.meshes)
         arrs = [[_f64(a[k], self.meshes[k].size) for k in range(n)] for a in (psi_real, psi_imag, potentials)]
         ptrs = [(_dp * n)(*[_ptr(a) for a in group]) for group in arrs]
         o = opts._c()
-        _check(lib().sem_batch_start_bdf2(self._h, ptrs[0], ptrs[1], ptrs[2], C.byref(o)))
+        _check(lib().sem_batch_start(self._h, int(scheme), ptrs[0], ptrs[1], ptrs[2], C.byref(o)))
+
+    def start_bdf2(self, psi_real, psi_imag, potentials, opts):
+        """Start a BDF2 session on every instance (state stays on the GPU)."""
+        self.start(0, psi_real, psi_imag, potentials, opts)
 
     def advance(self, T, max_steps=0):
         ok = C.c_int()

@@ -40,13 +40,15 @@ struct Results {
 // Identity of a captured batch of Lanczos steps: every pointer and value the
 // captured launches bake in besides the mesh's own arrays.
 struct GraphKey {
-    std::array<const void*, 9> ptr{};
+    std::array<const void*, 13> ptr{};
     int kind = 0, tsize = 0, steps = 0;
+    int variant = 0;  // 0 two-launch step; fused step: 1 + 2 (step mode 2) + 4 * contraction
     double alpha = 0;
     StepShape shape;
     bool operator==(const GraphKey& o) const
     {
-        return ptr == o.ptr && kind == o.kind && tsize == o.tsize && steps == o.steps && alpha == o.alpha &&
+        return ptr == o.ptr && kind == o.kind && tsize == o.tsize && steps == o.steps && variant == o.variant &&
+               alpha == o.alpha &&
                shape.op_cpb == o.shape.op_cpb && shape.op_grid == o.shape.op_grid &&
                shape.upd_cpb == o.shape.upd_cpb && shape.upd_grid == o.shape.upd_grid;
     }
@@ -61,6 +63,10 @@ struct KrylovOpts {
     // batched meshes: instances (nonzero entries) that take no part in this solve;
     // their Krylov state starts stopped, so x keeps its value there
     const uint8_t* frozen = nullptr;
+    // batched meshes: per-instance relative tolerance and iteration limit
+    // (replacing tol and maxit for instance k; maxit bounds the launches)
+    const double* inst_tol = nullptr;
+    const int* inst_maxit = nullptr;
 };
 
 // Workspace of sem_minres_noalloc: one array per GPU, held until the process
@@ -566,6 +572,39 @@ static double wdot_host(sem_mesh* m, const T* a, const T* b, int ncomp)
     return m->h_sc->red[7];
 }
 
+// the owner-weighted dot of every instance (one entry per instance)
+template <typename T>
+static std::vector<double> wdot_inst(sem_mesh* m, const T* a, const T* b, int ncomp)
+{
+    wdot_dev<T>(m, a, b, ncomp, 7);
+    m->read_scalars();
+    std::vector<double> v(m->n_inst);
+    for (int k = 0; k < m->n_inst; ++k) v[k] = m->h_sc[k].red[7];
+    return v;
+}
+
+// Krylov options that differ from launch_minres_init's defaults, per instance:
+// the stop test phibar < tol * b_ref + atol (minres.hpp:92 has b_ref = ||r0||,
+// atol = tol), b_ref = ||b|| (red[1]) when rel_to_b, instances frozen out of
+// the solve, per-instance tolerances and iteration limits
+static void patch_instances(sem_mesh* m, const KrylovOpts& o)
+{
+    if (!(o.rel_to_b || o.atol != o.tol || o.frozen || o.inst_tol || o.inst_maxit)) return;
+    m->read_scalars();
+    for (int k = 0; k < m->n_inst; ++k) {
+        KrylovScalars& h = m->h_sc[k];
+        if (o.rel_to_b) h.b_norm = std::sqrt(h.red[1]);
+        h.atol = o.atol;
+        if (o.inst_tol) h.tol = o.inst_tol[k];
+        if (o.inst_maxit) {
+            h.maxit = o.inst_maxit[k];
+            h.done = h.maxit < 1 ? 1 : 0;
+        }
+        freeze_instance(h, o.frozen, k);
+    }
+    SEM_HIP(hipMemcpyAsync(m->sc, m->h_sc, m->n_inst * sizeof(KrylovScalars), hipMemcpyHostToDevice, m->stream));
+}
+
 // ---------------------------------------------------------- fused MINRES --
 // minres.hpp:19-107 with one launch per Lanczos step (deferred scaling, see
 // MinresFusedArgs): step i finishes y_{i-1} from A y_{i-2}, applies A to it and
@@ -584,7 +623,7 @@ constexpr int kMaxFirstBatch = 2048;
 static bool sampled_step(int i) { return (((unsigned)i * 2654435761u) >> 20 & 3u) == 0; }
 
 // the one-launch Lanczos step runs for precision index ti (0 FP64, 1 FP32)
-static bool fused_step(const sem_mesh* m, int ti) { return m->step_mode[ti] >= 1 && !m->use_graphs; }
+static bool fused_step(const sem_mesh* m, int ti) { return m->step_mode[ti] >= 1; }
 
 template <typename T>
 static Results minres_fused1(sem_mesh* m, int kind, const T* cf, double alpha, const T* b, T* x, bool x_zero,
@@ -613,15 +652,7 @@ static Results minres_fused1(sem_mesh* m, int kind, const T* cf, double alpha, c
     wdot_dev<T>(m, Y[0], Y[0], nc, 0);
     launch_minres_init(m->sc, ni, 0, o.maxit, o.tol, s);
     if (o.rel_to_b) wdot_dev<T>(m, b, b, nc, 1);
-    if (o.rel_to_b || o.atol != o.tol || o.frozen) {
-        m->read_scalars();
-        for (int k = 0; k < ni; ++k) {
-            if (o.rel_to_b) m->h_sc[k].b_norm = std::sqrt(m->h_sc[k].red[1]);
-            m->h_sc[k].atol = o.atol;
-            freeze_instance(m->h_sc[k], o.frozen, k);
-        }
-        SEM_HIP(hipMemcpyAsync(m->sc, m->h_sc, ni * sizeof(KrylovScalars), hipMemcpyHostToDevice, s));
-    }
+    patch_instances(m, o);
     const int ti = sizeof(T) == 8 ? 0 : 1;
     const StepShape shp = m->shape[ti];
     const int slot = o.hint_slot >= 0 ? std::min(o.hint_slot, 7) : ti;
@@ -629,7 +660,16 @@ static Results minres_fused1(sem_mesh* m, int kind, const T* cf, double alpha, c
     // the stop test trails (iteration k is tested at the start of step k + 2),
     // plus a margin of two: launches past an instance's stop exit after
     // reading its state, which is cheaper than another host check
-    const int first_batch = std::max(m->batch, std::min(m->iter_hint[slot] + 4, kMaxFirstBatch));
+    // Graph mode: after the first (direct) batch, batches of G steps replay as
+    // one captured graph. The step kernel's buffers rotate with period 6 and its
+    // behaviour depends on the step index only below step 4, so a batch of G
+    // (a multiple of 6) steps captured from a step i = 1 (mod 6), i >= 7,
+    // replays every later such batch exactly; the first batch is rounded up to
+    // a multiple of 6 to start the replays there
+    const bool graphs = m->use_graphs && !m->timing;
+    const int G = (m->graph_steps + 5) / 6 * 6;
+    int first_batch = std::max(m->batch, std::min(m->iter_hint[slot] + 4, kMaxFirstBatch));
+    if (graphs) first_batch = (first_batch + 5) / 6 * 6;
     if (m->timing) {
         while ((int)m->ev.size() < 4 * first_batch) {
             hipEvent_t e;
@@ -665,53 +705,87 @@ static Results minres_fused1(sem_mesh* m, int kind, const T* cf, double alpha, c
             if (!m->h_sc[k].success) return false;
         return true;
     };
+    // step i of a batch (b_: its position in the batch, cur: the state buffer
+    // it reads, updated)
+    auto launch_step = [&](int i, int b_, int& cur, bool tim) {
+        MinresFusedArgs<T> a{};
+        a.z_in = Z[(i + 1) % 2];
+        a.z_out = Z[i % 2];
+        a.y2 = Y[(i + 1) % 3];
+        a.y3 = Y[i % 3];
+        a.y_out = Y[(i + 2) % 3];
+        a.wa = W[(i + 2) % 3];
+        a.wb = W[i % 3];
+        a.wn = W[(i + 1) % 3];
+        a.x = x;
+        a.geo = m->geo_t<T>();
+        a.cf = cf;
+        a.alpha = (T)alpha;
+        a.len = m->len();
+        a.nchunks = nch;
+        a.obits = m->obits;
+        a.eside = m->eside;
+        a.ecq = m->ecq;
+        a.c_off = m->c_off;
+        a.c_pos = m->c_pos;
+        a.sums = S[i % 2];
+        a.prev_sums = S[(i + 1) % 2];
+        a.it = itab;
+        if (ext) {
+            a.st_in = m->sc;
+            a.st_out = m->sc_alt;
+        } else {
+            a.st_in = buf[cur];
+            cur ^= 1;
+            a.st_out = buf[cur];
+        }
+        a.step = i;
+        a.first = ext || (b_ == 0);
+        a.mf = sizeof(T) == 4 ? m->contraction : 0;
+        a.dscale = dscale;
+        launch_minres_fused<T>(m->L.N, kind, a, m->dmat<T>(), shp.op_cpb, shp.op_grid, s,
+                               tim ? m->ev[4 * b_] : nullptr, tim ? m->ev[4 * b_ + 1] : nullptr);
+        if (ext) launch_minres_fused_close(itab, m->sc_alt, m->sc, S[i % 2], i + 1, s);
+    };
     int i = 1;
     bool done = o.maxit < 1;
     while (!done) {
+        if (graphs && i > 1) {
+            GraphKey key{};
+            key.ptr = {(const void*)Y[0], (const void*)Y[1], (const void*)Y[2], (const void*)W[0], (const void*)W[1],
+                       (const void*)W[2], (const void*)Z[0], (const void*)Z[1], (const void*)x, (const void*)cf,
+                       (const void*)dscale, (const void*)S[0], (const void*)S[1]};
+            key.kind = kind;
+            key.tsize = (int)sizeof(T);
+            key.alpha = alpha;
+            key.steps = G;
+            key.shape = shp;
+            key.variant = 1 + (ext ? 2 : 0) + 4 * (sizeof(T) == 4 ? m->contraction : 0);
+            hipGraphExec_t ex = m->find_graph(key);
+            if (!ex) {
+                // captured from step 7 (= 1 mod 6); replayed from i
+                hipGraph_t gr = nullptr;
+                SEM_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+                int cur = 0;
+                for (int b_ = 0; b_ < G; ++b_) launch_step(7 + b_, b_, cur, false);
+                if (!ext) launch_minres_fused_close(itab, buf[cur], m->sc, S[(7 + G + 1) % 2], 7 + G, s);
+                SEM_HIP(hipStreamEndCapture(s, &gr));
+                SEM_HIP(hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0));
+                m->add_graph(key, gr, ex);
+            }
+            SEM_HIP(hipGraphLaunch(ex, s));
+            i += G;
+            m->read_scalars();
+            done = all_done() || i > o.maxit + 2;
+            continue;
+        }
         int launched = 0;
         const int i0 = i;
         int cur = 0;
         const int nbatch = (i == 1) ? first_batch : m->batch;
         for (int b_ = 0; b_ < nbatch && i <= o.maxit + 2; ++b_, ++i) {
-            MinresFusedArgs<T> a{};
-            a.z_in = Z[(i + 1) % 2];
-            a.z_out = Z[i % 2];
-            a.y2 = Y[(i + 1) % 3];
-            a.y3 = Y[i % 3];
-            a.y_out = Y[(i + 2) % 3];
-            a.wa = W[(i + 2) % 3];
-            a.wb = W[i % 3];
-            a.wn = W[(i + 1) % 3];
-            a.x = x;
-            a.geo = m->geo_t<T>();
-            a.cf = cf;
-            a.alpha = (T)alpha;
-            a.len = m->len();
-            a.nchunks = nch;
-            a.obits = m->obits;
-            a.eside = m->eside;
-            a.ecq = m->ecq;
-            a.c_off = m->c_off;
-            a.c_pos = m->c_pos;
-            a.sums = S[i % 2];
-            a.prev_sums = S[(i + 1) % 2];
-            a.it = itab;
-            if (ext) {
-                a.st_in = m->sc;
-                a.st_out = m->sc_alt;
-            } else {
-                a.st_in = buf[cur];
-                cur ^= 1;
-                a.st_out = buf[cur];
-            }
-            a.step = i;
-            a.first = ext || (b_ == 0);
-            a.mf = sizeof(T) == 4 ? m->contraction : 0;
-            a.dscale = dscale;
             const bool tim = m->timing == 2 || (m->timing == 1 && sampled_step(i));
-            launch_minres_fused<T>(m->L.N, kind, a, m->dmat<T>(), shp.op_cpb, shp.op_grid, s,
-                                   tim ? m->ev[4 * b_] : nullptr, tim ? m->ev[4 * b_ + 1] : nullptr);
-            if (ext) launch_minres_fused_close(itab, m->sc_alt, m->sc, S[i % 2], i + 1, s);
+            launch_step(i, b_, cur, tim);
             ++launched;
         }
         // the batch ended after step i-1; close with what step i would compute first
@@ -748,9 +822,10 @@ static Results minres_fused1(sem_mesh* m, int kind, const T* cf, double alpha, c
     for (int k = 0; k < ni; ++k) {
         Results& r = m->inst_results[k];
         const KrylovScalars& h = m->h_sc[k];
+        const int mk = o.inst_maxit ? o.inst_maxit[k] : o.maxit;
         r.success = h.success;
-        r.n_iter = r.success ? h.iter : o.maxit + 1;
-        if (o.maxit < 1) r.n_iter = 1;
+        r.n_iter = r.success ? h.iter : mk + 1;
+        if (mk < 1) r.n_iter = 1;
         r.residual = h.phibar;
     }
     // the next solve's first batch follows this one's count, if it converged
@@ -804,16 +879,7 @@ static Results minres_fused(sem_mesh* m, int kind, const T* cf, double alpha, co
     if (o.rel_to_b) {  // b_norm = ||b|| (fair stop test for nonzero guesses)
         wdot_dev<T>(m, b, b, nc, 1);
     }
-    if (o.rel_to_b || o.atol != o.tol || o.frozen) {
-        // stop test phibar < tol * b_ref + atol (minres.hpp:92 has b_ref = ||r0||, atol = tol)
-        m->read_scalars();
-        for (int k = 0; k < ni; ++k) {
-            if (o.rel_to_b) m->h_sc[k].b_norm = std::sqrt(m->h_sc[k].red[1]);
-            m->h_sc[k].atol = o.atol;
-            freeze_instance(m->h_sc[k], o.frozen, k);
-        }
-        SEM_HIP(hipMemcpyAsync(m->sc, m->h_sc, ni * sizeof(KrylovScalars), hipMemcpyHostToDevice, s));
-    }
+    patch_instances(m, o);
     const int nch = m->L.nchunks;
     const int ti = sizeof(T) == 8 ? 0 : 1;
     const StepShape shp = m->shape[ti];
@@ -986,9 +1052,10 @@ static Results minres_fused(sem_mesh* m, int kind, const T* cf, double alpha, co
     for (int k = 0; k < ni; ++k) {
         Results& r = m->inst_results[k];
         const KrylovScalars& h = m->h_sc[k];
+        const int mk = o.inst_maxit ? o.inst_maxit[k] : o.maxit;
         r.success = h.success;
-        r.n_iter = r.success ? h.iter : o.maxit + 1;
-        if (o.maxit < 1) r.n_iter = 1;
+        r.n_iter = r.success ? h.iter : mk + 1;
+        if (mk < 1) r.n_iter = 1;
         r.residual = h.phibar;
     }
     // the next solve's first batch follows this one's count, if it converged
@@ -1202,39 +1269,94 @@ static Results solve_mixed(sem_mesh* m, int kind, const double* cf64, const floa
         lincomb<double>(m, r, 0, {{1.0, b}, {-1.0, Ax}}, n);
     }
     const float* S = (kind == K_RES && m->inner_scale && fused_step(m, 1)) ? inner_scale(m, cf64, alpha) : nullptr;
-    double r0 = std::sqrt(wdot_host<double>(m, r, r, nc));
-    const double bnorm = std::sqrt(wdot_host<double>(m, b, b, nc));
-    const double ref = o.rel_to_b ? bnorm : r0;
-    const double target = o.tol * ref + o.atol;
-    Results res;
-    res.residual = r0;
-    long inner_total = 0;
-    int sweeps = 0;
-    double rn = r0;
-    while (!(rn < target) && inner_total < o.maxit && sweeps < 50) {
+    // every instance of a batched mesh refines on its own: its own norms,
+    // target, sweeps and inner iteration budget; an instance that has met its
+    // target (or run out) sits out the later sweeps, its x untouched
+    const int ni = m->n_inst;
+    std::vector<double> rn = wdot_inst<double>(m, r, r, nc), bnorm = wdot_inst<double>(m, b, b, nc);
+    std::vector<double> target(ni), itol(ni);
+    std::vector<long> inner(ni, 0);
+    std::vector<int> sweeps(ni, 0), imax(ni, 0);
+    std::vector<uint8_t> idle(ni, 0);
+    for (int k = 0; k < ni; ++k) {
+        rn[k] = std::sqrt(rn[k]);
+        bnorm[k] = std::sqrt(bnorm[k]);
+        target[k] = o.tol * (o.rel_to_b ? bnorm[k] : rn[k]) + o.atol;
+    }
+    auto refresh = [&]() {
+        bool any = false;
+        for (int k = 0; k < ni; ++k) {
+            idle[k] = (o.frozen && o.frozen[k]) || rn[k] < target[k] || inner[k] >= o.maxit || sweeps[k] >= 50;
+            any |= !idle[k];
+        }
+        return any;
+    };
+    const size_t cl = (size_t)m->L.K * m->L.N * m->L.N;
+    while (refresh()) {
         if (S) launch_scale_cast<float, double>(r32, r, S, m->len(), nc, s);
         else launch_cast<float, double>(r32, r, n, s);
         KrylovOpts io;
-        io.maxit = (int)std::max<long>(1, o.maxit - inner_total);
-        // relative reduction of this sweep; never ask FP32 for more than it can give
-        io.tol = std::max(inner_tol, 0.5 * target / std::max(rn, 1e-300));
+        io.maxit = 1;
+        bool all_active = true;
+        for (int k = 0; k < ni; ++k) {
+            all_active &= !idle[k];
+            imax[k] = (int)std::max<long>(1, o.maxit - inner[k]);
+            // relative reduction of this sweep; never ask FP32 for more than it can give
+            itol[k] = std::max(inner_tol, 0.5 * target[k] / std::max(rn[k], 1e-300));
+            if (!idle[k]) io.maxit = std::max(io.maxit, imax[k]);
+        }
+        io.tol = itol[0];
         io.atol = 0;
-        io.hint_slot = 2 + std::min(sweeps, 5);
-        Results ir = minres_fused<float>(m, kind, cf32, alpha, r32, d32, true, io, S);
-        inner_total += ir.n_iter;
+        if (ni > 1) {
+            io.frozen = idle.data();
+            io.inst_tol = itol.data();
+            io.inst_maxit = imax.data();
+        }
+        int sw = 0;
+        for (int k = 0; k < ni; ++k)
+            if (!idle[k]) sw = std::max(sw, sweeps[k]);
+        io.hint_slot = 2 + std::min(sw, 5);
+        minres_fused<float>(m, kind, cf32, alpha, r32, d32, true, io, S);
         if (S) launch_scale_cast<double, float>(Ax, d32, S, m->len(), nc, s);
         else launch_cast<double, float>(Ax, d32, n, s);
-        lincomb<double>(m, x, 2, {{1.0, Ax}}, n);  // x += d
+        if (all_active) {
+            lincomb<double>(m, x, 2, {{1.0, Ax}}, n);  // x += d
+        } else {
+            for (int k = 0; k < ni; ++k) {  // x += d on the instances of this sweep
+                if (idle[k]) continue;
+                const size_t off = (size_t)m->h_inst_off[k] * cl;
+                const size_t cnt = (size_t)(m->h_inst_off[k + 1] - m->h_inst_off[k]) * cl;
+                for (int c = 0; c < nc; ++c)
+                    lincomb<double>(m, x + c * m->len() + off, 2, {{1.0, Ax + c * m->len() + off}}, cnt);
+            }
+        }
         apply_op<double>(m, kind, x, Ax, cf64, alpha);
         lincomb<double>(m, r, 0, {{1.0, b}, {-1.0, Ax}}, n);
-        rn = std::sqrt(wdot_host<double>(m, r, r, nc));
-        ++sweeps;
+        const std::vector<double> r2 = wdot_inst<double>(m, r, r, nc);
+        for (int k = 0; k < ni; ++k) {
+            if (idle[k]) continue;
+            inner[k] += m->inst_results[k].n_iter;
+            rn[k] = std::sqrt(r2[k]);
+            ++sweeps[k];
+        }
         m->stats.refinements++;
     }
-    res.success = rn < target;
-    res.n_iter = (int)inner_total;
-    res.residual = rn;
-    m->last_true_rel = rn / std::max(bnorm, 1e-300);
+    // aggregate over the instances of the solve: all succeeded, the most inner
+    // iterations, the largest residual
+    Results res;
+    res.success = 1;
+    m->last_true_rel = 0;
+    for (int k = 0; k < ni; ++k) {
+        Results& rk = m->inst_results[k];
+        rk.success = rn[k] < target[k];
+        rk.n_iter = (int)inner[k];
+        rk.residual = rn[k];
+        if (o.frozen && o.frozen[k]) continue;
+        res.success = res.success && rk.success;
+        res.n_iter = std::max(res.n_iter, rk.n_iter);
+        res.residual = std::max(res.residual, rk.residual);
+        m->last_true_rel = std::max(m->last_true_rel, rn[k] / std::max(bnorm[k], 1e-300));
+    }
     return res;
 }
 
@@ -1303,7 +1425,7 @@ struct Stepper {
     std::vector<double*> z, zprev, zprev2;
     bool have_prev = false, have_prev2 = false;
     double t = 0;
-    double pnorm = 0;          // dirk4s5 (schrodinger.hpp:214)
+    std::vector<double> pnorm; // dirk4s5 (schrodinger.hpp:214), per instance
     double* ps[2] = {nullptr, nullptr};  // bdf2 history (schrodinger.hpp:85)
     int nps = 0;
     double alpha_bdf = 0;
@@ -1387,10 +1509,12 @@ static Results solve_res(Stepper& st, double alpha, const double* b, double* x, 
     return r;
 }
 
-// root mean square of a device vector (all entries of ncomp components)
+// root mean square of a device vector (all entries of ncomp components, every
+// instance of a batched mesh)
 static double rms_dev(sem_mesh* m, const double* a, int ncomp)
 {
-    const double s = wdot_host<double>(m, a, a, ncomp);
+    double s = 0;
+    for (double v : wdot_inst<double>(m, a, a, ncomp)) s += v;
     return std::sqrt(s / std::max(1.0, (double)ncomp * m->hm.dof));
 }
 
@@ -1477,8 +1601,25 @@ static Results stage_solve(Stepper& st, double alpha, const double* b, double* z
             lincomb<double>(m, rz, 0, {{1.0, b}, {-1.0, Az}}, st.n);
             wdot_dev<double>(m, rz, rz, 2, 3);
             m->read_scalars();
-            if (m->h_sc[0].red[3] < m->h_sc[0].red[2]) launch_cast<double, double>(z, st.ext, st.n, m->stream);
-            else st.stats.learned_guess_chosen++;
+            if (!st.multi()) {
+                if (m->h_sc[0].red[3] < m->h_sc[0].red[2]) launch_cast<double, double>(z, st.ext, st.n, m->stream);
+                else st.stats.learned_guess_chosen++;
+            } else {
+                // each instance keeps the guess with its own smaller residual
+                const size_t cl = (size_t)m->L.K * m->L.N * m->L.N;
+                for (int k = 0; k < m->n_inst; ++k) {
+                    if (st.skip[k]) continue;
+                    if (!(m->h_sc[k].red[3] < m->h_sc[k].red[2])) {
+                        st.stats.learned_guess_chosen++;
+                        continue;
+                    }
+                    const size_t off = (size_t)m->h_inst_off[k] * cl;
+                    const size_t cnt = (size_t)(m->h_inst_off[k + 1] - m->h_inst_off[k]) * cl;
+                    for (int c = 0; c < 2; ++c)
+                        launch_cast<double, double>(z + c * m->len() + off, st.ext + c * m->len() + off, cnt,
+                                                    m->stream);
+                }
+            }
             }
         } else {
             launch_cast<double, double>(z, st.ext, st.n, m->stream);
@@ -1491,8 +1632,15 @@ static Results stage_solve(Stepper& st, double alpha, const double* b, double* z
             launch_cast<double, double>(z, st.zprev[stage], st.n, m->stream);
         zero = false;
     }
+    // a batch trains its one network only on stages that every instance solved
+    // (an instance that has stopped holds no solution to learn from)
+    bool whole = true;
+    if (st.multi())
+        for (int k = 0; k < m->n_inst; ++k) whole = whole && !st.skip[k];
     Results r = solve_res(st, alpha, b, z, zero, !zero);
-    if (training && r.success) {
+    if (st.multi())
+        for (int k = 0; k < m->n_inst; ++k) whole = whole && !st.skip[k];
+    if (training && r.success && whole) {
         // train on this stage's solution: target = (z - extrapolation) / s_out
         if (!st.have_out[stage]) {
             lincomb<double>(m, st.tmp, 0, {{1.0, z}, {-1.0, st.ext}}, st.n);
@@ -1572,8 +1720,6 @@ static Stepper* stepper_create(sem_mesh* m, int scheme, const double* re, const 
     if (o->guess < 0 || o->guess > 2) throw std::invalid_argument("schrodinger: unsupported guess mode");
     if (o->guess == 2 && scheme == 0) throw std::invalid_argument("schrodinger: the learned guess serves the SDIRK stages");
     if (o->precision < 0 || o->precision > 1) throw std::invalid_argument("schrodinger: unsupported precision");
-    if (m->n_inst > 1 && (o->precision != SEM_PREC_FP64 || o->guess == 2))
-        throw std::invalid_argument("schrodinger: batched meshes step in FP64 with the reference or extrapolated guess");
     std::unique_ptr<Stepper> st(new Stepper);
     st->m = m;
     st->scheme = scheme;
@@ -1622,7 +1768,10 @@ static Stepper* stepper_create(sem_mesh* m, int scheme, const double* re, const 
     m->upload(st->p, h.data(), 2);
     // psi arrives unmasked; unmask copies the owner values, the device form is consistent
     dss<double>(m, DSS_UNMASK, st->p, 2);
-    if (scheme == 2) st->pnorm = std::sqrt(wdot_host<double>(m, st->p, st->p, 2));
+    if (scheme == 2) {
+        st->pnorm = wdot_inst<double>(m, st->p, st->p, 2);
+        for (double& v : st->pnorm) v = std::sqrt(v);
+    }
     return st.release();
 }
 
@@ -1694,7 +1843,12 @@ static bool step_dirk4s5(Stepper& st)
                     {{25.0 / 24.0, z[0]}, {-49.0 / 48.0, z[1]}, {125.0 / 16.0, z[2]}, {-85.0 / 12.0, z[3]}, {0.25, z[4]}},
                     n);
     lincomb<double>(m, q, 1, {{-3.0 / 16.0, z[0]}, {-27.0 / 32.0, z[1]}, {25.0 / 32.0, z[2]}, {0.25, z[4]}}, n);
-    st.stats.last_error_estimate = std::sqrt(wdot_host<double>(m, q, q, 2)) / st.pnorm;
+    // the error estimate of every instance that completed the step; a batch reports the largest
+    const std::vector<double> eq = wdot_inst<double>(m, q, q, 2);
+    double est = 0;
+    for (int k = 0; k < m->n_inst; ++k)
+        if (!st.multi() || !st.skip[k]) est = std::max(est, std::sqrt(eq[k]) / st.pnorm[k]);
+    st.stats.last_error_estimate = est;
     record_history(st, 5);
     return true;
 }
@@ -1837,7 +1991,7 @@ static void stepper_save(Stepper& st, double* out)
     h[6] = st.stats.steps;
     h[7] = st.nps;
     h[8] = st.alpha_bdf;
-    h[9] = st.pnorm;
+    h[9] = st.pnorm.empty() ? 0.0 : st.pnorm[0];  // sessions on plain meshes (one instance)
     h[10] = st.have_prev;
     h[11] = st.have_prev2;
     h[12] = (double)st.zprev.size();
@@ -1898,7 +2052,7 @@ static void stepper_restore(Stepper& st, const double* in, size_t n)
     st.stats.steps = (int)h[6];
     st.nps = (int)h[7];
     st.alpha_bdf = h[8];
-    st.pnorm = h[9];
+    st.pnorm.assign(m->n_inst, h[9]);
     const double* s = h + kCkptHeader;
     const double* d = s + kCkptScales;
     m->upload(st.p, d, 2);
@@ -1998,7 +2152,7 @@ struct sem_batch {
     std::vector<Group> groups;
     std::vector<int> inst_group, inst_slot;
     std::vector<long> inst_ref_len;  // E * N * N of each instance
-    std::vector<std::unique_ptr<sem::Stepper>> session;  // one BDF2 session per group
+    std::vector<std::unique_ptr<sem::Stepper>> session;  // one stepper session per group
     sem_run_stats stats{};
     double seconds = 0;
     ~sem_batch()
@@ -2026,8 +2180,9 @@ static void unpack_union(const sem_mesh* um, int slot, const std::vector<double>
         std::copy(src.begin() + c * rl + e0 * NN, src.begin() + c * rl + (e0 + ek) * NN, dst + c * ek * NN);
 }
 
-// schrodinger_bdf2 (schrodinger.hpp:22-128) session on every instance of one group
-static Stepper* batch_group_start(sem_batch::Group& g, const double* const* re, const double* const* im,
+// a stepper session (scheme 0 schrodinger_bdf2, schrodinger.hpp:22-128;
+// 1 dirk4s3, :294-423; 2 dirk4s5, :138-285) on every instance of one group
+static Stepper* batch_group_start(sem_batch::Group& g, int scheme, const double* const* re, const double* const* im,
                                   const double* const* V, const sem_solver_opts* o)
 {
     sem_mesh* um = g.m.get();
@@ -2039,7 +2194,7 @@ static Stepper* batch_group_start(sem_batch::Group& g, const double* const* re, 
         pack_union(um, (int)s, im[k], ui, 1);
         pack_union(um, (int)s, V[k], uv, 1);
     }
-    Stepper* st = stepper_create(um, 0, ur.data(), ui.data(), uv.data(), o);
+    Stepper* st = stepper_create(um, scheme, ur.data(), ui.data(), uv.data(), o);
     st->inst_ok.assign(um->n_inst, 1);
     return st;
 }
@@ -2785,19 +2940,26 @@ int sem_batch_set_step_shape(sem_batch* b, int group, int precision, const int* 
     return sem_set_step_shape(b->groups[group].m.get(), precision, shape);
 }
 
-int sem_batch_start_bdf2(sem_batch* b, const double* const* re, const double* const* im, const double* const* V,
-                         const sem_solver_opts* opts)
+int sem_batch_start(sem_batch* b, int scheme, const double* const* re, const double* const* im,
+                    const double* const* V, const sem_solver_opts* opts)
 {
     return guarded([&] {
-        if (!b || !re || !im || !V || !opts) throw std::invalid_argument("sem_batch_start_bdf2: null argument");
+        if (!b || !re || !im || !V || !opts) throw std::invalid_argument("sem_batch_start: null argument");
+        if (scheme < 0 || scheme > 2) throw std::invalid_argument("sem_batch_start: unknown scheme");
         for (size_t k = 0; k < b->inst_group.size(); ++k)
-            if (!re[k] || !im[k] || !V[k]) throw std::invalid_argument("sem_batch_start_bdf2: null instance array");
+            if (!re[k] || !im[k] || !V[k]) throw std::invalid_argument("sem_batch_start: null instance array");
         b->session.clear();
-        for (auto& g : b->groups) b->session.emplace_back(batch_group_start(g, re, im, V, opts));
+        for (auto& g : b->groups) b->session.emplace_back(batch_group_start(g, scheme, re, im, V, opts));
         b->stats = sem_run_stats{};
         b->seconds = 0;
         return (int)SEM_OK;
     });
+}
+
+int sem_batch_start_bdf2(sem_batch* b, const double* const* re, const double* const* im, const double* const* V,
+                         const sem_solver_opts* opts)
+{
+    return sem_batch_start(b, 0, re, im, V, opts);
 }
 
 int sem_batch_advance(sem_batch* b, double T, long max_steps, int* success)
@@ -2823,9 +2985,16 @@ int sem_batch_advance(sem_batch* b, double T, long max_steps, int* success)
             if (e) std::rethrow_exception(e);
         sem_run_stats tot{};
         for (const auto& st : b->session) {
-            tot.steps = std::max(tot.steps, st->stats.steps);
-            tot.stage_solves += st->stats.stage_solves;
-            tot.total_iters += st->stats.total_iters;
+            const sem_run_stats& g = st->stats;
+            tot.steps = std::max(tot.steps, g.steps);
+            tot.stage_solves += g.stage_solves;
+            tot.total_iters += g.total_iters;
+            tot.refinements += g.refinements;
+            tot.last_error_estimate = std::max(tot.last_error_estimate, g.last_error_estimate);
+            tot.max_true_rel_residual = std::max(tot.max_true_rel_residual, g.max_true_rel_residual);
+            tot.mlp_train_steps += g.mlp_train_steps;
+            tot.mlp_last_loss = std::max(tot.mlp_last_loss, g.mlp_last_loss);
+            tot.learned_guess_chosen += g.learned_guess_chosen;
         }
         b->stats = tot;
         if (success) {
@@ -2861,14 +3030,32 @@ int sem_batch_times(const sem_batch* b, double* t, int* steps)
     return SEM_OK;
 }
 
-int sem_batch_schrodinger_bdf2(sem_batch* b, double* const* re, double* const* im, const double* const* V, double T,
-                               const sem_solver_opts* opts, int* ok)
+static int batch_run(sem_batch* b, int scheme, double* const* re, double* const* im, const double* const* V, double T,
+                     const sem_solver_opts* opts, int* ok)
 {
-    int rc = sem_batch_start_bdf2(b, re, im, V, opts);
+    int rc = sem_batch_start(b, scheme, re, im, V, opts);
     if (rc == SEM_OK) rc = sem_batch_advance(b, T, 0, nullptr);
     if (rc == SEM_OK) rc = sem_batch_read(b, re, im, ok, nullptr);
     if (b) b->session.clear();
     return rc;
+}
+
+int sem_batch_schrodinger_bdf2(sem_batch* b, double* const* re, double* const* im, const double* const* V, double T,
+                               const sem_solver_opts* opts, int* ok)
+{
+    return batch_run(b, 0, re, im, V, T, opts, ok);
+}
+
+int sem_batch_schrodinger_dirk4s3(sem_batch* b, double* const* re, double* const* im, const double* const* V,
+                                  double T, const sem_solver_opts* opts, int* ok)
+{
+    return batch_run(b, 1, re, im, V, T, opts, ok);
+}
+
+int sem_batch_schrodinger_dirk4s5(sem_batch* b, double* const* re, double* const* im, const double* const* V,
+                                  double T, const sem_solver_opts* opts, int* ok)
+{
+    return batch_run(b, 2, re, im, V, T, opts, ok);
 }
 
 int sem_batch_last_run_stats(const sem_batch* b, sem_run_stats* out, double* seconds)
