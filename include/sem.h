@@ -176,6 +176,12 @@ int sem_stepper_read(sem_stepper* s, double* re, double* im, double* t);
 int sem_stepper_stats(const sem_stepper* s, sem_run_stats* out);
 /* learned guess: Adam steps per stage solve while training (default 10) and learning rate (default 3e-3) */
 int sem_stepper_set_learning(sem_stepper* s, int epochs, float lr);
+/* learned guess, before its first stage solve: the training loss (0: mean
+ * squared error of the scaled correction z - ext; 1: the assembled residual
+ * ||Res (g - z)||^2 of the guess g, relative to the extrapolation's, the norm
+ * the guard compares and MINRES reduces) and the fourth input block (0: the
+ * stage's right-hand side b; 1: the extrapolation's residual b - Res ext) */
+int sem_stepper_set_training(sem_stepper* s, int loss, int inputs);
 /* learned guess: on (default) starts each stage solve from the network's guess
  * only when its true FP64 residual is below the extrapolation's (two extra
  * operator applies per stage solve); off always starts from the network's guess */

@@ -127,6 +127,7 @@ def lib():
             "sem_stepper_destroy": (None, [C.c_void_p]),
             "sem_stepper_set_learning": (C.c_int, [C.c_void_p, C.c_int, C.c_float]),
             "sem_stepper_set_guard": (C.c_int, [C.c_void_p, C.c_int]),
+            "sem_stepper_set_training": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
             "sem_element_apply_fp32": (C.c_int, [C.c_void_p, C.c_int, _dp, C.c_double, _dp, _dp]),
             "sem_set_contraction": (C.c_int, [C.c_void_p, C.c_int]),
             "sem_get_contraction": (C.c_int, [C.c_void_p, _ip]),
@@ -604,6 +605,12 @@ class Stepper:
         s = _Stats()
         _check(lib().sem_stepper_stats(self._h, C.byref(s)))
         return {k: getattr(s, k) for k, _ in _Stats._fields_}
+
+    def set_training(self, loss, inputs):
+        """Learned guess, before its first stage solve: loss 0 = squared error of
+        the correction, 1 = assembled residual; inputs 0 = right-hand side,
+        1 = the extrapolation's residual."""
+        _check(lib().sem_stepper_set_training(self._h, int(loss), int(inputs)))
 
     def set_guard(self, on):
         """Learned guess: keep the extrapolation when its true residual is smaller (default on)."""

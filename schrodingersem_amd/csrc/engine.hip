@@ -1445,6 +1445,9 @@ struct Stepper {
     // by one-hot input columns
     std::unique_ptr<Mlp> mlp;
     bool guard = true;  // keep the extrapolation when its residual is smaller
+    bool residual_loss = false;  // train on ||Res (g - z)|| instead of the correction's squared error
+    bool feat_rext = false;      // fourth input: b - Res ext instead of b
+    double* rext = nullptr;      // b - Res ext of the current stage
     double* Vraw = nullptr;
     double* ext = nullptr;
     double* tmp = nullptr;
@@ -1573,16 +1576,22 @@ static Results stage_solve(Stepper& st, double alpha, const double* b, double* z
     if (learned) {
         // extrapolation plus the network's correction, made consistent
         lincomb<double>(m, st.ext, 1, {{2.0, st.zprev[stage]}, {-1.0, st.zprev2[stage]}}, st.n);
+        // the extrapolation's residual b - Res ext (an input of the network, the
+        // guard's fallback residual and the residual loss's normaliser)
+        double* Az = m->buf<double>("guard_Az", st.n);
+        apply_op<double>(m, K_RES, st.ext, Az, st.V.c64, alpha);
+        lincomb<double>(m, st.rext, 0, {{1.0, b}, {-1.0, Az}}, st.n);
+        const double* fin = st.feat_rext ? st.rext : b;  // the network's fourth input
         if (!st.have_scale[stage]) {
             st.s_feat[stage][0] = 1.0 / std::max(rms_dev(m, st.zprev[stage], 2), 1e-300);
             st.s_feat[stage][1] = 1.0 / std::max(rms_dev(m, st.Vraw, 1), 1e-300);
-            st.s_feat[stage][2] = 1.0 / std::max(rms_dev(m, b, 2), 1e-300);
+            st.s_feat[stage][2] = 1.0 / std::max(rms_dev(m, fin, 2), 1e-300);
             st.have_scale[stage] = true;
         }
         if (st.have_out[stage] && !training) {
             Mlp& net = *st.mlp;
             net.reserve(m->L.E);
-            mlp_features(st, stage, b);
+            mlp_features(st, stage, fin);
             net.forward(m->L.E);
             launch_scatter_output(scatter_args(st, stage, z, st.ext, st.s_out[stage]), m->stream);
             dss<double>(m, DSS_UNMASK, z, 2);
@@ -1592,14 +1601,11 @@ static Results stage_solve(Stepper& st, double alpha, const double* b, double* z
             if (!st.guard) {
                 st.stats.learned_guess_chosen++;
             } else {
-            double* Az = m->buf<double>("guard_Az", st.n);
             double* rz = m->buf<double>("guard_r", st.n);
             apply_op<double>(m, K_RES, z, Az, st.V.c64, alpha);
             lincomb<double>(m, rz, 0, {{1.0, b}, {-1.0, Az}}, st.n);
             wdot_dev<double>(m, rz, rz, 2, 2);
-            apply_op<double>(m, K_RES, st.ext, Az, st.V.c64, alpha);
-            lincomb<double>(m, rz, 0, {{1.0, b}, {-1.0, Az}}, st.n);
-            wdot_dev<double>(m, rz, rz, 2, 3);
+            wdot_dev<double>(m, st.rext, st.rext, 2, 3);
             m->read_scalars();
             if (!st.multi()) {
                 if (m->h_sc[0].red[3] < m->h_sc[0].red[2]) launch_cast<double, double>(z, st.ext, st.n, m->stream);
@@ -1649,14 +1655,52 @@ static Results stage_solve(Stepper& st, double alpha, const double* b, double* z
         }
         Mlp& net = *st.mlp;
         net.reserve(m->L.E);
-        mlp_features(st, stage, b);
-        launch_gather_target(scatter_args(st, stage, nullptr, st.ext, 1.0 / st.s_out[stage]), net.T, z, m->stream);
-        for (int ep = 0; ep < st.epochs; ++ep) {
-            const bool last = ep + 1 == st.epochs;
-            const double loss = net.backward(m->L.E, last);
-            net.adam_step();
-            st.stats.mlp_train_steps++;
-            if (last) st.stats.mlp_last_loss = loss;
+        mlp_features(st, stage, st.feat_rext ? st.rext : b);
+        if (!st.residual_loss) {
+            // mean squared error of the scaled correction (z - ext) / s_out
+            launch_gather_target(scatter_args(st, stage, nullptr, st.ext, 1.0 / st.s_out[stage]), net.T, z,
+                                 m->stream);
+            for (int ep = 0; ep < st.epochs; ++ep) {
+                const bool last = ep + 1 == st.epochs;
+                const double loss = net.backward(m->L.E, last);
+                net.adam_step();
+                st.stats.mlp_train_steps++;
+                if (last) st.stats.mlp_last_loss = loss;
+            }
+        } else {
+            // residual loss ||Res (g - z)||^2 / ||b - Res ext||^2 of the assembled
+            // guess g (the owner copy of each node, Dirichlet entries zero): the
+            // norm the guard compares and MINRES reduces. Its gradient with
+            // respect to the owner copy of a node is 2 (Res Res (g - z)) there
+            // (Res is symmetric), zero on the other copies and on the boundary.
+            double* g = m->buf<double>("rl_g", st.n);
+            double* rr = m->buf<double>("rl_r", st.n);
+            double* gr = m->buf<double>("rl_gr", st.n);
+            double n2 = 0;
+            for (double v : wdot_inst<double>(m, st.rext, st.rext, 2)) n2 += v;
+            n2 = std::max(n2, 1e-300);
+            for (int ep = 0; ep < st.epochs; ++ep) {
+                const bool last = ep + 1 == st.epochs;
+                net.forward(m->L.E);
+                launch_scatter_output(scatter_args(st, stage, g, st.ext, st.s_out[stage]), m->stream);
+                dss<double>(m, DSS_UNMASK, g, 2);
+                dss<double>(m, DSS_ZERO_BND, g, 2);
+                lincomb<double>(m, g, 2, {{-1.0, z}}, st.n);  // g - z
+                apply_op<double>(m, K_RES, g, rr, st.V.c64, alpha);
+                apply_op<double>(m, K_RES, rr, gr, st.V.c64, alpha);
+                dss<double>(m, DSS_MASK, gr, 2);
+                dss<double>(m, DSS_ZERO_BND, gr, 2);
+                launch_gather_target(scatter_args(st, stage, nullptr, nullptr, 2.0 * st.s_out[stage] / n2), net.dY, gr,
+                                     m->stream);
+                net.backprop(m->L.E);
+                net.adam_step();
+                st.stats.mlp_train_steps++;
+                if (last) {
+                    double l = 0;
+                    for (double v : wdot_inst<double>(m, rr, rr, 2)) l += v;
+                    st.stats.mlp_last_loss = l / n2;
+                }
+            }
         }
     }
     return r;
@@ -1753,6 +1797,7 @@ static Stepper* stepper_create(sem_mesh* m, int scheme, const double* re, const 
         st->learn_steps = o->learn_steps > 0 ? o->learn_steps : 50;
         st->ext = st->alloc();
         st->tmp = st->alloc();
+        st->rext = st->alloc();
         st->Vraw = st->alloc();
         m->upload(st->Vraw, V, 1);
     }
@@ -2806,6 +2851,23 @@ int sem_stepper_set_learning(sem_stepper* h, int epochs, float lr)
     }
     h->st->epochs = epochs;
     h->st->mlp->lr = lr;
+    return SEM_OK;
+}
+
+int sem_stepper_set_training(sem_stepper* h, int loss, int inputs)
+{
+    if (!h || loss < 0 || loss > 1 || inputs < 0 || inputs > 1) return SEM_EINVAL;
+    if (!h->st->mlp) {
+        set_error("sem_stepper_set_training: the stepper was not created with the learned guess");
+        return SEM_EINVAL;
+    }
+    for (bool v : h->st->have_scale)
+        if (v) {
+            set_error("sem_stepper_set_training: the network has started training");
+            return SEM_EINVAL;
+        }
+    h->st->residual_loss = loss == 1;
+    h->st->feat_rext = inputs == 1;
     return SEM_OK;
 }
 

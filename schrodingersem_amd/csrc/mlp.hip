@@ -415,6 +415,16 @@ double Mlp::backward(int rows, bool host_loss)
     const int nb = std::min(grid_for((size_t)rows * out_pad), 4096);
     hipLaunchKernelGGL(k_mse, dim3(nb), dim3(256), 0, stream, Y, T, dY, rows, out_pad, out, red + 16);
     hipLaunchKernelGGL(k_sum_double, dim3(1), dim3(256), 0, stream, red + 16, nb, red);
+    backprop(rows);
+    if (!host_loss) return 0.0;
+    double loss = 0;
+    SEM_HIP(hipMemcpyAsync(&loss, red, sizeof(double), hipMemcpyDeviceToHost, stream));
+    SEM_HIP(hipStreamSynchronize(stream));
+    return loss / ((double)rows * out);
+}
+
+void Mlp::backprop(int rows)
+{
     const float *W2 = params + off[2], *W3 = params + off[4];
     float *gW1 = grads + off[0], *gb1 = grads + off[1], *gW2 = grads + off[2], *gb2 = grads + off[3],
           *gW3 = grads + off[4], *gb3 = grads + off[5];
@@ -432,11 +442,6 @@ double Mlp::backward(int rows, bool host_loss)
     // layer 1
     gemm(true, false, in_pad, h1, rows, X, in_pad, dA1, h1, gW1, h1, nullptr, split);
     colsum(dA1, rows, h1, gb1);
-    if (!host_loss) return 0.0;
-    double loss = 0;
-    SEM_HIP(hipMemcpyAsync(&loss, red, sizeof(double), hipMemcpyDeviceToHost, stream));
-    SEM_HIP(hipStreamSynchronize(stream));
-    return loss / ((double)rows * out);
 }
 
 void Mlp::adam_step()

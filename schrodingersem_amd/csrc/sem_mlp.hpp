@@ -36,6 +36,7 @@ struct Mlp {
     void get_params(float* host, bool grad);
     void forward(int rows);                       // X -> Y
     double backward(int rows, bool host_loss);    // X, T -> grads (MSE loss); returns the loss if host_loss
+    void backprop(int rows);                      // dY (and the activations of forward) -> grads
     void adam_step();
     void gemm(bool ta, bool tb, int M, int N, int K, const float* A, int lda, const float* B, int ldb, float* C, int ldc,
               const float* bias, int split);
