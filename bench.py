@@ -399,7 +399,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="dirk4s5", choices=["dirk4s5", "dirk4s3", "batched_bdf2"])
-    ap.add_argument("--tol", type=float, default=1e-13)
+    ap.add_argument("--tol", type=float, default=1e-12)
     ap.add_argument("--tol-truth", type=float, default=1e-15)
     ap.add_argument("--learn-steps", type=int, default=50,
                     help="training window of the guess network (steps before the checkpoint)")

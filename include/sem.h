@@ -82,6 +82,8 @@ typedef struct sem_run_stats {
     long mlp_train_steps;    /* Adam steps taken by the guess networks */
     double mlp_last_loss;    /* last training loss (mean squared error, scaled targets) */
     long learned_guess_chosen; /* stage solves (of each instance, in a batch) that started from the network's guess */
+    double learned_residual_digits; /* sum over those solves of log10(||b - Res ext|| / ||b - Res x0||): how many
+                                       digits of initial residual the network's guess x0 saves over the extrapolation ext */
 } sem_run_stats;
 
 typedef struct sem_mesh sem_mesh;
@@ -174,17 +176,20 @@ int sem_stepper_create(sem_mesh* m, int scheme /* 0 bdf2, 1 dirk4s3, 2 dirk4s5 *
 int sem_stepper_advance(sem_stepper* s, double T, long max_steps, sem_step_cb cb, void* user, int* success);
 int sem_stepper_read(sem_stepper* s, double* re, double* im, double* t);
 int sem_stepper_stats(const sem_stepper* s, sem_run_stats* out);
-/* learned guess: Adam steps per stage solve while training (default 10) and learning rate (default 3e-3) */
+/* learned guess: Adam steps per stage solve while training (default 5) and learning rate (default 3e-3) */
 int sem_stepper_set_learning(sem_stepper* s, int epochs, float lr);
 /* learned guess, before its first stage solve: the training loss (0: mean
  * squared error of the scaled correction z - ext; 1: the assembled residual
  * ||Res (g - z)||^2 of the guess g, relative to the extrapolation's, the norm
  * the guard compares and MINRES reduces) and the fourth input block (0: the
- * stage's right-hand side b; 1: the extrapolation's residual b - Res ext) */
+ * stage's right-hand side b; 1 (default): the extrapolation's residual b - Res ext) */
 int sem_stepper_set_training(sem_stepper* s, int loss, int inputs);
-/* learned guess: on (default) starts each stage solve from the network's guess
- * only when its true FP64 residual is below the extrapolation's (two extra
- * operator applies per stage solve); off always starts from the network's guess */
+/* learned guess: on (default) starts each stage solve from ext + c (g - ext),
+ * the point on the line through the extrapolation ext and the network's guess g
+ * with the smallest true FP64 residual ||b - Res x0|| (c = 0 is the
+ * extrapolation, so the start is never worse than the fallback; one extra
+ * operator apply and three dot products per stage solve); off always starts
+ * from the network's guess g */
 int sem_stepper_set_guard(sem_stepper* s, int on);
 /* Checkpoint and resume of a session (no counterpart in the reference, whose
  * steppers keep their BDF2 history, schrodinger.hpp:85-102, in a local deque

@@ -50,7 +50,8 @@ class _Stats(C.Structure):
     _fields_ = [("steps", C.c_int), ("stage_solves", C.c_int), ("total_iters", C.c_long),
                 ("refinements", C.c_long), ("last_error_estimate", C.c_double),
                 ("max_true_rel_residual", C.c_double), ("mlp_train_steps", C.c_long),
-                ("mlp_last_loss", C.c_double), ("learned_guess_chosen", C.c_long)]
+                ("mlp_last_loss", C.c_double), ("learned_guess_chosen", C.c_long),
+                ("learned_residual_digits", C.c_double)]
 
 
 STEP_CB = C.CFUNCTYPE(C.c_int, _dp, _dp, C.c_void_p)

@@ -1252,7 +1252,7 @@ static double inner_tol_floor(const sem_mesh* m) { return m->contraction == 1 ? 
 // (or tol*||b|| + atol when rel_to_b), the reference's test applied to the
 // true residual instead of MINRES's recurrence estimate.
 static Results solve_mixed(sem_mesh* m, int kind, const double* cf64, const float* cf32, double alpha, const double* b,
-                           double* x, bool x_zero, const KrylovOpts& o, double inner_tol)
+                           double* x, bool x_zero, const KrylovOpts& o, double inner_tol, const double* r_given = nullptr)
 {
     const int nc = op_ncomp(kind);
     const size_t n = (size_t)nc * m->len();
@@ -1264,6 +1264,10 @@ static Results solve_mixed(sem_mesh* m, int kind, const double* cf64, const floa
     if (x_zero) {
         launch_fill<double>(x, 0.0, n, s);
         launch_cast<double, double>(r, b, n, s);
+    } else if (r_given) {
+        // the caller already holds b - A x (the learned guess's combination);
+        // the stop test after every sweep still uses a freshly computed residual
+        launch_cast<double, double>(r, r_given, n, s);
     } else {
         apply_op<double>(m, kind, x, Ax, cf64, alpha);
         lincomb<double>(m, r, 0, {{1.0, b}, {-1.0, Ax}}, n);
@@ -1446,13 +1450,13 @@ struct Stepper {
     std::unique_ptr<Mlp> mlp;
     bool guard = true;  // keep the extrapolation when its residual is smaller
     bool residual_loss = false;  // train on ||Res (g - z)|| instead of the correction's squared error
-    bool feat_rext = false;      // fourth input: b - Res ext instead of b
+    bool feat_rext = true;       // fourth input: b - Res ext (true) or b
     double* rext = nullptr;      // b - Res ext of the current stage
     double* Vraw = nullptr;
     double* ext = nullptr;
     double* tmp = nullptr;
     int learn_steps = 50;
-    int epochs = 10;
+    int epochs = 5;
     double s_feat[5][3] = {};  // per stage: scales of z, V, b features
     double s_out[5] = {};      // per stage: scale of the correction
     bool have_scale[5] = {}, have_out[5] = {};
@@ -1477,7 +1481,8 @@ static void stage_rhs(Stepper& st, const double* q, double* out)
     apply_op<double>(st.m, K_F, q, out, st.V.c64, 0.0, st.opts.dt);
 }
 
-static Results solve_res(Stepper& st, double alpha, const double* b, double* x, bool zero, bool rel_to_b)
+static Results solve_res(Stepper& st, double alpha, const double* b, double* x, bool zero, bool rel_to_b,
+                         const double* r_given = nullptr)
 {
     sem_mesh* m = st.m;
     KrylovOpts ko;
@@ -1489,7 +1494,7 @@ static Results solve_res(Stepper& st, double alpha, const double* b, double* x, 
     Results r;
     const long ref0 = m->stats.refinements;
     if (st.opts.precision == SEM_PREC_MIXED) {
-        r = solve_mixed(m, K_RES, st.V.c64, st.V.c32, alpha, b, x, zero, ko, inner_tol_floor(m));
+        r = solve_mixed(m, K_RES, st.V.c64, st.V.c32, alpha, b, x, zero, ko, inner_tol_floor(m), r_given);
         st.stats.max_true_rel_residual = std::max(st.stats.max_true_rel_residual, m->last_true_rel);
     } else
         r = minres_fused<double>(m, K_RES, st.V.c64, alpha, b, x, zero, ko);
@@ -1573,6 +1578,7 @@ static Results stage_solve(Stepper& st, double alpha, const double* b, double* z
     // the fallback's trajectory); afterwards it is frozen and proposes guesses
     const bool learned = st.opts.guess == 2 && st.have_prev2;
     const bool training = learned && st.stats.steps < st.learn_steps;
+    const double* r_given = nullptr;  // b - Res z of the start, when already computed
     if (learned) {
         // extrapolation plus the network's correction, made consistent
         lincomb<double>(m, st.ext, 1, {{2.0, st.zprev[stage]}, {-1.0, st.zprev2[stage]}}, st.n);
@@ -1596,36 +1602,47 @@ static Results stage_solve(Stepper& st, double alpha, const double* b, double* z
             launch_scatter_output(scatter_args(st, stage, z, st.ext, st.s_out[stage]), m->stream);
             dss<double>(m, DSS_UNMASK, z, 2);
             dss<double>(m, DSS_ZERO_BND, z, 2);
-            // guard: the extrapolation stays the fallback; keep whichever
-            // guess has the smaller true FP64 residual ||b - Res z||
+            // guard: the extrapolation stays the fallback. The guess is
+            // ext + c (z - ext) with the c that minimises the true FP64
+            // residual ||b - Res x0|| = ||r_ext - c (r_ext - r_z)||: c = 0 is
+            // the extrapolation and c = 1 the network's guess, so the start is
+            // never worse than either, and both residuals are already at hand
+            // (no further operator apply; the refinement starts from r0).
             if (!st.guard) {
                 st.stats.learned_guess_chosen++;
             } else {
-            double* rz = m->buf<double>("guard_r", st.n);
-            apply_op<double>(m, K_RES, z, Az, st.V.c64, alpha);
-            lincomb<double>(m, rz, 0, {{1.0, b}, {-1.0, Az}}, st.n);
-            wdot_dev<double>(m, rz, rz, 2, 2);
-            wdot_dev<double>(m, st.rext, st.rext, 2, 3);
-            m->read_scalars();
-            if (!st.multi()) {
-                if (m->h_sc[0].red[3] < m->h_sc[0].red[2]) launch_cast<double, double>(z, st.ext, st.n, m->stream);
-                else st.stats.learned_guess_chosen++;
-            } else {
-                // each instance keeps the guess with its own smaller residual
+                double* rz = m->buf<double>("guard_r", st.n);
+                apply_op<double>(m, K_RES, z, Az, st.V.c64, alpha);
+                lincomb<double>(m, rz, 0, {{1.0, b}, {-1.0, Az}}, st.n);
+                wdot_dev<double>(m, rz, rz, 2, 2);
+                wdot_dev<double>(m, st.rext, st.rext, 2, 3);
+                wdot_dev<double>(m, st.rext, rz, 2, 4);
+                m->read_scalars();
                 const size_t cl = (size_t)m->L.K * m->L.N * m->L.N;
                 for (int k = 0; k < m->n_inst; ++k) {
-                    if (st.skip[k]) continue;
-                    if (!(m->h_sc[k].red[3] < m->h_sc[k].red[2])) {
+                    if (st.multi() && st.skip[k]) continue;
+                    const double gg = m->h_sc[k].red[2], ee = m->h_sc[k].red[3], eg = m->h_sc[k].red[4];
+                    const double dd = ee - 2.0 * eg + gg;  // ||r_ext - r_z||^2 = ||Res (z - ext)||^2
+                    const double c = dd > 0 ? (ee - eg) / dd : 0.0;
+                    // residual of the combination: evaluated below; the
+                    // statistic uses the smaller of its two exact end points
+                    // when cancellation makes the closed form unreliable
+                    double r2 = ee - c * (ee - eg);
+                    if (!(r2 > 1e-12 * ee)) r2 = std::min(ee, gg);
+                    if (c != 0.0) {
                         st.stats.learned_guess_chosen++;
-                        continue;
+                        if (ee > 0 && r2 > 0) st.stats.learned_residual_digits += 0.5 * std::log10(ee / r2);
                     }
-                    const size_t off = (size_t)m->h_inst_off[k] * cl;
-                    const size_t cnt = (size_t)(m->h_inst_off[k + 1] - m->h_inst_off[k]) * cl;
-                    for (int c = 0; c < 2; ++c)
-                        launch_cast<double, double>(z + c * m->len() + off, st.ext + c * m->len() + off, cnt,
-                                                    m->stream);
+                    // x0 = (1 - c) ext + c z, r0 = (1 - c) r_ext + c r_z, on instance k's chunks
+                    const size_t off = st.multi() ? (size_t)m->h_inst_off[k] * cl : 0;
+                    const size_t cnt = st.multi() ? (size_t)(m->h_inst_off[k + 1] - m->h_inst_off[k]) * cl : m->len();
+                    for (int q = 0; q < 2; ++q) {
+                        const size_t o = q * m->len() + off;
+                        lincomb<double>(m, z + o, 1, {{1.0 - c, st.ext + o}, {c, z + o}}, cnt);
+                        lincomb<double>(m, rz + o, 1, {{1.0 - c, st.rext + o}, {c, rz + o}}, cnt);
+                    }
                 }
-            }
+                r_given = rz;
             }
         } else {
             launch_cast<double, double>(z, st.ext, st.n, m->stream);
@@ -1643,7 +1660,7 @@ static Results stage_solve(Stepper& st, double alpha, const double* b, double* z
     bool whole = true;
     if (st.multi())
         for (int k = 0; k < m->n_inst; ++k) whole = whole && !st.skip[k];
-    Results r = solve_res(st, alpha, b, z, zero, !zero);
+    Results r = solve_res(st, alpha, b, z, zero, !zero, r_given);
     if (st.multi())
         for (int k = 0; k < m->n_inst; ++k) whole = whole && !st.skip[k];
     if (training && r.success && whole) {
@@ -3057,6 +3074,7 @@ int sem_batch_advance(sem_batch* b, double T, long max_steps, int* success)
             tot.mlp_train_steps += g.mlp_train_steps;
             tot.mlp_last_loss = std::max(tot.mlp_last_loss, g.mlp_last_loss);
             tot.learned_guess_chosen += g.learned_guess_chosen;
+            tot.learned_residual_digits += g.learned_residual_digits;
         }
         b->stats = tot;
         if (success) {

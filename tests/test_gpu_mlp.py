@@ -121,10 +121,12 @@ def test_learned_guess_in_stepper(sem):
     for name in res:
         assert rel(res[name][1], truth) < 1e-9, name
     assert res["learned"][2]["mlp_train_steps"] > 0
-    # the residual guard picks the network's guess for some stage solves, and
-    # never does worse than the extrapolation by more than the guard's choice
-    # of a lower starting residual allows
+    # every frozen stage solve starts from the minimal-residual point between
+    # the extrapolation and the network's guess (never a larger residual than
+    # the extrapolation's), and the run takes no more iterations than the
+    # fallback; the gain at the workload's size is in test_gpu_learned_gain.py
     assert res["learned"][2]["learned_guess_chosen"] > 0
+    assert res["learned"][2]["learned_residual_digits"] > 0
     assert res["ext"][2]["learned_guess_chosen"] == 0
     print("iterations over the 8 frozen steps: extrapolated %d, learned %d" % (res["ext"][0], res["learned"][0]))
-    assert res["learned"][0] <= 1.05 * res["ext"][0]
+    assert res["learned"][0] <= res["ext"][0]
