@@ -137,6 +137,15 @@ int sem_element_apply_fp32(sem_mesh* m, int kind, const double* coef, double alp
  * reference. */
 int sem_set_contraction(sem_mesh* m, int mode);
 int sem_get_contraction(const sem_mesh* m, int* mode);
+/* FP32 MINRES with the one-launch Lanczos step (the inner solves of the mixed
+ * precision): on (default) keeps every Lanczos vector in device memory, up to
+ * budget_bytes (0: keep the current budget, 8 GiB by default), and forms the
+ * solution from them once at the end instead of updating w and x every step
+ * (the same iterate, minres.hpp:82-86 unrolled); off uses the w/x recurrence.
+ * A solve that fills the budget stops there unconverged, as at maxit (the
+ * FP64 refinement then continues with another inner solve). Batched meshes
+ * keep the recurrence. No counterpart in the reference. */
+int sem_set_stored_basis(sem_mesh* m, int on, long budget_bytes);
 /* galerkin_op: unmask -> element op -> global_sum -> mask (galerkin.hpp:24-35);
  * 2-component kinds take and return [re | im]. */
 int sem_galerkin_apply(sem_mesh* m, int kind, const double* coef, double alpha, const double* in, double* out);

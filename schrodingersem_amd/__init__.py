@@ -132,6 +132,7 @@ def lib():
             "sem_element_apply_fp32": (C.c_int, [C.c_void_p, C.c_int, _dp, C.c_double, _dp, _dp]),
             "sem_set_contraction": (C.c_int, [C.c_void_p, C.c_int]),
             "sem_get_contraction": (C.c_int, [C.c_void_p, _ip]),
+            "sem_set_stored_basis": (C.c_int, [C.c_void_p, C.c_int, C.c_long]),
             "sem_minres": (C.c_int, [C.POINTER(_Krylov), _dp, _dp, C.c_int, C.c_double, C.POINTER(_Results)]),
             "sem_pcg": (C.c_int, [C.POINTER(_Krylov), _dp, _dp, C.c_int, C.c_double, C.POINTER(_Results)]),
             "sem_minres_noalloc": (C.c_int, [C.POINTER(_Krylov), _dp, _dp, C.c_int, C.c_double, C.POINTER(_Results)]),
@@ -1033,6 +1034,11 @@ def element_op_fp32(kind, u, mesh, coef=None, alpha=0.0):
 def set_contraction(mesh, mode):
     """Stiffness contractions of the FP32 inner solver: 0 VALU, 1 bf16 MFMA, 3 split-bf16 MFMA."""
     _check(lib().sem_set_contraction(mesh._h, int(mode)))
+
+
+def set_stored_basis(mesh, on, budget_bytes=0):
+    """FP32 one-launch MINRES: keep every Lanczos vector (on, default) or update w/x each step."""
+    _check(lib().sem_set_stored_basis(mesh._h, int(bool(on)), int(budget_bytes)))
 
 
 def get_contraction(mesh):

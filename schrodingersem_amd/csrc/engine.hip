@@ -137,6 +137,10 @@ struct sem_mesh {
     // stiffness contractions of the FP32 inner solver's operator: 0 VALU FMAs,
     // 1 bf16 MFMA tiles, 3 split-bf16 MFMA tiles (FP32 accumulation)
     int contraction = 0;
+    // FP32 one-launch MINRES keeps every Lanczos vector in HBM (stored basis,
+    // no w/x streams per step) within basis_budget bytes; 0: the w/x recurrence
+    int basis = 1;
+    size_t basis_budget = (size_t)8 << 30;
     bool is_geo = false;
     const double* is_cf = nullptr;
     long is_gen = -1;
@@ -619,6 +623,7 @@ static void patch_instances(sem_mesh* m, const KrylovOpts& o)
 // Upper bound of the first batch of Lanczos steps launched before a host
 // check of the stop flags (and of the timing events kept per batch).
 constexpr int kMaxFirstBatch = 2048;
+constexpr int kBasisMax = 4096;  // most Lanczos vectors a stored-basis MINRES solve keeps
 
 static bool sampled_step(int i) { return (((unsigned)i * 2654435761u) >> 20 & 3u) == 0; }
 
@@ -640,17 +645,41 @@ static Results minres_fused1(sem_mesh* m, int kind, const T* cf, double alpha, c
     double* S[2] = {m->buf<double>("mr_S0", 3 * (size_t)nch), m->buf<double>("mr_S1", 3 * (size_t)nch)};
     hipStream_t s = m->stream;
     const int ni = m->n_inst;
+    const bool graphs = m->use_graphs && !m->timing;
+    // Stored basis (FP32, one instance, no graph replay): every Lanczos vector
+    // y_j keeps its own slot of HBM, the steps skip the w/x update (five of
+    // the step's vector streams per component) and record each iteration's
+    // update coefficients; x = x0 + sum_j c_j y_{j-1} is formed once at the
+    // end (minres.hpp:82-86 unrolled: W R = V, x = x0 + V R^-1 t). The basis
+    // holds at most basis_budget bytes; a solve that reaches its capacity
+    // stops there unconverged, as at maxit.
+    int cap = 0;
+    T* Bv = nullptr;
+    double* rec = nullptr;
+    if (sizeof(T) == 4 && m->basis && ni == 1 && !graphs) {
+        const size_t fit = m->basis_budget / (n * sizeof(T));
+        cap = (int)std::min<size_t>({(size_t)o.maxit + 2, (size_t)kBasisMax, fit});
+        if (cap >= 16) {
+            Bv = m->buf<T>("mr_basis", (size_t)cap * n);
+            rec = m->buf<double>("mr_rec", 5 * (size_t)cap);
+        } else {
+            cap = 0;
+        }
+    }
+    const int maxit = cap ? std::min(o.maxit, cap - 2) : o.maxit;
+    T* y0 = cap ? Bv : Y[0];
     if (x_zero) {
-        launch_fill<T>(x, T(0), n, s);
-        launch_cast<T, T>(Y[0], b, n, s);
+        if (!cap) launch_fill<T>(x, T(0), n, s);
+        launch_cast<T, T>(y0, b, n, s);
     } else {
         T* Ax = m->buf<T>("mr_Ax" + tp, n);
         apply_op<T>(m, kind, x, Ax, cf, alpha);
-        lincomb<T>(m, Y[0], 0, {{1.0, b}, {-1.0, Ax}}, n);  // y_0 = r1 = b - A(x)
+        lincomb<T>(m, y0, 0, {{1.0, b}, {-1.0, Ax}}, n);  // y_0 = r1 = b - A(x)
     }
-    for (int q = 0; q < 3; ++q) launch_fill<T>(W[q], T(0), n, s);
-    wdot_dev<T>(m, Y[0], Y[0], nc, 0);
-    launch_minres_init(m->sc, ni, 0, o.maxit, o.tol, s);
+    if (!cap)
+        for (int q = 0; q < 3; ++q) launch_fill<T>(W[q], T(0), n, s);
+    wdot_dev<T>(m, y0, y0, nc, 0);
+    launch_minres_init(m->sc, ni, 0, maxit, o.tol, s);
     if (o.rel_to_b) wdot_dev<T>(m, b, b, nc, 1);
     patch_instances(m, o);
     const int ti = sizeof(T) == 8 ? 0 : 1;
@@ -666,7 +695,6 @@ static Results minres_fused1(sem_mesh* m, int kind, const T* cf, double alpha, c
     // (a multiple of 6) steps captured from a step i = 1 (mod 6), i >= 7,
     // replays every later such batch exactly; the first batch is rounded up to
     // a multiple of 6 to start the replays there
-    const bool graphs = m->use_graphs && !m->timing;
     const int G = (m->graph_steps + 5) / 6 * 6;
     int first_batch = std::max(m->batch, std::min(m->iter_hint[slot] + 4, kMaxFirstBatch));
     if (graphs) first_batch = (first_batch + 5) / 6 * 6;
@@ -681,7 +709,7 @@ static Results minres_fused1(sem_mesh* m, int kind, const T* cf, double alpha, c
         // per component, plus the diagonal scale when there is one
         const double EN2 = (double)m->L.E * m->L.N * m->L.N;
         const double geo_cf = (kind == K_RES) ? 5 : (kind == K_HELMHOLTZ ? 4 : 3);
-        m->kt_bytes[ti] = (geo_cf + 10.0 * nc) * sizeof(T) * EN2 + (dscale ? sizeof(float) * EN2 : 0.0);
+        m->kt_bytes[ti] = (geo_cf + (cap ? 5.0 : 10.0) * nc) * sizeof(T) * EN2 + (dscale ? sizeof(float) * EN2 : 0.0);
     }
     KrylovScalars* buf[2] = {m->sc, m->sc_alt};
     const InstTable itab = m->it();
@@ -718,6 +746,12 @@ static Results minres_fused1(sem_mesh* m, int kind, const T* cf, double alpha, c
         a.wb = W[i % 3];
         a.wn = W[(i + 1) % 3];
         a.x = x;
+        if (cap) {  // y_j in slot j
+            a.y_out = Bv + (size_t)(i - 1) * n;
+            a.y2 = Bv + (size_t)std::max(i - 2, 0) * n;
+            a.y3 = Bv + (size_t)std::max(i - 3, 0) * n;
+            a.rec = rec;
+        }
         a.geo = m->geo_t<T>();
         a.cf = cf;
         a.alpha = (T)alpha;
@@ -745,7 +779,7 @@ static Results minres_fused1(sem_mesh* m, int kind, const T* cf, double alpha, c
         a.dscale = dscale;
         launch_minres_fused<T>(m->L.N, kind, a, m->dmat<T>(), shp.op_cpb, shp.op_grid, s,
                                tim ? m->ev[4 * b_] : nullptr, tim ? m->ev[4 * b_ + 1] : nullptr);
-        if (ext) launch_minres_fused_close(itab, m->sc_alt, m->sc, S[i % 2], i + 1, s);
+        if (ext) launch_minres_fused_close(itab, m->sc_alt, m->sc, S[i % 2], i + 1, s, rec);
     };
     int i = 1;
     bool done = o.maxit < 1;
@@ -783,13 +817,13 @@ static Results minres_fused1(sem_mesh* m, int kind, const T* cf, double alpha, c
         const int i0 = i;
         int cur = 0;
         const int nbatch = (i == 1) ? first_batch : m->batch;
-        for (int b_ = 0; b_ < nbatch && i <= o.maxit + 2; ++b_, ++i) {
+        for (int b_ = 0; b_ < nbatch && i <= maxit + 2; ++b_, ++i) {
             const bool tim = m->timing == 2 || (m->timing == 1 && sampled_step(i));
             launch_step(i, b_, cur, tim);
             ++launched;
         }
         // the batch ended after step i-1; close with what step i would compute first
-        if (!ext) launch_minres_fused_close(itab, buf[cur], m->sc, S[(i + 1) % 2], i, s);
+        if (!ext) launch_minres_fused_close(itab, buf[cur], m->sc, S[(i + 1) % 2], i, s, rec);
         m->read_scalars();
         if (m->timing) {
             const int last = max_iter() + 2;
@@ -801,11 +835,34 @@ static Results minres_fused1(sem_mesh* m, int kind, const T* cf, double alpha, c
                 m->kt_launches[ti] += 1;
             }
         }
-        done = all_done() || i > o.maxit + 2;
+        done = all_done() || i > maxit + 2;
     }
     bool pending = false;
     for (int k = 0; k < ni; ++k) pending |= m->h_sc[k].p3_pending != 0;
-    if (pending) {
+    if (cap) {
+        // x = x0 + sum_j c_j y_{j-1}: with the records of iterations 1..k, the
+        // coefficient of w_j in x is t_j = phi_j - t_{j+1} delta_{j+1} / gamma_{j+1}
+        // - t_{j+2} eps_{j+2} / gamma_{j+2}, and w_j holds y_{j-1} / (beta_j gamma_j)
+        const int k = m->h_sc[0].iter;
+        if (k > 0) {
+            std::vector<double> r(5 * (size_t)k), t(k + 2, 0.0), c(k);
+            SEM_HIP(hipMemcpyAsync(r.data(), rec, r.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+            SEM_HIP(hipStreamSynchronize(s));
+            for (int j = k; j >= 1; --j) {
+                double v = r[5 * (j - 1) + 4];
+                if (j + 1 <= k) v -= t[j + 1] * r[5 * j + 1] * r[5 * j + 3];
+                if (j + 2 <= k) v -= t[j + 2] * r[5 * (j + 1) + 1] * r[5 * (j + 1) + 2];
+                t[j] = v;
+                c[j - 1] = v * r[5 * (j - 1) + 1] * r[5 * (j - 1)];
+            }
+            double* dc = m->buf<double>("mr_coef", (size_t)k);
+            SEM_HIP(hipMemcpyAsync(dc, c.data(), (size_t)k * sizeof(double), hipMemcpyHostToDevice, s));
+            launch_basis_combine<T>(x, !x_zero, Bv, n, n, dc, k, s);
+            SEM_HIP(hipStreamSynchronize(s));  // c lives on this stack frame
+        } else if (x_zero) {
+            launch_fill<T>(x, T(0), n, s);
+        }
+    } else if (pending) {
         // iteration k = iter of each instance: v = y_{k-1}, w_{k-2}, w_{k-1} -> w_k;
         // the flush kernel's rotation (R[(k+2)%3], W[(k+1)%3], W[(k+2)%3] -> W[k%3]) matches
         MinresFlushArgs<T> fa{};
@@ -822,7 +879,7 @@ static Results minres_fused1(sem_mesh* m, int kind, const T* cf, double alpha, c
     for (int k = 0; k < ni; ++k) {
         Results& r = m->inst_results[k];
         const KrylovScalars& h = m->h_sc[k];
-        const int mk = o.inst_maxit ? o.inst_maxit[k] : o.maxit;
+        const int mk = o.inst_maxit ? o.inst_maxit[k] : maxit;
         r.success = h.success;
         r.n_iter = r.success ? h.iter : mk + 1;
         if (mk < 1) r.n_iter = 1;
@@ -2666,6 +2723,17 @@ int sem_set_contraction(sem_mesh* m, int mode)
         return SEM_EINVAL;
     }
     m->contraction = mode;
+    return SEM_OK;
+}
+
+int sem_set_stored_basis(sem_mesh* m, int on, long budget_bytes)
+{
+    if (!m || budget_bytes < 0) {
+        set_error("sem_set_stored_basis: bad argument");
+        return SEM_EINVAL;
+    }
+    m->basis = on != 0;
+    if (budget_bytes > 0) m->basis_budget = (size_t)budget_bytes;
     return SEM_OK;
 }
 

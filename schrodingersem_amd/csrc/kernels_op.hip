@@ -888,8 +888,11 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 3 : 1, 8))) void k_minres_fu
             }
             // this kernel applies the pending w/x update; the state it stores drops it
             pend = sc.p3_pending != 0;
+            if (c0 == ci.y && threadIdx.x == 0 && pend && a.rec) minres_record(a.rec, sc);
             sc.p3_pending = 0;
             if (c0 == ci.y && threadIdx.x == 0) *static_cast<MinresState*>(a.st_out + inst) = sc;
+            // stored basis: no w/x update (the solution is formed at the end)
+            if (a.rec) pend = false;
         }
         // a stopped instance has no work left: skip the rest of its items
         // (its sums are never read again)
@@ -1013,7 +1016,7 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 3 : 1, 8))) void k_minres_fu
 // Closes a batch of fused steps: the recurrence step `step` would evaluate at
 // its start, with the same reduction; one workgroup per instance.
 __global__ __launch_bounds__(256) void k_minres_fused_close(InstTable it, const KrylovScalars* in, KrylovScalars* out,
-                                                            const double* sums, int nch, int step)
+                                                            const double* sums, int nch, int step, double* rec)
 {
     __shared__ double red[12];
     const int k = blockIdx.x;
@@ -1025,14 +1028,16 @@ __global__ __launch_bounds__(256) void k_minres_fused_close(InstTable it, const 
     if (threadIdx.x == 0) {
         MinresState sc = *static_cast<const MinresState*>(in + k);
         minres_fused_prologue(sc, step, P);
+        if (rec && sc.p3_pending) minres_record(rec, sc);
         *static_cast<MinresState*>(out + k) = sc;
     }
 }
 
 void launch_minres_fused_close(const InstTable& it, const KrylovScalars* in, KrylovScalars* out, const double* sums,
-                               int step, hipStream_t s)
+                               int step, hipStream_t s, double* rec)
 {
-    hipLaunchKernelGGL(k_minres_fused_close, dim3(it.n_inst), dim3(256), 0, s, it, in, out, sums, it.nchunks, step);
+    hipLaunchKernelGGL(k_minres_fused_close, dim3(it.n_inst), dim3(256), 0, s, it, in, out, sums, it.nchunks, step,
+                       rec);
 }
 #endif
 

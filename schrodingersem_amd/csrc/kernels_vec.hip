@@ -166,6 +166,40 @@ void launch_lincomb(const LinComb<T>& a, hipStream_t s)
 template void launch_lincomb<double>(const LinComb<double>&, hipStream_t);
 template void launch_lincomb<float>(const LinComb<float>&, hipStream_t);
 
+// x (+)= sum_j coef[j] * basis_j: each thread accumulates its entries in FP64
+// over the stored vectors in j order (eight loads in flight per round)
+template <typename T>
+__global__ __launch_bounds__(256) void k_basis_combine(T* __restrict__ x, int accumulate, const T* __restrict__ basis,
+                                                       size_t stride, size_t n, const double* __restrict__ coef, int k)
+{
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
+        double acc = accumulate ? (double)x[i] : 0.0;
+        int j = 0;
+        for (; j + 8 <= k; j += 8) {
+            T v[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) v[u] = basis[(size_t)(j + u) * stride + i];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) acc += coef[j + u] * (double)v[u];
+        }
+        for (; j < k; ++j) acc += coef[j] * (double)basis[(size_t)j * stride + i];
+        x[i] = (T)acc;
+    }
+}
+
+template <typename T>
+void launch_basis_combine(T* x, bool accumulate, const T* basis, size_t stride, size_t n, const double* coef, int k,
+                          hipStream_t s)
+{
+    if (n == 0) return;
+    const int nb = (int)std::min<size_t>((n + 255) / 256, 8192);
+    hipLaunchKernelGGL((k_basis_combine<T>), dim3(nb), dim3(256), 0, s, x, accumulate ? 1 : 0, basis, stride, n,
+                       coef, k);
+}
+template void launch_basis_combine<float>(float*, bool, const float*, size_t, size_t, const double*, int, hipStream_t);
+template void launch_basis_combine<double>(double*, bool, const double*, size_t, size_t, const double*, int,
+                                           hipStream_t);
+
 template <typename D, typename S>
 __global__ __launch_bounds__(256) void k_cast(D* out, const S* in, size_t n)
 {

@@ -246,6 +246,20 @@ __device__ __forceinline__ void minres_beta_update(MinresState& sc, double s)
     }
 }
 
+// Stored-basis MINRES: the update coefficients of iteration j = sc.iter
+// (minres.hpp:82-86: w_j = (v_j - eps_j w_{j-2} - delta_j w_{j-1}) / gamma_j,
+// x += phi_j w_j, v_j = y_{j-1} / beta_j), kept for forming x from the stored
+// y vectors at the end
+__device__ __forceinline__ void minres_record(double* rec, const MinresState& sc)
+{
+    double* r = rec + 5 * (size_t)(sc.iter - 1);
+    r[0] = sc.p3_ibp;
+    r[1] = sc.p3_ig;
+    r[2] = sc.p3_eo;
+    r[3] = sc.p3_dl;
+    r[4] = sc.p3_phi;
+}
+
 // Recurrence at the start of fused Lanczos step i >= 2 (deferred scaling: step
 // i finishes y_{i-1} = A y_{i-2} / beta_{i-1} - (beta_{i-1}/beta_{i-2}) y_{i-3}
 // - (alpha_{i-1}/beta_{i-1}) y_{i-2}, then applies A to it). P are the sums of

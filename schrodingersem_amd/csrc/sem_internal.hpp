@@ -245,6 +245,11 @@ struct MinresFusedArgs {
     int step;   // i (1-based)
     int first;  // first launch of a batch: the state already holds this step's recurrence
     int mf;     // stiffness contractions on MFMA (0 VALU, 1 bf16, 3 split bf16)
+    // stored-basis mode (non-null): every y_j stays in its own slot, the step
+    // skips the w/x update and records iteration j's update coefficients
+    // (1/beta_j, 1/gamma_j, eps_j, delta_j, phi_j) at rec[5 (j - 1)]; the
+    // solution is formed from the stored vectors at the end
+    double* rec;
 };
 
 // Work-item shape of the two Lanczos-step kernels: CPB chunks per workgroup
@@ -289,7 +294,11 @@ void launch_minres_fused(int N, int kind, const MinresFusedArgs<T>& a, const T* 
 // recurrence closing a batch of fused steps: what step `step` would compute
 // at its start, one workgroup per instance (reads in[k], writes out[k])
 void launch_minres_fused_close(const InstTable& it, const KrylovScalars* in, KrylovScalars* out, const double* sums,
-                               int step, hipStream_t s);
+                               int step, hipStream_t s, double* rec = nullptr);
+// x[i] (+)= sum_j coef[j] * basis[j * stride + i], i < n, the sum in FP64 in j order
+template <typename T>
+void launch_basis_combine(T* x, bool accumulate, const T* basis, size_t stride, size_t n, const double* coef, int k,
+                          hipStream_t s);
 // update kernel of a Lanczos step
 template <typename T>
 void launch_minres_upd(int N, const MinresUpdArgs<T>& a, int cpb, int grid, hipStream_t s, hipEvent_t e0 = nullptr,
