@@ -1015,16 +1015,42 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 3 : 1, 8))) void k_minres_fu
 #ifndef SEM_OP_FP32
 // Closes a batch of fused steps: the recurrence step `step` would evaluate at
 // its start, with the same reduction; one workgroup per instance.
-__global__ __launch_bounds__(256) void k_minres_fused_close(InstTable it, const KrylovScalars* in, KrylovScalars* out,
+__global__ __launch_bounds__(768) void k_minres_fused_close(InstTable it, const KrylovScalars* in, KrylovScalars* out,
                                                             const double* sums, int nch, int step, double* rec)
 {
     __shared__ double red[12];
     const int k = blockIdx.x;
     const int p0 = it.inst_off[k];
+    const int n = it.inst_off[k + 1] - p0;
+    // twelve waves: wave 4a + g adds virtual lanes 64 g .. 64 g + 63 of array a
+    // (group_sum3's decomposition and order, so the same bits), every load of
+    // an instance of up to 8192 chunks in one round
+    {
+        const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+        const int a = wv >> 2, g = wv & 3;
+        const double* pa = sums + p0 + (size_t)a * nch;
+        constexpr int KB = 32;
+        double sacc = 0;
+        for (int kb = 0; kb * 256 < n; kb += KB) {
+            double v[KB];
+#pragma unroll
+            for (int u = 0; u < KB; ++u) {
+                const int q = (kb + u) * 256 + g * 64 + lane;
+                v[u] = q < n ? pa[q] : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < KB; ++u) {
+                const int q = (kb + u) * 256 + g * 64 + lane;
+                if (q < n) sacc += v[u];
+            }
+        }
+        const double t = wave_sum(sacc);
+        if (lane == 0) red[4 * a + g] = t;
+    }
+    __syncthreads();
     double P[3];
-    // every load of an instance of up to 8192 chunks in one round (the order of
-    // the additions does not depend on the load budget)
-    group_sum3<32>(sums + p0, nch, it.inst_off[k + 1] - p0, red, 4, P);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) P[a] = ((red[4 * a] + red[4 * a + 1]) + red[4 * a + 2]) + red[4 * a + 3];
     if (threadIdx.x == 0) {
         MinresState sc = *static_cast<const MinresState*>(in + k);
         minres_fused_prologue(sc, step, P);
@@ -1036,7 +1062,7 @@ __global__ __launch_bounds__(256) void k_minres_fused_close(InstTable it, const 
 void launch_minres_fused_close(const InstTable& it, const KrylovScalars* in, KrylovScalars* out, const double* sums,
                                int step, hipStream_t s, double* rec)
 {
-    hipLaunchKernelGGL(k_minres_fused_close, dim3(it.n_inst), dim3(256), 0, s, it, in, out, sums, it.nchunks, step,
+    hipLaunchKernelGGL(k_minres_fused_close, dim3(it.n_inst), dim3(768), 0, s, it, in, out, sums, it.nchunks, step,
                        rec);
 }
 #endif

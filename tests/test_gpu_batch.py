@@ -163,6 +163,9 @@ def test_batch_sdirk_matches_single_runs_bitwise(sem, scheme, precision):
     fn = (None, "schrodinger_dirk4s3", "schrodinger_dirk4s5")[scheme]
     single, iters = [], 0
     for m, _, _, V, re, im in inst:
+        # batches run the FP32 inner MINRES with the w/x recurrence; the single
+        # runs use it too (a stored-basis solve rounds its iterate differently)
+        sem.set_stored_basis(m, 0)
         a, b = re.copy(), im.copy()
         assert getattr(sem, fn)(a, b, V, m, T, opts)
         st = m.last_run_stats()
@@ -236,6 +239,7 @@ def test_batch_instance_failure(sem, scheme, precision):
 
     def single(x, maxit):
         m, _, _, V, re, im = x
+        sem.set_stored_basis(m, 0)  # the batch's FP32 inner MINRES form (w/x recurrence)
         a, b = re.copy(), im.copy()
         ok = getattr(sem, fn)(a, b, V, m, T, sem.SolverOpts(max_iter=maxit, tol=1e-12, dt=dt, precision=precision))
         return ok, a, b, m.last_run_stats()["steps"]
@@ -278,6 +282,7 @@ def test_fused_step_graphs_bitwise(sem, mode):
     rng = np.random.default_rng(21)
     b = np.concatenate([sem.gproj(rng.standard_normal(m.size), m), sem.gproj(rng.standard_normal(m.size), m)])
     sem.set_step_mode(m, mode, sem.PREC_MIXED)
+    sem.set_stored_basis(m, 0)  # graph replay runs the w/x recurrence; so do the direct launches here
     out = []
     for graphs, g in ((False, 12), (True, 12), (True, 6), (True, 9)):
         sem.set_graph_mode(m, graphs, g)
