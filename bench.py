@@ -336,9 +336,13 @@ def bench_sdirk(sem, args, workload):
             kern = json.load(fh)["kernels"]
         # the committed PMC summary of the same kernel form (its algorithmic
         # bytes per launch recorded with it; a summary of another form is ignored)
-        name = "void sem::%s<%d, float, 3, %d, %s>" % (kbase, p + 1, shape[0], "true" if shape[1] else "false")
-        if name in kern and kern[name].get("algorithmic_bytes") == bpl:
-            traffic = round(kern[name]["traffic_bytes"])
+        # (rocprofv3 names carry the full signature; either chunks-per-workgroup
+        # shape of the one-shot form moves the same bytes)
+        for name, e in kern.items():
+            if (name.startswith("void sem::%s<%d, float, 3, " % (kbase, p + 1)) and ", false, 0>" in name
+                    and e.get("algorithmic_bytes") == bpl):
+                traffic = round(e["traffic_bytes"])
+                break
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "kernel": "%s<%d,float,RES,%d,%s>" % (kbase, p + 1, shape[0], "true" if shape[1] else "false"),
