@@ -65,18 +65,22 @@ def log(*a):
 
 # --------------------------------------------------------------- batched --
 def bench_batched(sem, args):
-    """64 uneven BDF2 instances: aggregate DOF*steps/s, batched vs one at a time, same steps."""
+    """64 uneven BDF2 instances: aggregate DOF*steps/s, batched vs one at a time, same steps.
+    Mixed precision (default) or FP64; the one-at-a-time runs use the batch's
+    work shapes, graph replay and FP32 inner-solver form, so each instance's
+    result must equal its batched result bit for bit."""
     from schrodingersem_amd import configs
     dt, tol = 2e-3, args.batch_tol
+    prec = sem.PREC_MIXED if args.batch_precision == "mixed" else sem.PREC_FP64
     inst = configs.config4(sem, args.instances)
     dof = np.array([x["mesh"].dof() for x in inst], dtype=np.float64)
     log("instances: %d, degrees %s, total dof %d" % (len(inst), sorted({x["p"] for x in inst}), int(dof.sum())))
     batch = sem.Batch([x["mesh"] for x in inst])
     groups = batch.info()
     t0 = time.perf_counter()
-    tuned = batch.autotune(sem.PREC_FP64, 24)
+    tuned = batch.autotune(prec, 24)
     log("autotune %.1f s: %s" % (time.perf_counter() - t0, tuned))
-    opts = sem.SolverOpts(max_iter=100000, tol=tol, dt=dt)
+    opts = sem.SolverOpts(max_iter=100000, tol=tol, dt=dt, precision=prec)
     nsteps = args.batch_steps
     T = (nsteps - 0.5) * dt
     batch.start_bdf2([x["re"] for x in inst], [x["im"] for x in inst], [x["V"] for x in inst], opts)
@@ -97,8 +101,9 @@ def bench_batched(sem, args):
     same = True
     for k, x in enumerate(inst):
         m = x["mesh"]
-        sem.set_step_shape(m, tuned[x["p"]], sem.PREC_FP64)
+        sem.set_step_shape(m, tuned[x["p"]], prec)
         sem.set_graph_mode(m, True, 12)
+        sem.set_stored_basis(m, 0)
         a, b = x["re"].copy(), x["im"].copy()
         t0 = time.perf_counter()
         assert sem.schrodinger_bdf2(a, b, x["V"], m, T, opts)
@@ -114,10 +119,10 @@ def bench_batched(sem, args):
         "n_gpus": 1,
         "steps": nsteps,
         "higher_is_better": True,
-        "dtype": "f64",
+        "dtype": "f64" if prec == sem.PREC_FP64 else "f64+f32",
         "data": "synthetic",
         "config": {"workload": "batched_bdf2_64_instances", "n": "U{16..96}", "degrees": [4, 6, 8, 10, 12],
-                   "dt": dt, "tol": tol, "total_dof": int(dof.sum())},
+                   "dt": dt, "tol": tol, "total_dof": int(dof.sum()), "precision": args.batch_precision},
         "seconds_batched": round(tb, 3),
         "one_at_a_time": {"DOF*steps/s": round(rate_s, 1), "steps": nsteps, "seconds": round(ts, 3)},
         "speedup_batched_vs_one_at_a_time": round(rate_b / rate_s, 3),
@@ -234,8 +239,23 @@ def cpu_baseline(args, xy, q, p, V, re, im, scheme_alpha, tol):
             row[uname] = round(1000 * statistics.median(ts), 2)
         units["threads_%d" % threads] = row
         log("cpu oracle, %d threads: %s" % (threads, row))
-    oracle.set_threads(nproc)
     b = sem_dt() * om.galerkin(oracle.SCHRO_F, p0, V)
+    # one MINRES iteration at each thread count: a 6-iteration stage solve
+    # minus a 2-iteration one (setup and the first operator apply cancel),
+    # median of 5
+    for threads in sorted({nproc, 1}, reverse=True):
+        oracle.set_threads(threads)
+        ts = []
+        for _ in range(5):
+            t0 = time.perf_counter()
+            om.stage_solve(V, scheme_alpha, b, None, 2, tol)
+            t1 = time.perf_counter()
+            om.stage_solve(V, scheme_alpha, b, None, 6, tol)
+            t2 = time.perf_counter()
+            ts.append(((t2 - t1) - (t1 - t0)) / 4.0)
+        units["threads_%d" % threads]["minres_iteration_ms_median5"] = round(1000 * statistics.median(ts), 2)
+        log("cpu oracle, %d threads: one MINRES iteration %.2f ms" % (threads, 1000 * statistics.median(ts)))
+    oracle.set_threads(nproc)
     t0 = time.perf_counter()
     zo, ok, it, rr = om.stage_solve(V, scheme_alpha, b, None, 100000, tol)
     tc = time.perf_counter() - t0
@@ -245,8 +265,9 @@ def cpu_baseline(args, xy, q, p, V, re, im, scheme_alpha, tol):
     log("cpu oracle: stage solve %.2f s, %d iterations" % (tc, it))
     return {"value": round(1.0 / tc, 4), "unit": "stage solves/s", "cores": nproc, "kind": "port",
             "sample": "one first-stage solve of the same workload at tol %g (%d MINRES iterations), "
-                      "%d threads; units: glaplace over all elements, Res apply (galerkin_op), medians of 5"
-                      % (tol, it, nproc),
+                      "%d threads; units at %d and 1 threads: glaplace over all elements, Res apply "
+                      "(galerkin_op), one MINRES iteration, medians of 5; one step = %d such stage solves"
+                      % (tol, it, nproc, nproc, 5 if scheme_alpha == 0.25 * sem_dt() else 3),
             "units": units}
 
 
@@ -475,6 +496,7 @@ def main():
     ap.add_argument("--instances", type=int, default=64)
     ap.add_argument("--batch-steps", type=int, default=100)
     ap.add_argument("--batch-tol", type=float, default=1e-8)
+    ap.add_argument("--batch-precision", default="mixed", choices=["mixed", "fp64"])
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on one GPU only: the hot path is single-GPU here (multi-GPU sharding is a design "

@@ -679,9 +679,11 @@ static Results minres_fused1(sem_mesh* m, int kind, const T* cf, double alpha, c
     if (!cap)
         for (int q = 0; q < 3; ++q) launch_fill<T>(W[q], T(0), n, s);
     wdot_dev<T>(m, y0, y0, nc, 0);
-    launch_minres_init(m->sc, ni, 0, maxit, o.tol, s);
     if (o.rel_to_b) wdot_dev<T>(m, b, b, nc, 1);
-    patch_instances(m, o);
+    // the stop test's atol and reference norm go in with the start; only
+    // per-instance options need a host round trip
+    launch_minres_init(m->sc, ni, 0, maxit, o.tol, s, o.atol, o.rel_to_b ? 1 : -1);
+    if (o.frozen || o.inst_tol || o.inst_maxit) patch_instances(m, o);
     const int ti = sizeof(T) == 8 ? 0 : 1;
     const StepShape shp = m->shape[ti];
     const int slot = o.hint_slot >= 0 ? std::min(o.hint_slot, 7) : ti;
@@ -843,22 +845,12 @@ static Results minres_fused1(sem_mesh* m, int kind, const T* cf, double alpha, c
         // x = x0 + sum_j c_j y_{j-1}: with the records of iterations 1..k, the
         // coefficient of w_j in x is t_j = phi_j - t_{j+1} delta_{j+1} / gamma_{j+1}
         // - t_{j+2} eps_{j+2} / gamma_{j+2}, and w_j holds y_{j-1} / (beta_j gamma_j)
+        // (on the device: no host round trip)
         const int k = m->h_sc[0].iter;
         if (k > 0) {
-            std::vector<double> r(5 * (size_t)k), t(k + 2, 0.0), c(k);
-            SEM_HIP(hipMemcpyAsync(r.data(), rec, r.size() * sizeof(double), hipMemcpyDeviceToHost, s));
-            SEM_HIP(hipStreamSynchronize(s));
-            for (int j = k; j >= 1; --j) {
-                double v = r[5 * (j - 1) + 4];
-                if (j + 1 <= k) v -= t[j + 1] * r[5 * j + 1] * r[5 * j + 3];
-                if (j + 2 <= k) v -= t[j + 2] * r[5 * (j + 1) + 1] * r[5 * (j + 1) + 2];
-                t[j] = v;
-                c[j - 1] = v * r[5 * (j - 1) + 1] * r[5 * (j - 1)];
-            }
-            double* dc = m->buf<double>("mr_coef", (size_t)k);
-            SEM_HIP(hipMemcpyAsync(dc, c.data(), (size_t)k * sizeof(double), hipMemcpyHostToDevice, s));
+            double* dc = m->buf<double>("mr_coef", 2 * (size_t)cap + 2);
+            launch_basis_coef(rec, k, dc, s);
             launch_basis_combine<T>(x, !x_zero, Bv, n, n, dc, k, s);
-            SEM_HIP(hipStreamSynchronize(s));  // c lives on this stack frame
         } else if (x_zero) {
             launch_fill<T>(x, T(0), n, s);
         }
@@ -1334,7 +1326,15 @@ static Results solve_mixed(sem_mesh* m, int kind, const double* cf64, const floa
     // target, sweeps and inner iteration budget; an instance that has met its
     // target (or run out) sits out the later sweeps, its x untouched
     const int ni = m->n_inst;
-    std::vector<double> rn = wdot_inst<double>(m, r, r, nc), bnorm = wdot_inst<double>(m, b, b, nc);
+    // ||r||^2 and ||b||^2 of every instance, one host read for both
+    wdot_dev<double>(m, r, r, nc, 5);
+    wdot_dev<double>(m, b, b, nc, 6);
+    m->read_scalars();
+    std::vector<double> rn(ni), bnorm(ni);
+    for (int k = 0; k < ni; ++k) {
+        rn[k] = m->h_sc[k].red[5];
+        bnorm[k] = m->h_sc[k].red[6];
+    }
     std::vector<double> target(ni), itol(ni);
     std::vector<long> inner(ni, 0);
     std::vector<int> sweeps(ni, 0), imax(ni, 0);

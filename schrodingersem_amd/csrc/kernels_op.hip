@@ -833,7 +833,13 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 3 : 1, 8))) void k_minres_fu
     static_assert(MF == 0 || sizeof(T) == 4, "MFMA contractions serve the FP32 step");
     __shared__ T lds[MF ? 1 : NW][2][MF ? 1 : S::LDS];
     __shared__ float mfs[MF ? NW : 1][MF ? 64 * kMfPitch : 1];
-    __shared__ T xch[NC == 2 ? NW : 1][64 * N];  // component exchange (Res)
+    // component exchange (Res): with VALU contractions it lives in the wave's
+    // own transpose buffers (2 S::LDS >= 64 N values), free until the
+    // stiffness apply that follows the exchange's second barrier; the LDS a
+    // workgroup holds decides how many fit on a CU
+    constexpr bool XALIAS = MF == 0;
+    __shared__ T xch_own[(NC == 2 && !XALIAS) ? NW : 1][(NC == 2 && !XALIAS) ? 64 * N : 1];
+    static_assert(!XALIAS || 2 * S::LDS >= 64 * N, "exchange does not fit the transpose buffers");
     __shared__ double wpart[3][NW];
     __shared__ double red[12];
     const int wv = threadIdx.x >> 6;
@@ -930,8 +936,8 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 3 : 1, 8))) void k_minres_fu
                 for (int i = 0; i < N; ++i) sv[i] = act ? (T)a.dscale[base + (size_t)i * S::KN] : T(0);
             }
             if constexpr (NC == 2) {
-                T* mine = xch[wv];
-                T* other = xch[sub * NC + (1 - w)];
+                T* mine = XALIAS ? &lds[wv][0][0] : xch_own[wv];
+                T* other = XALIAS ? &lds[sub * NC + (1 - w)][0][0] : xch_own[sub * NC + (1 - w)];
 #pragma unroll
                 for (int i = 0; i < N; ++i) mine[i * 64 + lane] = scl ? yc[i] * sv[i] : yc[i];
                 __syncthreads();

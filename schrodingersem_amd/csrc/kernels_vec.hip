@@ -166,6 +166,28 @@ void launch_lincomb(const LinComb<T>& a, hipStream_t s)
 template void launch_lincomb<double>(const LinComb<double>&, hipStream_t);
 template void launch_lincomb<float>(const LinComb<float>&, hipStream_t);
 
+// one thread: the backward recurrence is sequential and k is a few hundred
+__global__ __launch_bounds__(64) void k_basis_coef(const double* __restrict__ rec, int k, double* __restrict__ coef)
+{
+#pragma clang fp contract(off)
+    if (threadIdx.x != 0) return;
+    double t1 = 0.0, t2 = 0.0;  // t_{j+1}, t_{j+2}
+    for (int j = k; j >= 1; --j) {
+        const double* r = rec + 5 * (size_t)(j - 1);
+        double v = r[4];
+        if (j + 1 <= k) v -= t1 * rec[5 * (size_t)j + 1] * rec[5 * (size_t)j + 3];
+        if (j + 2 <= k) v -= t2 * rec[5 * (size_t)(j + 1) + 1] * rec[5 * (size_t)(j + 1) + 2];
+        t2 = t1;
+        t1 = v;
+        coef[j - 1] = v * r[1] * r[0];
+    }
+}
+
+void launch_basis_coef(const double* rec, int k, double* coef, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_basis_coef, dim3(1), dim3(64), 0, s, rec, k, coef);
+}
+
 // x (+)= sum_j coef[j] * basis_j: each thread accumulates its entries in FP64
 // over the stored vectors in j order (eight loads in flight per round)
 template <typename T>
@@ -402,12 +424,12 @@ void launch_fin_sum_inst(const double* partial, const InstTable& it, KrylovScala
 
 // --------------------------------------------------------- MINRES scalars --
 // minres.hpp:24-44 given red[slot] = dot(r1, r1); one workgroup per instance
-__global__ void k_minres_init(KrylovScalars* scs, int slot, int maxit, double tol)
+__global__ void k_minres_init(KrylovScalars* scs, int slot, int maxit, double tol, double atol, int bref_slot)
 {
     if (threadIdx.x != 0) return;
     KrylovScalars* sc = scs + blockIdx.x;
     const double r = sc->red[slot];
-    sc->b_norm = sqrt(r);
+    sc->b_norm = sqrt(bref_slot >= 0 ? sc->red[bref_slot] : r);
     const double beta1 = sqrt(r);
     sc->beta = beta1;
     sc->beta_old = 0;
@@ -419,7 +441,7 @@ __global__ void k_minres_init(KrylovScalars* scs, int slot, int maxit, double to
     sc->cs = -1;
     sc->sn = 0;
     sc->tol = tol;
-    sc->atol = tol;
+    sc->atol = atol >= 0 ? atol : tol;
     sc->alpha = 0;
     sc->ib = 1.0 / beta1;
     sc->cr1 = 0;
@@ -431,9 +453,10 @@ __global__ void k_minres_init(KrylovScalars* scs, int slot, int maxit, double to
     sc->done = (maxit < 1) ? 1 : 0;
 }
 
-void launch_minres_init(KrylovScalars* sc, int n_inst, int slot, int maxit, double tol, hipStream_t s)
+void launch_minres_init(KrylovScalars* sc, int n_inst, int slot, int maxit, double tol, hipStream_t s, double atol,
+                        int bref_slot)
 {
-    hipLaunchKernelGGL(k_minres_init, dim3(n_inst), dim3(64), 0, s, sc, slot, maxit, tol);
+    hipLaunchKernelGGL(k_minres_init, dim3(n_inst), dim3(64), 0, s, sc, slot, maxit, tol, atol, bref_slot);
 }
 
 // Final pending w/x update of every instance whose last step left one

@@ -128,12 +128,21 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, const float* 
     }
 }
 
-// out[i] = sum over s of slices[s][i] in slice order
+// out[i] = sum over s of slices[s][i] in slice order (eight loads in flight,
+// the additions in slice order)
 __global__ __launch_bounds__(256) void k_sum_slices(const float* __restrict__ sl, int ns, size_t n, float* out)
 {
     for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
         float s = 0.f;
-        for (int z = 0; z < ns; ++z) s += sl[(size_t)z * n + i];
+        int z = 0;
+        for (; z + 8 <= ns; z += 8) {
+            float v[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) v[u] = sl[(size_t)(z + u) * n + i];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) s += v[u];
+        }
+        for (; z < ns; ++z) s += sl[(size_t)z * n + i];
         out[i] = s;
     }
 }
@@ -146,7 +155,15 @@ __global__ __launch_bounds__(256) void k_colsum_part(const float* __restrict__ X
     if (c >= cols) return;
     const int r0 = blockIdx.y * rb, r1 = min(rows, r0 + rb);
     float s = 0.f;
-    for (int r = r0; r < r1; ++r) s += X[(size_t)r * cols + c];
+    int r = r0;
+    for (; r + 8 <= r1; r += 8) {  // eight loads in flight, additions in row order
+        float v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = X[(size_t)(r + u) * cols + c];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) s += v[u];
+    }
+    for (; r < r1; ++r) s += X[(size_t)r * cols + c];
     part[(size_t)blockIdx.y * cols + c] = s;
 }
 
@@ -428,7 +445,7 @@ void Mlp::backprop(int rows)
     const float *W2 = params + off[2], *W3 = params + off[4];
     float *gW1 = grads + off[0], *gb1 = grads + off[1], *gW2 = grads + off[2], *gb2 = grads + off[3],
           *gW3 = grads + off[4], *gb3 = grads + off[5];
-    const int split = std::min(kSplits, std::max(1, rows / 512));
+    const int split = std::min(kSplits, std::max(1, rows / 1024));
     // layer 3
     gemm(true, false, h2, out_pad, rows, A2, h2, dY, out_pad, gW3, out_pad, nullptr, split);
     colsum(dY, rows, out_pad, gb3);

@@ -295,6 +295,11 @@ void launch_minres_fused(int N, int kind, const MinresFusedArgs<T>& a, const T* 
 // at its start, one workgroup per instance (reads in[k], writes out[k])
 void launch_minres_fused_close(const InstTable& it, const KrylovScalars* in, KrylovScalars* out, const double* sums,
                                int step, hipStream_t s, double* rec = nullptr);
+// coefficients of the stored-basis MINRES solution from the records of
+// iterations 1..k (rec[5 (j-1) ..]: 1/beta_j, 1/gamma_j, eps_j, delta_j, phi_j):
+// t_j = phi_j - t_{j+1} delta_{j+1} / gamma_{j+1} - t_{j+2} eps_{j+2} / gamma_{j+2},
+// coef[j-1] = t_j / (beta_j gamma_j); coef holds k + 2 doubles of workspace after the k results
+void launch_basis_coef(const double* rec, int k, double* coef, hipStream_t s);
 // x[i] (+)= sum_j coef[j] * basis[j * stride + i], i < n, the sum in FP64 in j order
 template <typename T>
 void launch_basis_combine(T* x, bool accumulate, const T* basis, size_t stride, size_t n, const double* coef, int k,

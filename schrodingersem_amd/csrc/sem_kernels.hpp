@@ -77,7 +77,11 @@ void launch_fin_sum(const double* partial, int n, double* dst, hipStream_t s);
 void launch_fin_sum_inst(const double* partial, const InstTable& it, KrylovScalars* sc, int slot, hipStream_t s);
 
 // MINRES set-up of every instance from sc[k].red[slot] = dot(r1, r1)
-void launch_minres_init(KrylovScalars* sc, int n_inst, int slot, int maxit, double tol, hipStream_t s);
+// the MINRES start of minres.hpp:24-40 from ||r1||^2 in red[slot]; atol < 0
+// means atol = tol (minres.hpp:92); bref_slot >= 0 takes the stop test's
+// reference norm from red[bref_slot] instead (||b||, the test with x0 = 0)
+void launch_minres_init(KrylovScalars* sc, int n_inst, int slot, int maxit, double tol, hipStream_t s,
+                        double atol = -1.0, int bref_slot = -1);
 
 template <typename T>
 struct MinresFlushArgs {

@@ -10,8 +10,8 @@ namespace sem {
 // Row-major activations, one row per element. X: [rows x in_pad],
 // Y: [rows x out_pad]; T holds training targets with Y's layout.
 struct Mlp {
-    static constexpr int kSplits = 16;   // row-block split of the weight-gradient GEMMs
-    static constexpr int kColRows = 256; // rows per block of the bias-gradient column sums
+    static constexpr int kSplits = 32;   // row-block split of the weight-gradient GEMMs
+    static constexpr int kColRows = 64;  // rows per block of the bias-gradient column sums
     int in = 0, h1 = 0, h2 = 0, out = 0, in_pad = 0, out_pad = 0;
     size_t off[6] = {0, 0, 0, 0, 0, 0};  // W1, b1, W2, b2, W3, b3 in the parameter block
     size_t nparams = 0;

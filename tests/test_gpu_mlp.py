@@ -49,7 +49,7 @@ def test_forward_matches_numpy(net):
 def test_gradients_match_numpy(net):
     from oracle import mlp_oracle as mo
     p = randomize(net, 3)
-    X, T = data(net, 1500, 4)  # several row blocks and split slices
+    X, T = data(net, 4500, 4)  # several column-sum row blocks and four split-K slices (ragged last one)
     loss, g = net.gradients(X, T)
     lo, go = mo.gradients(p, X, T, net.dims, net.in_pad, net.out_pad, net.offsets)
     assert abs(loss - lo) < 1e-4 * lo
