@@ -492,7 +492,7 @@ static void setup_device(sem_mesh* m)
     SEM_HIP(hipMalloc(&m->sc_alt, scb));
     SEM_HIP(hipMemset(m->sc_alt, 0, scb));
     SEM_HIP(hipHostMalloc(&m->h_sc, scb, hipHostMallocDefault));
-    m->n_partial = 2 * L.nchunks + m->dss_nb + 1024;
+    m->n_partial = 3 * L.nchunks + m->dss_nb + 1024;
     SEM_HIP(hipMalloc(&m->partial, (size_t)m->n_partial * sizeof(double)));
 }
 
@@ -530,8 +530,30 @@ static void apply_op(sem_mesh* m, int kind, const T* in, T* out, const T* cf, do
     a.post = (T)post;
     a.len = m->len();
     a.nchunks = m->L.nchunks;
+    const int nc = op_ncomp(kind);
+    if (post > 0) {
+        // the element operator into scratch, then the DSS as one gather per
+        // column into out (every entry written once, coalesced); the same bits
+        // as the pair/corner DSS below for a positive post
+        T* raw = m->buf<T>(sizeof(T) == 8 ? "ap_raw_d" : "ap_raw_f", (size_t)nc * m->len());
+        a.out = raw;
+        launch_element_op_d<T>(m->L.N, kind, a, m->dmat<T>(), m->stream);
+        GatherArgs<T> g{};
+        g.in = raw;
+        g.out = out;
+        g.post = (T)post;
+        g.len = m->len();
+        g.nchunks = m->L.nchunks;
+        g.ncomp = nc;
+        g.eside = m->eside;
+        g.ecq = m->ecq;
+        g.c_off = m->c_off;
+        g.c_pos = m->c_pos;
+        launch_gather_assemble<T>(m->L.N, g, m->stream);
+        return;
+    }
     launch_element_op_d<T>(m->L.N, kind, a, m->dmat<T>(), m->stream);
-    DssArgs<T> d = dss_args<T>(m, out, op_ncomp(kind));
+    DssArgs<T> d = dss_args<T>(m, out, nc);
     d.post[0] = d.post[1] = (T)post;
     launch_dss<T>(DSS_SUM_ZB, d, m->stream);
 }
@@ -566,6 +588,13 @@ static void wdot_dev(sem_mesh* m, const T* a, const T* b, int ncomp, int slot)
 {
     launch_wdot<T>(a, b, m->len(), ncomp, m->L, m->obits, m->partial, m->stream);
     launch_fin_sum_inst(m->partial, m->it(), m->sc, slot, m->stream);
+}
+
+// <a,a>, <b,b>, <a,b> of every instance into sc[k].red[slot0 .. slot0 + 2] (device), one pass
+static void wdot3_dev(sem_mesh* m, const double* a, const double* b, int ncomp, int slot0)
+{
+    launch_wdot3<double>(a, b, m->len(), ncomp, m->L, m->obits, m->partial, m->stream);
+    launch_fin_sum3_inst(m->partial, m->it(), m->sc, m->L.nchunks, slot0, m->stream);
 }
 
 template <typename T>
@@ -1301,12 +1330,13 @@ static double inner_tol_floor(const sem_mesh* m) { return m->contraction == 1 ? 
 // (or tol*||b|| + atol when rel_to_b), the reference's test applied to the
 // true residual instead of MINRES's recurrence estimate.
 static Results solve_mixed(sem_mesh* m, int kind, const double* cf64, const float* cf32, double alpha, const double* b,
-                           double* x, bool x_zero, const KrylovOpts& o, double inner_tol, const double* r_given = nullptr)
+                           double* x, bool x_zero, const KrylovOpts& o, double inner_tol, double* r_given = nullptr)
 {
     const int nc = op_ncomp(kind);
     const size_t n = (size_t)nc * m->len();
     hipStream_t s = m->stream;
-    double* r = m->buf<double>("mx_r", n);
+    // the caller's buffer holding b - A x becomes the residual buffer
+    double* r = r_given ? r_given : m->buf<double>("mx_r", n);
     double* Ax = m->buf<double>("mx_Ax", n);
     float* r32 = m->buf<float>("mx_r32", n);
     float* d32 = m->buf<float>("mx_d32", n);
@@ -1314,9 +1344,8 @@ static Results solve_mixed(sem_mesh* m, int kind, const double* cf64, const floa
         launch_fill<double>(x, 0.0, n, s);
         launch_cast<double, double>(r, b, n, s);
     } else if (r_given) {
-        // the caller already holds b - A x (the learned guess's combination);
-        // the stop test after every sweep still uses a freshly computed residual
-        launch_cast<double, double>(r, r_given, n, s);
+        // the caller already holds b - A x (the learned guess's combination) in
+        // r; the stop test after every sweep still uses a freshly computed residual
     } else {
         apply_op<double>(m, kind, x, Ax, cf64, alpha);
         lincomb<double>(m, r, 0, {{1.0, b}, {-1.0, Ax}}, n);
@@ -1326,14 +1355,13 @@ static Results solve_mixed(sem_mesh* m, int kind, const double* cf64, const floa
     // target, sweeps and inner iteration budget; an instance that has met its
     // target (or run out) sits out the later sweeps, its x untouched
     const int ni = m->n_inst;
-    // ||r||^2 and ||b||^2 of every instance, one host read for both
-    wdot_dev<double>(m, r, r, nc, 5);
-    wdot_dev<double>(m, b, b, nc, 6);
+    // ||r||^2 and ||b||^2 of every instance in one pass and one host read
+    wdot3_dev(m, r, b, nc, 4);
     m->read_scalars();
     std::vector<double> rn(ni), bnorm(ni);
     for (int k = 0; k < ni; ++k) {
-        rn[k] = m->h_sc[k].red[5];
-        bnorm[k] = m->h_sc[k].red[6];
+        rn[k] = m->h_sc[k].red[4];
+        bnorm[k] = m->h_sc[k].red[5];
     }
     std::vector<double> target(ni), itol(ni);
     std::vector<long> inner(ni, 0);
@@ -1539,7 +1567,7 @@ static void stage_rhs(Stepper& st, const double* q, double* out)
 }
 
 static Results solve_res(Stepper& st, double alpha, const double* b, double* x, bool zero, bool rel_to_b,
-                         const double* r_given = nullptr)
+                         double* r_given = nullptr)
 {
     sem_mesh* m = st.m;
     KrylovOpts ko;
@@ -1635,7 +1663,7 @@ static Results stage_solve(Stepper& st, double alpha, const double* b, double* z
     // the fallback's trajectory); afterwards it is frozen and proposes guesses
     const bool learned = st.opts.guess == 2 && st.have_prev2;
     const bool training = learned && st.stats.steps < st.learn_steps;
-    const double* r_given = nullptr;  // b - Res z of the start, when already computed
+    double* r_given = nullptr;  // b - Res z of the start, when already computed (solve_mixed works in it)
     if (learned) {
         // extrapolation plus the network's correction, made consistent
         lincomb<double>(m, st.ext, 1, {{2.0, st.zprev[stage]}, {-1.0, st.zprev2[stage]}}, st.n);
@@ -1671,9 +1699,7 @@ static Results stage_solve(Stepper& st, double alpha, const double* b, double* z
                 double* rz = m->buf<double>("guard_r", st.n);
                 apply_op<double>(m, K_RES, z, Az, st.V.c64, alpha);
                 lincomb<double>(m, rz, 0, {{1.0, b}, {-1.0, Az}}, st.n);
-                wdot_dev<double>(m, rz, rz, 2, 2);
-                wdot_dev<double>(m, st.rext, st.rext, 2, 3);
-                wdot_dev<double>(m, st.rext, rz, 2, 4);
+                wdot3_dev(m, rz, st.rext, 2, 2);  // red[2] = ||r_z||^2, red[3] = ||r_ext||^2, red[4] = <r_z, r_ext>
                 m->read_scalars();
                 const size_t cl = (size_t)m->L.K * m->L.N * m->L.N;
                 for (int k = 0; k < m->n_inst; ++k) {

@@ -1297,6 +1297,36 @@ static void minres_fused_N(int kind, const MinresFusedArgs<T>& a, const T* d, in
     }
 }
 
+// The DSS of an element operator's output as one gather per element column
+// (ColumnGather), writing the assembled vector out of place: one workgroup per
+// chunk, one wave per component.
+template <int N, typename T>
+__global__ __launch_bounds__(128) void k_gather_assemble(GatherArgs<T> a)
+{
+    using S = Shape<N>;
+    const int chunk = blockIdx.x;
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (lane >= S::KN) return;  // no LDS and no barrier below
+    const int el = lane / N, l = lane % N;
+    const size_t base = (size_t)chunk * S::K * N * N + lane;
+    ColumnGather<N, T> g;
+    g.issue(a.in + w * a.len, base, chunk * S::K + el, l, a.eside, a.ecq, a.c_off, a.c_pos);
+    g.finish();
+    const bool side = (l == 0 || l == N - 1);
+    T* o = a.out + w * a.len + base;
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        const bool ring = side || i == 0 || i == N - 1;
+        o[(size_t)i * S::KN] = ring ? a.post * g.v[i] : g.v[i];
+    }
+}
+
+template <int N, typename T>
+static void gather_assemble_N(const GatherArgs<T>& a, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_gather_assemble<N, T>), dim3(a.nchunks), dim3(64 * a.ncomp), 0, s, a);
+}
+
 #define SEM_DISPATCH_N(N, CALL)                                   \
     switch (N) {                                                  \
     case 2: CALL(2); break;                                       \
@@ -1323,6 +1353,15 @@ void launch_element_op_d(int N, int kind, const OpArgs<T>& a, const T* d, hipStr
 #define CALL_EO(n) element_op_N<n, T>(kind, a, d, s)
     SEM_DISPATCH_N(N, CALL_EO)
 #undef CALL_EO
+}
+
+template <typename T>
+void launch_gather_assemble(int N, const GatherArgs<T>& a, hipStream_t s)
+{
+    if (a.ncomp < 1 || a.ncomp > 2 || a.in == a.out) throw std::runtime_error("gather_assemble: bad arguments");
+#define CALL_GA(n) gather_assemble_N<n, T>(a, s)
+    SEM_DISPATCH_N(N, CALL_GA)
+#undef CALL_GA
 }
 
 template <typename T>
@@ -1380,6 +1419,7 @@ void launch_stiff_diag(int N, const double* geo, size_t len, int nchunks, double
 }
 
 template void launch_element_op_d<double>(int, int, const OpArgs<double>&, const double*, hipStream_t);
+template void launch_gather_assemble<double>(int, const GatherArgs<double>&, hipStream_t);
 template void launch_minres_op_d<double>(int, int, const MinresOpArgs<double>&, const double*, int, int, hipStream_t,
                                           hipEvent_t, hipEvent_t);
 template void launch_minres_upd<double>(int, const MinresUpdArgs<double>&, int, int, hipStream_t, hipEvent_t, hipEvent_t);
@@ -1393,6 +1433,7 @@ void launch_minres_fused<double>(int, int, const MinresFusedArgs<double>&, const
 }
 #else
 template void launch_element_op_d<float>(int, int, const OpArgs<float>&, const float*, hipStream_t);
+template void launch_gather_assemble<float>(int, const GatherArgs<float>&, hipStream_t);
 template void launch_minres_op_d<float>(int, int, const MinresOpArgs<float>&, const float*, int, int, hipStream_t,
                                           hipEvent_t, hipEvent_t);
 template void launch_minres_upd<float>(int, const MinresUpdArgs<float>&, int, int, hipStream_t, hipEvent_t, hipEvent_t);

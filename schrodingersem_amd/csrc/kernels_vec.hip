@@ -166,17 +166,26 @@ void launch_lincomb(const LinComb<T>& a, hipStream_t s)
 template void launch_lincomb<double>(const LinComb<double>&, hipStream_t);
 template void launch_lincomb<float>(const LinComb<float>&, hipStream_t);
 
-// one thread: the backward recurrence is sequential and k is a few hundred
-__global__ __launch_bounds__(64) void k_basis_coef(const double* __restrict__ rec, int k, double* __restrict__ coef)
+// one workgroup: the records go to LDS with all 256 threads (up to 1536 of
+// them), then one thread runs the sequential backward recurrence from LDS
+constexpr int kCoefLds = 1536;
+__global__ __launch_bounds__(256) void k_basis_coef(const double* __restrict__ rec, int k, double* __restrict__ coef)
 {
 #pragma clang fp contract(off)
+    __shared__ double sr[5 * kCoefLds];
+    const bool staged = k <= kCoefLds;
+    if (staged) {
+        for (int q = threadIdx.x; q < 5 * k; q += 256) sr[q] = rec[q];
+        __syncthreads();
+    }
     if (threadIdx.x != 0) return;
+    const double* rr = staged ? sr : rec;
     double t1 = 0.0, t2 = 0.0;  // t_{j+1}, t_{j+2}
     for (int j = k; j >= 1; --j) {
-        const double* r = rec + 5 * (size_t)(j - 1);
+        const double* r = rr + 5 * (size_t)(j - 1);
         double v = r[4];
-        if (j + 1 <= k) v -= t1 * rec[5 * (size_t)j + 1] * rec[5 * (size_t)j + 3];
-        if (j + 2 <= k) v -= t2 * rec[5 * (size_t)(j + 1) + 1] * rec[5 * (size_t)(j + 1) + 2];
+        if (j + 1 <= k) v -= t1 * rr[5 * (size_t)j + 1] * rr[5 * (size_t)j + 3];
+        if (j + 2 <= k) v -= t2 * rr[5 * (size_t)(j + 1) + 1] * rr[5 * (size_t)(j + 1) + 2];
         t2 = t1;
         t1 = v;
         coef[j - 1] = v * r[1] * r[0];
@@ -185,7 +194,7 @@ __global__ __launch_bounds__(64) void k_basis_coef(const double* __restrict__ re
 
 void launch_basis_coef(const double* rec, int k, double* coef, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_basis_coef, dim3(1), dim3(64), 0, s, rec, k, coef);
+    hipLaunchKernelGGL(k_basis_coef, dim3(1), dim3(256), 0, s, rec, k, coef);
 }
 
 // x (+)= sum_j coef[j] * basis_j: each thread accumulates its entries in FP64
@@ -373,6 +382,48 @@ template void launch_wdot<double>(const double*, const double*, size_t, int, con
 template void launch_wdot<float>(const float*, const float*, size_t, int, const Layout&, const uint8_t*, double*,
                                  hipStream_t);
 
+// <a, a>, <b, b> and <a, b> in one pass, each accumulated as k_wdot does (the
+// same bits as three k_wdot launches); partials at partial[q * nch + chunk]
+template <typename T>
+__global__ __launch_bounds__(64) void k_wdot3(const T* a, const T* b, size_t len, int ncomp, int N, int K,
+                                             const uint8_t* obits, double* partial, int nch)
+{
+    const int lane = threadIdx.x & 63, KN = K * N;
+    double paa = 0, pbb = 0, pab = 0;
+    if (lane < KN) {
+        const int el = lane / N, j = lane % N;
+        const int e = blockIdx.x * K + el;
+        const unsigned bits = obits[e];
+        const size_t base = (size_t)blockIdx.x * K * N * N + lane;
+        for (int c = 0; c < ncomp; ++c)
+            for (int i = 0; i < N; ++i) {
+                const size_t k = (size_t)c * len + base + (size_t)i * KN;
+                const double x = (double)a[k], y = (double)b[k], w = owner_weight(i, j, N, bits);
+                paa += w * (x * x);
+                pbb += w * (y * y);
+                pab += w * (x * y);
+            }
+    }
+    paa = wsum(paa);
+    pbb = wsum(pbb);
+    pab = wsum(pab);
+    if (lane == 0) {
+        partial[blockIdx.x] = paa;
+        partial[(size_t)nch + blockIdx.x] = pbb;
+        partial[2 * (size_t)nch + blockIdx.x] = pab;
+    }
+}
+
+template <typename T>
+void launch_wdot3(const T* a, const T* b, size_t len, int ncomp, const Layout& L, const uint8_t* obits, double* partial,
+                  hipStream_t s)
+{
+    hipLaunchKernelGGL((k_wdot3<T>), dim3(L.nchunks), dim3(64), 0, s, a, b, len, ncomp, L.N, L.K, obits, partial,
+                       L.nchunks);
+}
+template void launch_wdot3<double>(const double*, const double*, size_t, int, const Layout&, const uint8_t*, double*,
+                                   hipStream_t);
+
 // plain Euclidean dot (generic solvers), grid-stride with fixed grid
 template <typename T>
 __global__ __launch_bounds__(256) void k_edot(const T* a, const T* b, size_t n, double* partial)
@@ -420,6 +471,25 @@ __global__ __launch_bounds__(256) void k_fin_sum_inst(const double* partial, Ins
 void launch_fin_sum_inst(const double* partial, const InstTable& it, KrylovScalars* sc, int slot, hipStream_t s)
 {
     hipLaunchKernelGGL(k_fin_sum_inst, dim3(it.n_inst), dim3(256), 0, s, partial, it, sc, slot);
+}
+
+// three arrays of per-chunk partials (stride nch) at once, each summed as
+// k_fin_sum_inst sums it, into red[slot0 + q]
+__global__ __launch_bounds__(256) void k_fin_sum3_inst(const double* partial, InstTable it, KrylovScalars* sc, int nch,
+                                                       int slot0)
+{
+    __shared__ double sh[12];
+    const int k = blockIdx.x, p0 = it.inst_off[k];
+    double out[3];
+    group_sum3<16>(partial + p0, nch, it.inst_off[k + 1] - p0, sh, 4, out);
+    if (threadIdx.x == 0)
+        for (int q = 0; q < 3; ++q) sc[k].red[slot0 + q] = out[q];
+}
+
+void launch_fin_sum3_inst(const double* partial, const InstTable& it, KrylovScalars* sc, int nch, int slot0,
+                          hipStream_t s)
+{
+    hipLaunchKernelGGL(k_fin_sum3_inst, dim3(it.n_inst), dim3(256), 0, s, partial, it, sc, nch, slot0);
 }
 
 // --------------------------------------------------------- MINRES scalars --

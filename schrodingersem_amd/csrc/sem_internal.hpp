@@ -275,6 +275,25 @@ struct PcgOpArgs {
 // (column-major, N*N values) copied into the kernel-argument segment.
 template <typename T>
 void launch_element_op_d(int N, int kind, const OpArgs<T>& a, const T* d, hipStream_t s);
+// Assembled, consistent form of an element operator's output (the DSS as a
+// gather): every ring entry sums its node's copies in the reference's order
+// (global_sum.hpp:28-34, :54-63) and is scaled by post, interior entries are
+// copied, Dirichlet entries are zero. out must not alias in.
+template <typename T>
+struct GatherArgs {
+    const T* in;
+    T* out;
+    T post;
+    size_t len;
+    int nchunks;
+    int ncomp;
+    const int* eside;
+    const int4* ecq;
+    const int* c_off;
+    const int* c_pos;
+};
+template <typename T>
+void launch_gather_assemble(int N, const GatherArgs<T>& a, hipStream_t s);
 // e0/e1 (optional): events that take the kernel's own start and end timestamps
 template <typename T>
 void launch_minres_op_d(int N, int kind, const MinresOpArgs<T>& a, const T* d, int cpb, int grid, hipStream_t s,

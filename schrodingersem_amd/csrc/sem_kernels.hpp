@@ -80,6 +80,13 @@ void launch_fin_sum_inst(const double* partial, const InstTable& it, KrylovScala
 // the MINRES start of minres.hpp:24-40 from ||r1||^2 in red[slot]; atol < 0
 // means atol = tol (minres.hpp:92); bref_slot >= 0 takes the stop test's
 // reference norm from red[bref_slot] instead (||b||, the test with x0 = 0)
+// <a,a>, <b,b>, <a,b> per chunk in one pass (partials at q * nchunks + chunk), and
+// their per-instance fixed-order sums into red[slot0 .. slot0 + 2]
+template <typename T>
+void launch_wdot3(const T* a, const T* b, size_t len, int ncomp, const Layout& L, const uint8_t* obits, double* partial,
+                  hipStream_t s);
+void launch_fin_sum3_inst(const double* partial, const InstTable& it, KrylovScalars* sc, int nch, int slot0,
+                          hipStream_t s);
 void launch_minres_init(KrylovScalars* sc, int n_inst, int slot, int maxit, double tol, hipStream_t s,
                         double atol = -1.0, int bref_slot = -1);
 
