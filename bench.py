@@ -28,7 +28,7 @@ residual ||b - A x|| <= tol ||b|| + tol, the reference's test with a zero
 guess; fp64 uses the reference's own test (its guesses are zero).
 
 Also measured: live HIP-event timing of the dominant kernel (the fused FP32
-Lanczos-step kernel of mixed_learned, one Lanczos step in four sampled) ->
+Lanczos-step kernel of mixed_learned, one Lanczos step in sixteen sampled) ->
 roofline (traffic from the committed rocprofv3 PMC summary of this workload);
 glaplace GB/s (FP64 and FP32) on a 512 x 512, degree-7 mesh (671 MB of FP64
 algorithmic traffic, above the 256 MB Infinity Cache), on the 64 x 64 degree-8
@@ -41,7 +41,9 @@ over the mode's whole wall time per inner iteration).
 
 Other workloads: --workload dirk4s3 (128 x 128 quads, degree 8, DIRK4S3, the
 same arms), --workload batched_bdf2 (64 uneven BDF2 instances, batched vs one
-at a time over the same steps), --workload glaplace (the glaplace apply alone).
+at a time over the same steps), --workload glaplace (the glaplace apply alone),
+--workload dirk4s5_full / dirk4s3_full (the SDIRK workloads over their whole
+500- and 200-step horizons from t = 0, training window included).
 --arms restricts the SDIRK arms (profiling runs).
 """
 import argparse
@@ -448,6 +450,82 @@ def bench_sdirk(sem, args, workload):
     return 0 if line["all_modes_within_1e-9"] is not False else 1
 
 
+def bench_sdirk_full(sem, args, workload):
+    """The SDIRK workloads over their whole stated horizon, every arm from t = 0.
+    dirk4s5_full: 500 DIRK4S5 steps on 256 x 256 quads at degree 7 - the
+    learned guess (network trained on steps 1-50, frozen for 450) against the
+    extrapolated-guess fallback, both mixed precision to the same tolerance;
+    wall time end to end (training included), inner iterations of the first 50
+    and of the last 450 steps, and the two runs' psi against each other.
+    dirk4s3_full: 200 DIRK4S3 steps on 128 x 128 quads at degree 8 - plain
+    FP64, mixed + extrapolated and mixed + learned against an FP64 run at
+    tolerance 1e-15: relative L2 error and norm drift of each at the end."""
+    from schrodingersem_amd import configs
+    if workload == "dirk4s5_full":
+        scheme, n, p, gen, nsteps = 2, args.n5, args.p5, configs.config5, 500
+        wname = "learned_dirk4s5_%dx%d_p%d_500_steps" % (n, n, p)
+        arms = [("mixed_learned", sem.PREC_MIXED, sem.GUESS_LEARNED, args.tol),
+                ("mixed_ext", sem.PREC_MIXED, sem.GUESS_EXTRAPOLATED, args.tol)]
+    else:
+        scheme, n, p, gen, nsteps = 1, args.n3, args.p3, configs.config3, 200
+        wname = "schrodinger_dirk4s3_%dx%d_p%d_200_steps" % (n, n, p)
+        arms = [("truth", sem.PREC_FP64, sem.GUESS_REFERENCE, args.tol_truth),
+                ("fp64", sem.PREC_FP64, sem.GUESS_REFERENCE, args.tol),
+                ("mixed_ext", sem.PREC_MIXED, sem.GUESS_EXTRAPOLATED, args.tol),
+                ("mixed_learned", sem.PREC_MIXED, sem.GUESS_LEARNED, args.tol)]
+    _DT[0] = configs.DT_REF
+    m, xy, q, V, re, im = gen(sem, n, p)
+    log("mesh: E=%d N=%d dof=%d, %d steps" % (m.n_elements, m.N, m.dof(), nsteps))
+    for prec in (sem.PREC_FP64, sem.PREC_MIXED):
+        sem.autotune(m, sem.OP_SCHRO_RES, prec, 24)
+    n0 = np.linalg.norm(np.concatenate([re, im]))
+    res = {}
+    for name, prec, guess, tol in arms:
+        opts = sem.SolverOpts(max_iter=100000, tol=tol, dt=sem_dt(), precision=prec, guess=guess,
+                              learn_steps=args.learn_steps)
+        st = sem.Stepper(m, scheme, re, im, V, opts)
+        t_first = advance_timed(st, args.learn_steps, name + " (first %d steps)" % args.learn_steps)
+        s1 = st.stats()
+        t_rest = advance_timed(st, nsteps - args.learn_steps, name + " (rest)")
+        s2 = st.stats()
+        a, b, _ = st.read()
+        st.close()
+        psi = np.concatenate([a, b])
+        res[name] = {"seconds": t_first + t_rest, "seconds_first": t_first, "seconds_rest": t_rest,
+                     "stage_solves": s2["stage_solves"], "iters_first": s1["total_iters"],
+                     "iters_rest": s2["total_iters"] - s1["total_iters"],
+                     "refinements": s2["refinements"], "learned_guess_chosen": s2["learned_guess_chosen"],
+                     "psi": psi, "drift": float(abs(np.linalg.norm(psi) / n0 - 1.0))}
+        log("  %s: %.2f s (%.2f + %.2f), iterations %d + %d" % (name, t_first + t_rest, t_first, t_rest,
+                                                                 s1["total_iters"], s2["total_iters"] - s1["total_iters"]))
+    out = {}
+    truth = res.pop("truth", None)
+    ref = truth["psi"] if truth else res["mixed_ext"]["psi"]
+    for k, r in res.items():
+        out[k] = {"seconds_end_to_end": round(r["seconds"], 3),
+                  "seconds_first_%d_steps" % args.learn_steps: round(r["seconds_first"], 3),
+                  "seconds_last_%d_steps" % (nsteps - args.learn_steps): round(r["seconds_rest"], 3),
+                  "stage_solves_per_s": round(r["stage_solves"] / r["seconds"], 3),
+                  "inner_iters_first_%d_steps" % args.learn_steps: r["iters_first"],
+                  "inner_iters_last_%d_steps" % (nsteps - args.learn_steps): r["iters_rest"],
+                  "refinements": r["refinements"], "learned_guess_chosen": r["learned_guess_chosen"],
+                  ("rel_l2_vs_truth" if truth else "rel_l2_vs_mixed_ext"):
+                      float(np.linalg.norm(r["psi"] - ref) / np.linalg.norm(ref)),
+                  "psi_norm_drift": r["drift"]}
+    head = "mixed_learned"
+    line = {"metric": "SDIRK stage solves/s at 1e-9 rel-L2 over the workload's whole horizon, end to end",
+            "value": out[head]["stage_solves_per_s"], "unit": "stage solves/s", "n_gpus": 1, "steps": nsteps,
+            "higher_is_better": True, "dtype": "f64+f32+bf16", "data": "synthetic",
+            "config": {"workload": wname, "dof": m.dof(), "tol": args.tol, "dt": configs.DT,
+                       "training_steps": args.learn_steps},
+            "modes": out}
+    if truth:
+        line["truth"] = {"tol": args.tol_truth, "psi_norm_drift": truth["drift"], "seconds": round(truth["seconds"], 3)}
+        line["all_modes_within_1e-9"] = bool(all(v["rel_l2_vs_truth"] < 1e-9 for v in out.values()))
+    print(json.dumps(line), flush=True)
+    return 0 if line.get("all_modes_within_1e-9", True) else 1
+
+
 def bench_glaplace(sem, args):
     """glaplace alone: FP64 and FP32 on 512 x 512 quads at degree 7 (671 MB of FP64
     algorithmic traffic, above the Infinity Cache) and FP64 on the 64 x 64,
@@ -476,7 +554,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", default="dirk4s5", choices=["dirk4s5", "dirk4s3", "batched_bdf2", "glaplace"])
+    ap.add_argument("--workload", default="dirk4s5",
+                    choices=["dirk4s5", "dirk4s3", "batched_bdf2", "glaplace", "dirk4s5_full", "dirk4s3_full"])
     ap.add_argument("--arms", default="",
                     help="comma list of SDIRK arms to run (default all; without 'truth' no error is measured)")
     ap.add_argument("--tol", type=float, default=1e-12)
@@ -510,6 +589,8 @@ def main():
         return bench_batched(sem, args)
     if args.workload == "glaplace":
         return bench_glaplace(sem, args)
+    if args.workload.endswith("_full"):
+        return bench_sdirk_full(sem, args, args.workload)
     return bench_sdirk(sem, args, args.workload)
 
 

@@ -374,7 +374,7 @@ int sem_quad_metrics(const double* xs, const double* ys, int n, const double* z,
  * per launch and the algorithmic bytes per launch. */
 int sem_time_element_op(sem_mesh* m, int kind, int precision, int reps, double* us_per_launch, double* bytes_per_launch);
 /* Live timing of the two MINRES Lanczos-step kernels: on = 1 times the
- * launches of one Lanczos step in four (picked by a hash of the step index,
+ * launches of one Lanczos step in sixteen (picked by a hash of the step index,
  * so the sample does not follow the batches between host checks) with HIP
  * events that take the
  * kernels' own start and end timestamps (hipExtLaunchKernel), on = 2 every

@@ -185,7 +185,7 @@ struct sem_mesh {
     int batch = 8;          // Lanczos steps launched between host checks of the stop flag
     int iter_hint[8] = {};  // iterations of the last MINRES solve per hint slot
     // live timing of the two Lanczos-step kernels: 0 off, 1 one Lanczos step
-    // in four (sampled_step), 2 every step
+    // in sixteen (sampled_step), 2 every step
     int timing = 0;
     std::vector<hipEvent_t> ev;
     // slots: 0/1 operator kernel FP64/FP32, 2/3 update kernel FP64/FP32
@@ -646,7 +646,7 @@ static void patch_instances(sem_mesh* m, const KrylovOpts& o)
 // recurrence as the two-launch path; the operator is applied before the 1/beta
 // scaling instead of after, so results agree to rounding. Buffers: y_k in
 // Y[k % 3], A y in Z[step % 2], w_k in W[k % 3], sums in S[step % 2].
-// Kernel timing mode 1 times one Lanczos step in four, picked by a hash of
+// Kernel timing mode 1 times one Lanczos step in sixteen, picked by a hash of
 // the step index so that the sample does not follow the batch structure
 // (the first launch after a host check starts on an idle GPU)
 // Upper bound of the first batch of Lanczos steps launched before a host
@@ -654,7 +654,7 @@ static void patch_instances(sem_mesh* m, const KrylovOpts& o)
 constexpr int kMaxFirstBatch = 2048;
 constexpr int kBasisMax = 4096;  // most Lanczos vectors a stored-basis MINRES solve keeps
 
-static bool sampled_step(int i) { return (((unsigned)i * 2654435761u) >> 20 & 3u) == 0; }
+static bool sampled_step(int i) { return (((unsigned)i * 2654435761u) >> 20 & 15u) == 0; }
 
 // the one-launch Lanczos step runs for precision index ti (0 FP64, 1 FP32)
 static bool fused_step(const sem_mesh* m, int ti) { return m->step_mode[ti] >= 1; }
