@@ -87,6 +87,24 @@ def test_dss_bit_identical(mesh_pairs, sem):
         assert np.array_equal(np.signbit(sem.global_sum(z, m)), np.signbit(o.global_sum(z))), name
 
 
+@pytest.mark.parametrize("kind", [0, 1, 3])
+def test_galerkin_gather_equals_pair_dss(mesh_pairs, sem, kind):
+    # galerkin_op assembles the element operator's output with one gather per
+    # element column; the same apply assembled by the pair/corner DSS
+    # (global_sum, then mask) gives the same bits, the valence-6 corner of
+    # small_mesh included
+    rng = np.random.default_rng(40 + kind)
+    for name, m, o in mesh_pairs:
+        nc = 2 if kind == 3 else 1
+        u = rng.standard_normal(nc * m.size)
+        coef = rng.uniform(0.5, 2.0, m.size) if kind in (1, 3) else None
+        y = sem.galerkin_op(kind, u, m, coef, 0.21)
+        uu = np.concatenate([sem.unmask(u[c * m.size:(c + 1) * m.size], m) for c in range(nc)])
+        e = sem.element_op(kind, uu, m, coef, 0.21)
+        ref = np.concatenate([sem.mask(sem.global_sum(e[c * m.size:(c + 1) * m.size], m), m) for c in range(nc)])
+        assert np.array_equal(y, ref), (name, kind)
+
+
 def test_dss_repeatable(mesh_pairs, sem):
     rng = np.random.default_rng(4)
     name, m, o = mesh_pairs[0]
