@@ -1781,7 +1781,7 @@ static Results stage_solve(Stepper& st, double alpha, const double* b, double* z
             n2 = std::max(n2, 1e-300);
             for (int ep = 0; ep < st.epochs; ++ep) {
                 const bool last = ep + 1 == st.epochs;
-                net.forward(m->L.E);
+                net.forward(m->L.E, true);  // activations for backprop
                 launch_scatter_output(scatter_args(st, stage, g, st.ext, st.s_out[stage]), m->stream);
                 dss<double>(m, DSS_UNMASK, g, 2);
                 dss<double>(m, DSS_ZERO_BND, g, 2);

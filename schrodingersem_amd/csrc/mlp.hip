@@ -35,6 +35,12 @@ __device__ __forceinline__ short f2bf(float f)
     return (short)(u >> 16);
 }
 
+__device__ __forceinline__ float gelu(float x) { return 0.5f * x * (1.f + erff(x * 0.70710678118654752f)); }
+__device__ __forceinline__ float gelu_d(float x)
+{
+    return 0.5f * (1.f + erff(x * 0.70710678118654752f)) + x * 0.39894228040143268f * expf(-0.5f * x * x);
+}
+
 // C[z] = op(A) op(B) (+ bias) over k in [z*kchunk, (z+1)*kchunk); op(A) is M x K,
 // op(B) is K x N. TA: A stored K x M (A(m,k) = A[k*lda + m]); TB: B stored
 // N x K (B(k,n) = B[n*ldb + k]). Split slice z is written at C + z*cstride.
@@ -43,11 +49,14 @@ __device__ __forceinline__ short f2bf(float f)
 // contiguous ([m][k] and [n][k]), so each MFMA operand is one 8-byte LDS read;
 // each thread loads 8 consecutive floats of a tile (two 16-byte loads for
 // full, aligned tiles), and the next tile's loads are in flight during the
-// current tile's MFMAs.
-template <bool TA, bool TB>
+// current tile's MFMAs. Epilogue EPI: 0 C = result (+ bias); 1 C = GELU(result
+// + bias); 2 C = result + bias and C2 = GELU(C) (the pre-activation kept for
+// the backward pass), the same values as a separate GELU pass over C.
+template <bool TA, bool TB, int EPI = 0>
 __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, const float* __restrict__ A, int lda,
                                               const float* __restrict__ B, int ldb, float* __restrict__ C, int ldc,
-                                              size_t cstride, const float* __restrict__ bias, int kchunk)
+                                              size_t cstride, const float* __restrict__ bias, int kchunk,
+                                              float* __restrict__ C2 = nullptr)
 {
     constexpr int P = 40;  // LDS row pitch in bfloat16 (80 bytes: 16-byte aligned rows)
     __shared__ __align__(16) short As[64][P];
@@ -123,7 +132,12 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, const float* 
         if (row < M && col < N) {
             float v = acc[r];
             if (bias) v += bias[col];
-            Cz[(size_t)row * ldc + col] = v;
+            if constexpr (EPI == 1) {
+                Cz[(size_t)row * ldc + col] = gelu(v);
+            } else {
+                Cz[(size_t)row * ldc + col] = v;
+                if constexpr (EPI == 2) C2[(size_t)row * ldc + col] = gelu(v);
+            }
         }
     }
 }
@@ -167,16 +181,6 @@ __global__ __launch_bounds__(256) void k_colsum_part(const float* __restrict__ X
     part[(size_t)blockIdx.y * cols + c] = s;
 }
 
-__device__ __forceinline__ float gelu(float x) { return 0.5f * x * (1.f + erff(x * 0.70710678118654752f)); }
-__device__ __forceinline__ float gelu_d(float x)
-{
-    return 0.5f * (1.f + erff(x * 0.70710678118654752f)) + x * 0.39894228040143268f * expf(-0.5f * x * x);
-}
-
-__global__ __launch_bounds__(256) void k_gelu(const float* __restrict__ H, float* __restrict__ A, size_t n)
-{
-    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) A[i] = gelu(H[i]);
-}
 
 __global__ __launch_bounds__(256) void k_gelu_bwd(const float* __restrict__ H, float* __restrict__ dA, size_t n)
 {
@@ -413,22 +417,31 @@ void Mlp::colsum(const float* Xm, int rows, int cols, float* out)
     hipLaunchKernelGGL(k_sum_slices, dim3(grid_for(cols)), dim3(256), 0, stream, colpart, nb, (size_t)cols, out);
 }
 
-void Mlp::forward(int rows)
+void Mlp::forward(int rows, bool keep)
 {
     reserve(rows);
     const float *W1 = params + off[0], *b1 = params + off[1], *W2 = params + off[2], *b2 = params + off[3],
                 *W3 = params + off[4], *b3 = params + off[5];
-    gemm(false, false, rows, h1, in_pad, X, in_pad, W1, h1, H1, h1, b1, 1);
-    hipLaunchKernelGGL(k_gelu, dim3(grid_for((size_t)rows * h1)), dim3(256), 0, stream, H1, A1, (size_t)rows * h1);
-    gemm(false, false, rows, h2, h1, A1, h1, W2, h2, H2, h2, b2, 1);
-    hipLaunchKernelGGL(k_gelu, dim3(grid_for((size_t)rows * h2)), dim3(256), 0, stream, H2, A2, (size_t)rows * h2);
+    // GELU in the GEMM epilogue; the pre-activations H1, H2 are written only
+    // when a backward pass follows
+    auto layer = [&](int n, int k, const float* in, const float* W, const float* b, float* H, float* Aout) {
+        dim3 g((n + 63) / 64, (rows + 63) / 64, 1);
+        if (keep)
+            hipLaunchKernelGGL((k_gemm<false, false, 2>), g, dim3(256), 0, stream, rows, n, k, in, k, W, n, H, n,
+                               (size_t)0, b, k, Aout);
+        else
+            hipLaunchKernelGGL((k_gemm<false, false, 1>), g, dim3(256), 0, stream, rows, n, k, in, k, W, n, Aout, n,
+                               (size_t)0, b, k, nullptr);
+    };
+    layer(h1, in_pad, X, W1, b1, H1, A1);
+    layer(h2, h1, A1, W2, b2, H2, A2);
     gemm(false, false, rows, out_pad, h2, A2, h2, W3, out_pad, Y, out_pad, b3, 1);
 }
 
 // forward + mean-squared-error backward; loss (sum of squares / (rows*out)) in red[0] when host_loss
 double Mlp::backward(int rows, bool host_loss)
 {
-    forward(rows);
+    forward(rows, true);
     const int nb = std::min(grid_for((size_t)rows * out_pad), 4096);
     hipLaunchKernelGGL(k_mse, dim3(nb), dim3(256), 0, stream, Y, T, dY, rows, out_pad, out, red + 16);
     hipLaunchKernelGGL(k_sum_double, dim3(1), dim3(256), 0, stream, red + 16, nb, red);

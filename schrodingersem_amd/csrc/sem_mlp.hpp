@@ -34,7 +34,7 @@ struct Mlp {
     void reserve(int rows);
     void set_params(const float* host);
     void get_params(float* host, bool grad);
-    void forward(int rows);                       // X -> Y
+    void forward(int rows, bool keep = false);    // X -> Y (keep: H1, A1, H2, A2 for a backward pass)
     double backward(int rows, bool host_loss);    // X, T -> grads (MSE loss); returns the loss if host_loss
     void backprop(int rows);                      // dY (and the activations of forward) -> grads
     void adam_step();
