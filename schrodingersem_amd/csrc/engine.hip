@@ -1197,8 +1197,21 @@ static void autotune_steps(sem_mesh* m, int kind, int steps, std::vector<double>
     o.atol = 0;
     o.hint_slot = 7;
     m->timing = 2;
+    // the one-launch FP32 step runs one work item per workgroup for every
+    // shape: its persistent candidates repeat the one-shot timing of their
+    // chunks per workgroup and are not picked
+    const bool fused1 = fused_step(m, ti);
     for (int rep = 0; rep < 2; ++rep)  // first pass warms up and allocates
         for (size_t c = 0; c < cand.size(); ++c) {
+            if (fused1 && cand[c].second > 0) {
+                if (rep == 1)
+                    for (size_t c0 = 0; c0 < cand.size(); ++c0)
+                        if (cand[c0].second == 0 && cand[c0].first == cand[c].first) {
+                            us[2 * c] = us[2 * c0];
+                            us[2 * c + 1] = us[2 * c0 + 1];
+                        }
+                continue;
+            }
             m->shape[ti] = StepShape{cand[c].first, cand[c].second, cand[c].first, cand[c].second};
             for (int q = 0; q < 4; ++q) {
                 m->kt_launches[q] = 0;
@@ -1215,6 +1228,7 @@ static void autotune_steps(sem_mesh* m, int kind, int steps, std::vector<double>
     // in fused mode only the operator (fused) kernel runs; the update shape stays
     const bool fused = fused_step(m, ti);
     for (size_t c = 0; c < cand.size(); ++c) {
+        if (fused1 && cand[c].second > 0) continue;
         if (us[2 * c] < b_op) {
             b_op = us[2 * c];
             best.op_cpb = cand[c].first;

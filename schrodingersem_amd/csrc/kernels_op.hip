@@ -1266,21 +1266,17 @@ template <int N, typename T, int CPB, bool P>
 static void minres_fused_NC(int kind, const MinresFusedArgs<T>& a, const Dm<N, T>& D, int grid, hipStream_t s,
                             hipEvent_t e0, hipEvent_t e1)
 {
-    const int items = a.nchunks / CPB;
-    dim3 g(P ? std::min(grid, items) : items);
-    dim3 b2(128 * CPB);
+    (void)grid;
+    // One work item per workgroup for every shape: the persistent form of
+    // this kernel spilled registers and moved 1.4-1.9x its algorithmic bytes
+    // (and was 3-4x slower), so a persistent request runs the one-shot grid;
+    // results do not depend on the shape.
+    const dim3 g(a.nchunks / CPB);
+    const dim3 b2(128 * CPB);
     if (kind != K_RES) throw std::runtime_error("minres: the one-launch step serves the Schrodinger stage operator");
-    if (a.mf == 0) {
-        SEM_LAUNCH_TIMED((k_minres_fused<N, T, K_RES, CPB, P>), g, b2, s, e0, e1, a, D);
-    } else if constexpr (!P) {
-        // the MFMA variants are built for the one-item-per-workgroup shapes
-        if (a.mf == 1) SEM_LAUNCH_TIMED((k_minres_fused<N, T, K_RES, CPB, false, 1>), g, b2, s, e0, e1, a, D);
-        else SEM_LAUNCH_TIMED((k_minres_fused<N, T, K_RES, CPB, false, 3>), g, b2, s, e0, e1, a, D);
-    } else {
-        const dim3 gi(items);
-        if (a.mf == 1) SEM_LAUNCH_TIMED((k_minres_fused<N, T, K_RES, CPB, false, 1>), gi, b2, s, e0, e1, a, D);
-        else SEM_LAUNCH_TIMED((k_minres_fused<N, T, K_RES, CPB, false, 3>), gi, b2, s, e0, e1, a, D);
-    }
+    if (a.mf == 0) SEM_LAUNCH_TIMED((k_minres_fused<N, T, K_RES, CPB, false>), g, b2, s, e0, e1, a, D);
+    else if (a.mf == 1) SEM_LAUNCH_TIMED((k_minres_fused<N, T, K_RES, CPB, false, 1>), g, b2, s, e0, e1, a, D);
+    else SEM_LAUNCH_TIMED((k_minres_fused<N, T, K_RES, CPB, false, 3>), g, b2, s, e0, e1, a, D);
 }
 
 template <int N, typename T>
