@@ -1559,6 +1559,13 @@ struct Stepper {
     double s_feat[5][3] = {};  // per stage: scales of z, V, b features
     double s_out[5] = {};      // per stage: scale of the correction
     bool have_scale[5] = {}, have_out[5] = {};
+    // the network's Adam steps on a training sample run on their own stream,
+    // overlapped with the next stage solves; ev_data: the sample's features
+    // and targets are in place; ev_trained: its Adam steps are done (the
+    // network's inputs and parameters may be touched again)
+    hipStream_t train_stream = nullptr;
+    hipEvent_t ev_data = nullptr, ev_trained = nullptr;
+    bool train_pending = false, loss_pending = false;
 
     double* alloc()
     {
@@ -1569,8 +1576,30 @@ struct Stepper {
     }
     ~Stepper()
     {
+        if (train_stream) hipStreamSynchronize(train_stream);
         if (m) hipStreamSynchronize(m->stream);
         for (void* d : owned) hipFree(d);
+        if (ev_data) hipEventDestroy(ev_data);
+        if (ev_trained) hipEventDestroy(ev_trained);
+        if (train_stream) hipStreamDestroy(train_stream);
+    }
+    // the main stream waits for the last training sample's Adam steps
+    void wait_training()
+    {
+        if (train_pending) SEM_HIP(hipStreamWaitEvent(m->stream, ev_trained, 0));
+    }
+    // host: the training stream has drained; the last loss is read
+    void finish_training()
+    {
+        if (!train_pending) return;
+        SEM_HIP(hipStreamSynchronize(train_stream));
+        train_pending = false;
+        if (loss_pending) {
+            double l = 0;
+            SEM_HIP(hipMemcpy(&l, mlp->red, sizeof(double), hipMemcpyDeviceToHost));
+            stats.mlp_last_loss = l / ((double)m->L.E * mlp->out);
+            loss_pending = false;
+        }
     }
 };
 
@@ -1695,6 +1724,7 @@ static Results stage_solve(Stepper& st, double alpha, const double* b, double* z
         }
         if (st.have_out[stage] && !training) {
             Mlp& net = *st.mlp;
+            st.wait_training();
             net.reserve(m->L.E);
             mlp_features(st, stage, fin);
             net.forward(m->L.E);
@@ -1768,19 +1798,26 @@ static Results stage_solve(Stepper& st, double alpha, const double* b, double* z
             st.have_out[stage] = true;
         }
         Mlp& net = *st.mlp;
+        st.wait_training();  // the previous sample's Adam steps still read X and T
         net.reserve(m->L.E);
         mlp_features(st, stage, st.feat_rext ? st.rext : b);
         if (!st.residual_loss) {
-            // mean squared error of the scaled correction (z - ext) / s_out
+            // mean squared error of the scaled correction (z - ext) / s_out;
+            // the Adam steps run on the training stream while the next stage
+            // solves go on (the window's solves start from the extrapolation)
             launch_gather_target(scatter_args(st, stage, nullptr, st.ext, 1.0 / st.s_out[stage]), net.T, z,
                                  m->stream);
+            SEM_HIP(hipEventRecord(st.ev_data, m->stream));
+            SEM_HIP(hipStreamWaitEvent(st.train_stream, st.ev_data, 0));
+            net.stream = st.train_stream;
             for (int ep = 0; ep < st.epochs; ++ep) {
-                const bool last = ep + 1 == st.epochs;
-                const double loss = net.backward(m->L.E, last);
+                net.backward(m->L.E, false);
                 net.adam_step();
                 st.stats.mlp_train_steps++;
-                if (last) st.stats.mlp_last_loss = loss;
             }
+            net.stream = m->stream;
+            SEM_HIP(hipEventRecord(st.ev_trained, st.train_stream));
+            st.train_pending = st.loss_pending = true;
         } else {
             // residual loss ||Res (g - z)||^2 / ||b - Res ext||^2 of the assembled
             // guess g (the owner copy of each node, Dirichlet entries zero): the
@@ -1908,6 +1945,10 @@ static Stepper* stepper_create(sem_mesh* m, int scheme, const double* re, const 
         const int N = m->L.N, NN = N * N;
         st->mlp.reset(new Mlp(7 * NN + kStageCols, 256, 256, 2 * NN, 0x5EED0001ull, m->stream));
         st->mlp->lr = 3e-3f;
+        st->mlp->reserve(m->L.E);  // allocated once: the training stream uses these buffers
+        SEM_HIP(hipStreamCreateWithFlags(&st->train_stream, hipStreamNonBlocking));
+        SEM_HIP(hipEventCreateWithFlags(&st->ev_data, hipEventDisableTiming));
+        SEM_HIP(hipEventCreateWithFlags(&st->ev_trained, hipEventDisableTiming));
         st->learn_steps = o->learn_steps > 0 ? o->learn_steps : 50;
         st->ext = st->alloc();
         st->tmp = st->alloc();
@@ -2100,6 +2141,7 @@ static bool stepper_advance(Stepper& st, double T, long max_steps, sem_step_cb c
     st.m->sync();
     if (st.multi())
         for (int q = 0; q < ni; ++q) success = success && !st.frozen[q];
+    st.finish_training();  // the window's last Adam steps are part of this call
     return success;
 }
 
