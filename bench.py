@@ -350,8 +350,13 @@ def bench_sdirk(sem, args, workload):
     avg_us = total_us / max(launches, 1)
     achieved = bpl / (avg_us * 1e-6) / 1e9 if launches else 0.0
     shape = tuned["fp32"]["shape"]
-    fused = sem.get_step_mode(m, sem.PREC_MIXED) >= 1
+    step_mode = sem.get_step_mode(m, sem.PREC_MIXED)
+    fused = step_mode >= 1
     kbase = "k_minres_fused" if fused else "k_minres_op"
+    # the one-launch step's compiled form: 3 in step mode 2 with the stored
+    # basis (the library default), 0 otherwise; the two-launch step's template
+    # argument there is its persistent flag
+    form = (3 if step_mode == 2 else 0) if fused else ("true" if shape[1] else "false")
     traffic = None
     prof = os.path.join(ROOT, "profiles", "hbm_traffic_%s.json" % wname)
     if os.path.exists(prof):
@@ -362,13 +367,14 @@ def bench_sdirk(sem, args, workload):
         # (rocprofv3 names carry the full signature; either chunks-per-workgroup
         # shape of the one-shot form moves the same bytes)
         for name, e in kern.items():
-            if (name.startswith("void sem::%s<%d, float, 3, " % (kbase, p + 1)) and ", false, 0>" in name
+            if (name.startswith("void sem::%s<%d, float, 3, " % (kbase, p + 1))
+                    and (", %s, 0>" % form if fused else ", %s>" % form) in name
                     and e.get("algorithmic_bytes") == bpl):
                 traffic = round(e["traffic_bytes"])
                 break
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "kernel": "%s<%d,float,RES,%d,%s>" % (kbase, p + 1, shape[0], "true" if shape[1] else "false"),
+                "kernel": "%s<%d,float,RES,%d,%s>" % (kbase, p + 1, shape[0], form),
                 "avg_us": round(avg_us, 2), "launches": launches, "bytes_per_launch": bpl}
     # one MINRES iteration against the reference's data model (132 E N^2 bytes
     # of FP32 vector and geometry streams plus two DSS passes of 20 bytes per

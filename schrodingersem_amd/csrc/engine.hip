@@ -94,7 +94,8 @@ struct sem_mesh {
     Layout L;
     hipStream_t stream = nullptr;
     double* geo = nullptr;
-    float* geo32 = nullptr;
+    float* qmap = nullptr;  // QuadMap::map (8 floats per device element)
+    QuadMap qm{};
     uint8_t* obits = nullptr;
     std::vector<uint8_t> h_obits;
     // DSS lists (device layout positions)
@@ -200,7 +201,7 @@ struct sem_mesh {
         for (hipEvent_t e : ev) hipEventDestroy(e);
         for (auto& kv : pool) hipFree(kv.second.p);
         hipFree(geo);
-        hipFree(geo32);
+        hipFree(qmap);
         hipFree(obits);
         hipFree(pairs);
         hipFree(c_off);
@@ -274,7 +275,7 @@ struct sem_mesh {
 template <>
 const double* sem_mesh::geo_t<double>() const { return geo; }
 template <>
-const float* sem_mesh::geo_t<float>() const { return geo32; }
+const float* sem_mesh::geo_t<float>() const { return nullptr; }  // FP32 applies use qm
 template <>
 const double* sem_mesh::dmat<double>() const { return D64.data(); }
 template <>
@@ -307,11 +308,29 @@ static void setup_device(sem_mesh* m)
                 }
         to_device_layout(L, tmp.data(), g.data() + f * L.len);
     }
-    std::vector<float> g32(g.begin(), g.end());
     SEM_HIP(hipMalloc(&m->geo, g.size() * sizeof(double)));
-    SEM_HIP(hipMalloc(&m->geo32, g32.size() * sizeof(float)));
     SEM_HIP(hipMemcpy(m->geo, g.data(), g.size() * sizeof(double), hipMemcpyHostToDevice));
-    SEM_HIP(hipMemcpy(m->geo32, g32.data(), g32.size() * sizeof(float), hipMemcpyHostToDevice));
+    // the FP32 applies' geometry: each element's bilinear map (zero for
+    // padding elements and ghosts, whose factors are zero)
+    {
+        std::vector<float> qm((size_t)L.nchunks * L.K * 8, 0.0f);
+        for (int e = 0; e < hm.n_el; ++e) {
+            if (!m->ghost.empty() && m->ghost[e]) continue;
+            double xs[4], ys[4];
+            for (int k = 0; k < 4; ++k) {
+                xs[k] = hm.node_xy[2 * (size_t)hm.quads[4 * (size_t)e + k]];
+                ys[k] = hm.node_xy[2 * (size_t)hm.quads[4 * (size_t)e + k] + 1];
+            }
+            quad_map(xs, ys, qm.data() + 8 * (size_t)e);
+        }
+        SEM_HIP(hipMalloc(&m->qmap, qm.size() * sizeof(float)));
+        SEM_HIP(hipMemcpy(m->qmap, qm.data(), qm.size() * sizeof(float), hipMemcpyHostToDevice));
+        m->qm.map = reinterpret_cast<const float4*>(m->qmap);
+        for (int k = 0; k < kMaxN; ++k) {
+            m->qm.z[k] = k < N ? (float)hm.x[k] : 0.0f;
+            m->qm.w[k] = k < N ? (float)hm.w[k] : 0.0f;
+        }
+    }
 
     // DSS lists
     auto side_ij = [&](int s, int k, int& i, int& j) {
@@ -525,6 +544,7 @@ static void apply_op(sem_mesh* m, int kind, const T* in, T* out, const T* cf, do
     a.in = in;
     a.out = out;
     a.geo = m->geo_t<T>();
+    a.qm = m->qm;
     a.cf = cf;
     a.alpha = (T)alpha;
     a.post = (T)post;
@@ -737,10 +757,13 @@ static Results minres_fused1(sem_mesh* m, int kind, const T* cf, double alpha, c
         }
         // algorithmic bytes of one launch: finishing y (z in, y_{i-2}, y_{i-3} in,
         // y out), the operator apply (its inputs and outputs) and the w/x update,
-        // per component, plus the diagonal scale when there is one
+        // per component, plus the diagonal scale when there is one; the
+        // geometry is 4 stored factors per node in FP64 and each element's
+        // 32-byte bilinear map in FP32
         const double EN2 = (double)m->L.E * m->L.N * m->L.N;
-        const double geo_cf = (kind == K_RES) ? 5 : (kind == K_HELMHOLTZ ? 4 : 3);
-        m->kt_bytes[ti] = (geo_cf + (cap ? 5.0 : 10.0) * nc) * sizeof(T) * EN2 + (dscale ? sizeof(float) * EN2 : 0.0);
+        const double cfv = (kind == K_RES || kind == K_HELMHOLTZ) ? 1 : 0;
+        const double geo = sizeof(T) == 8 ? (kind == K_RES ? 4.0 : 3.0) * sizeof(T) * EN2 : 32.0 * m->L.E;
+        m->kt_bytes[ti] = (cfv + (cap ? 5.0 : 10.0) * nc) * sizeof(T) * EN2 + geo + (dscale ? sizeof(float) * EN2 : 0.0);
     }
     KrylovScalars* buf[2] = {m->sc, m->sc_alt};
     const InstTable itab = m->it();
@@ -784,6 +807,7 @@ static Results minres_fused1(sem_mesh* m, int kind, const T* cf, double alpha, c
             a.rec = rec;
         }
         a.geo = m->geo_t<T>();
+        a.qm = m->qm;
         a.cf = cf;
         a.alpha = (T)alpha;
         a.len = m->len();
@@ -806,6 +830,7 @@ static Results minres_fused1(sem_mesh* m, int kind, const T* cf, double alpha, c
         }
         a.step = i;
         a.first = ext || (b_ == 0);
+        a.ext = ext ? 1 : 0;
         a.mf = sizeof(T) == 4 ? m->contraction : 0;
         a.dscale = dscale;
         launch_minres_fused<T>(m->L.N, kind, a, m->dmat<T>(), shp.op_cpb, shp.op_grid, s,
@@ -977,8 +1002,9 @@ static Results minres_fused(sem_mesh* m, int kind, const T* cf, double alpha, co
         // r1 read for y -= cr1*r1, and the w/x update (wa, wb, x in; w, x out),
         // per component
         const double EN2 = (double)m->L.E * m->L.N * m->L.N;
-        const double op_vals = (kind == K_RES) ? 9 : (kind == K_HELMHOLTZ ? 7 : 5);
-        m->kt_bytes[ti] = (op_vals + 6.0 * nc) * sizeof(T) * EN2;
+        const double op_vals = (kind == K_RES) ? 5 : (kind == K_HELMHOLTZ ? 4 : 2);  // vectors and coefficient
+        const double geo = sizeof(T) == 8 ? (kind == K_RES ? 4.0 : 3.0) * sizeof(T) * EN2 : 32.0 * m->L.E;
+        m->kt_bytes[ti] = (op_vals + 6.0 * nc) * sizeof(T) * EN2 + geo;
         // update kernel: yraw and r2 in, y out, per component
         m->kt_bytes[2 + ti] = 3.0 * nc * sizeof(T) * EN2;
     }
@@ -1011,6 +1037,7 @@ static Results minres_fused(sem_mesh* m, int kind, const T* cf, double alpha, co
         a.r1 = R[(i + 1) % 3];
         a.y = Yraw;
         a.geo = m->geo_t<T>();
+        a.qm = m->qm;
         a.cf = cf;
         a.alpha = (T)alpha;
         a.len = m->len();
@@ -2784,7 +2811,8 @@ int sem_element_apply_fp32(sem_mesh* m, int kind, const double* coef, double alp
         OpArgs<float> a{};
         a.in = fin;
         a.out = fout;
-        a.geo = m->geo32;
+        a.geo = nullptr;
+        a.qm = m->qm;
         a.cf = f.c32;
         a.alpha = (float)alpha;
         a.post = 1.0f;
@@ -3839,18 +3867,21 @@ int sem_time_element_op(sem_mesh* m, int kind, int precision, int reps, double* 
         const size_t n = (size_t)nc * m->len();
         const int N = m->L.N;
         const double EN2 = (double)m->L.E * N * N;
-        // algorithmic bytes per launch in the reference data model, 8 B (FP64) or 4 B (FP32) values
+        // algorithmic bytes per launch: in FP64 the reference data model (8 B
+        // values); in FP32, 4 B values without the geometric factors, which
+        // the kernels form from each element's 32-byte bilinear map
         const double vb = precision ? 4.0 : 8.0;
-        double fields;
+        double fields, geo;
         switch (kind) {
-        case K_LAPLACE: fields = 5; break;    // u, A, B, C in; y out
-        case K_HELMHOLTZ: fields = 7; break;  // u, c, J, A, B, C in; y out
-        case K_MASS: fields = 3; break;       // u, J in; y out
-        case K_RES: fields = 9; break;        // u, v, V, J, A, B, C in; 2 out
-        case K_F: fields = 9; break;          // u, v, V, J, A, B, C in; 2 out
-        default: fields = 7; break;
+        case K_LAPLACE: fields = 5, geo = 3; break;    // u, A, B, C in; y out
+        case K_HELMHOLTZ: fields = 7, geo = 4; break;  // u, c, J, A, B, C in; y out
+        case K_MASS: fields = 3, geo = 1; break;       // u, J in; y out
+        case K_RES: fields = 9, geo = 4; break;        // u, v, V, J, A, B, C in; 2 out
+        case K_F: fields = 9, geo = 4; break;          // u, v, V, J, A, B, C in; 2 out
+        default: fields = 7, geo = 4; break;           // u, V, J, A, B, C in; y out
         }
-        if (bytes_per_launch) *bytes_per_launch = fields * vb * EN2;
+        if (bytes_per_launch)
+            *bytes_per_launch = precision ? (fields - geo) * vb * EN2 + 32.0 * m->L.E : fields * vb * EN2;
         hipEvent_t e0, e1;
         SEM_HIP(hipEventCreate(&e0));
         SEM_HIP(hipEventCreate(&e1));
@@ -3861,7 +3892,7 @@ int sem_time_element_op(sem_mesh* m, int kind, int precision, int reps, double* 
             float* cf = m->buf<float>("t_cf32", m->len());
             launch_fill<float>(in, 0.5f, n, m->stream);
             launch_fill<float>(cf, 0.25f, m->len(), m->stream);
-            OpArgs<float> a{in, out, m->geo32, cf, 0.1f, 1.0f, m->len(), m->L.nchunks, kind == K_RES ? m->contraction : 0};
+            OpArgs<float> a{in, out, nullptr, cf, 0.1f, 1.0f, m->len(), m->L.nchunks, kind == K_RES ? m->contraction : 0, m->qm};
             launch_element_op_d<float>(N, kind, a, m->D32.data(), m->stream);
             SEM_HIP(hipEventRecord(e0, m->stream));
             for (int r = 0; r < reps; ++r) launch_element_op_d<float>(N, kind, a, m->D32.data(), m->stream);

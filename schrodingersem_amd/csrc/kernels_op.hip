@@ -76,10 +76,13 @@ __device__ __forceinline__ void xpose_r2c(T* buf, int el, int l, bool act, const
     for (int i = 0; i < N; ++i) col[i] = act ? b[l * P + i] : T(0);
 }
 
-// out(:,l) = (K q)(:,l), the element stiffness action of glaplace.
-template <int N, typename T>
+// out(:,l) = (K q)(:,l), the element stiffness action of glaplace; geo(i, a,
+// b, c) gives gA, gB, gC at node (i, l) where G1 and G2 are formed (FP32
+// kernels compute them there from the element map, so they are not held in
+// registers across the first two contractions).
+template <int N, typename T, class GeoF>
 __device__ __forceinline__ void stiffness(const Dm<N, T>& D, T* lds0, T* lds1, int el, int l, bool act, const T (&q)[N],
-                                          const T (&ga)[N], const T (&gb)[N], const T (&gc)[N], T (&out)[N])
+                                          const GeoF& geo, T (&out)[N])
 {
     T qxi[N], tmp[N], qer[N], qe[N];
 #pragma unroll
@@ -101,8 +104,10 @@ __device__ __forceinline__ void stiffness(const Dm<N, T>& D, T* lds0, T* lds1, i
     T g1[N], g2[N];
 #pragma unroll
     for (int i = 0; i < N; ++i) {
-        g1[i] = ga[i] * qxi[i] - gb[i] * qe[i];
-        g2[i] = gc[i] * qe[i] - gb[i] * qxi[i];
+        T ga, gb, gc;
+        geo(i, ga, gb, gc);
+        g1[i] = ga * qxi[i] - gb * qe[i];
+        g2[i] = gc * qe[i] - gb * qxi[i];
     }
 #pragma unroll
     for (int i = 0; i < N; ++i) {  // D^T G1
@@ -122,6 +127,20 @@ __device__ __forceinline__ void stiffness(const Dm<N, T>& D, T* lds0, T* lds1, i
     xpose_r2c<N>(lds1, el, l, act, qer, qe);
 #pragma unroll
     for (int i = 0; i < N; ++i) out[i] += qe[i];
+}
+
+// the same with gA, gB, gC in registers
+template <int N, typename T>
+__device__ __forceinline__ void stiffness(const Dm<N, T>& D, T* lds0, T* lds1, int el, int l, bool act, const T (&q)[N],
+                                          const T (&ga)[N], const T (&gb)[N], const T (&gc)[N], T (&out)[N])
+{
+    stiffness<N>(D, lds0, lds1, el, l, act, q,
+                 [&](int i, T& a, T& b, T& c) {
+                     a = ga[i];
+                     b = gb[i];
+                     c = gc[i];
+                 },
+                 out);
 }
 
 // ----------------------------------------------- MFMA stacked contractions --
@@ -314,6 +333,86 @@ struct Col {
     }
 };
 
+// FP32 geometric factors of column l of device element e, formed from the
+// element's bilinear map (QuadMap): the arithmetic of quad_metrics
+// (Quad.hpp:39-54, the reference's C index included) times w_i w_j, with
+// explicit FMAs so that every kernel forms the same bits. A zero map (padding
+// elements, ghosts) gives zero factors.
+template <int N>
+struct QuadColumn {
+    float4 u, v;  // (ax, bx, ay, by), (cx, dx, cy, dy)
+    float xxi, yxi, wl;  // X_xi, Y_xi at node l; w_l
+    __device__ __forceinline__ QuadColumn(const QuadMap& q, int e, int l, bool act)
+    {
+        u = act ? q.map[2 * e] : make_float4(0.f, 0.f, 0.f, 0.f);
+        v = act ? q.map[2 * e + 1] : make_float4(0.f, 0.f, 0.f, 0.f);
+        float zl = 0.f;
+        wl = 0.f;
+#pragma unroll
+        for (int k = 0; k < N; ++k)
+            if (l == k) {
+                zl = q.z[k];
+                wl = q.w[k];
+            }
+        xxi = __builtin_fmaf(u.y, zl, u.x);
+        yxi = __builtin_fmaf(u.w, zl, u.z);
+    }
+    // J(i, l), with X_eta(i) and Y_eta(i)
+    __device__ __forceinline__ float jac(const QuadMap& q, int i, float& xe, float& ye) const
+    {
+        xe = __builtin_fmaf(v.y, q.z[i], v.x);
+        ye = __builtin_fmaf(v.w, q.z[i], v.z);
+        return __builtin_fmaf(xxi, ye, -(xe * yxi));
+    }
+    // gA, gB, gC at node (i, l)
+    __device__ __forceinline__ void at(const QuadMap& q, int i, float& ga, float& gb, float& gc) const
+    {
+        float xe, ye;
+        const float j = jac(q, i, xe, ye);
+        const float s = (j != 0.f) ? (q.w[i] * wl) * __builtin_amdgcn_rcpf(j) : 0.f;
+        const float yxi_i = __builtin_fmaf(u.w, q.z[i], u.z);
+        ga = __builtin_fmaf(ye, ye, xe * xe) * s;
+        gb = __builtin_fmaf(ye, yxi, xe * xxi) * s;
+        gc = __builtin_fmaf(yxi_i, yxi_i, xxi * xxi) * s;
+    }
+    __device__ __forceinline__ void stiff(const QuadMap& q, float (&ga)[N], float (&gb)[N], float (&gc)[N]) const
+    {
+#pragma unroll
+        for (int i = 0; i < N; ++i) at(q, i, ga[i], gb[i], gc[i]);
+    }
+    __device__ __forceinline__ void mass(const QuadMap& q, float (&gj)[N]) const
+    {
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+            float xe, ye;
+            gj[i] = (q.w[i] * wl) * jac(q, i, xe, ye);
+        }
+    }
+};
+
+// gA, gB, gC of column l of element e: formed from the map in FP32, loaded in FP64
+template <int N, typename T, class C>
+__device__ __forceinline__ void geo_stiff(const T* geo, size_t len, const QuadMap& qm, const C& c, int e, int l,
+                                          bool act, T (&ga)[N], T (&gb)[N], T (&gc)[N])
+{
+    if constexpr (sizeof(T) == 4) {
+        QuadColumn<N>(qm, e, l, act).stiff(qm, ga, gb, gc);
+    } else {
+        c.ld(geo, ga);
+        c.ld(geo + len, gb);
+        c.ld(geo + 2 * len, gc);
+    }
+}
+
+// gJ of column l of element e
+template <int N, typename T, class C>
+__device__ __forceinline__ void geo_mass(const T* geo, size_t len, const QuadMap& qm, const C& c, int e, int l,
+                                         bool act, T (&gj)[N])
+{
+    if constexpr (sizeof(T) == 4) QuadColumn<N>(qm, e, l, act).mass(qm, gj);
+    else c.ld(geo + 3 * len, gj);
+}
+
 // Element operator of every sem_op_kind; ring entries are written raw (the DSS
 // kernel assembles them), interior entries are final and scaled by `post`.
 template <int N, typename T, int KIND, bool NT, int MF = 0>
@@ -334,10 +433,9 @@ __global__ __launch_bounds__(64) void k_element_op(OpArgs<T> a, Dm<N, T> D)
     const int l = act ? lane % N : 0;
     const Col<N, T, NT> c{(size_t)blockIdx.x * S::K * N * N + lane, act};
     const size_t len = a.len;
+    const int e = blockIdx.x * S::K + el;
     T ga[N], gb[N], gc[N];
-    c.ld(a.geo, ga);
-    c.ld(a.geo + len, gb);
-    c.ld(a.geo + 2 * len, gc);
+    geo_stiff<N>(a.geo, len, a.qm, c, e, l, act, ga, gb, gc);
     auto finish = [&](T* dst, T (&y)[N]) {
 #pragma unroll
         for (int i = 0; i < N; ++i)
@@ -357,7 +455,7 @@ __global__ __launch_bounds__(64) void k_element_op(OpArgs<T> a, Dm<N, T> D)
         stiff(q1, ky);
         if constexpr (KIND == K_RES) {
             T gj[N];
-            c.ld(a.geo + 3 * len, gj);
+            geo_mass<N>(a.geo, len, a.qm, c, e, l, act, gj);
 #pragma unroll
             for (int i = 0; i < N; ++i) y[i] = gj[i] * q0[i] + a.alpha * (ky[i] + cf[i] * q1[i]);
         } else {
@@ -368,7 +466,7 @@ __global__ __launch_bounds__(64) void k_element_op(OpArgs<T> a, Dm<N, T> D)
         stiff(q0, ky);
         if constexpr (KIND == K_RES) {
             T gj[N];
-            c.ld(a.geo + 3 * len, gj);
+            geo_mass<N>(a.geo, len, a.qm, c, e, l, act, gj);
 #pragma unroll
             for (int i = 0; i < N; ++i) y[i] = a.alpha * (ky[i] + cf[i] * q0[i]) - gj[i] * q1[i];
         } else {
@@ -381,7 +479,7 @@ __global__ __launch_bounds__(64) void k_element_op(OpArgs<T> a, Dm<N, T> D)
         c.ld(a.in, q);
         if constexpr (KIND == K_MASS) {
             T gj[N];
-            c.ld(a.geo + 3 * len, gj);
+            geo_mass<N>(a.geo, len, a.qm, c, e, l, act, gj);
 #pragma unroll
             for (int i = 0; i < N; ++i) y[i] = gj[i] * q[i];
         } else {
@@ -533,9 +631,7 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 1, 8))) void k_minres_op
             T ga[N], gb[N], gc[N], qs[N], y[N];
             // stiffness input: the other component for Res, the only one otherwise
             c.ld(a.r2 + (NC == 2 ? (1 - w) : 0) * len, qs);
-            c.ld(a.geo, ga);
-            c.ld(a.geo + len, gb);
-            c.ld(a.geo + 2 * len, gc);
+            geo_stiff<N>(a.geo, len, a.qm, c, chunk * S::K + el, l, act, ga, gb, gc);
 #pragma unroll
             for (int i = 0; i < N; ++i) qs[i] = ib * qs[i];
             stiffness<N>(D, lds[wv][0], lds[wv][1], el, l, act, qs, ga, gb, gc, y);
@@ -544,7 +640,7 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 1, 8))) void k_minres_op
                 T cf[N], gj[N];
                 c.ld(a.r2 + w * len, qm);
                 c.ld(a.cf, cf);
-                c.ld(a.geo + 3 * len, gj);
+                geo_mass<N>(a.geo, len, a.qm, c, chunk * S::K + el, l, act, gj);
 #pragma unroll
                 for (int i = 0; i < N; ++i) {
                     qm[i] = ib * qm[i];
@@ -820,13 +916,17 @@ __global__ __launch_bounds__(128 * CPB) void k_minres_upd(MinresUpdArgs<T> a)
 // (minres_fused_prologue) from the previous step's per-chunk sums; the
 // workgroup holding an instance's first chunk stores the state in the other
 // buffer. In step mode 2 the closing kernel evaluates the recurrence after
-// every step and each launch starts from that state (first = 1); a variant
-// compiled without the in-kernel reduction (fewer registers) was slower.
-template <int N, typename T, int KIND, int CPB, bool PERSIST, int MF = 0>
+// every step and each launch starts from that state (first = 1).
+// FORM 0 serves every case; FORM 3 is compiled for step mode 2 with the stored
+// basis (the default for large meshes): without the in-kernel reduction and
+// the w/x update its registers fit four waves per SIMD.
+template <int N, typename T, int KIND, int CPB, int FORM, int MF = 0>
 __global__ __launch_bounds__(64 * CPB * ((KIND == K_RES) ? 2 : 1))
 __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 3 : 1, 8))) void k_minres_fused(MinresFusedArgs<T> a, Dm<N, T> D)
 {
     using S = Shape<N>;
+    constexpr bool EXT = (FORM & 1) != 0;    // step mode 2: the recurrence is closed outside
+    constexpr bool BASIS = (FORM & 2) != 0;  // stored basis: no w/x update
     constexpr int NC = (KIND == K_RES) ? 2 : 1;
     constexpr int NW = NC * CPB;
     // MF: the stiffness contractions on MFMA (1 bf16, 3 split bf16), FP32 only
@@ -874,7 +974,7 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 3 : 1, 8))) void k_minres_fu
         const bool early3 = early && (a.step >= 3 || sc.has_r1);
         // the recurrence's sums are loaded first, so that its wait does not
         // include the gather
-        const bool rec = fresh && !a.first && a.step >= 2 && !sc.done;
+        const bool rec = !EXT && fresh && !a.first && a.step >= 2 && !sc.done;
         // all of a 128 x 128 mesh's sums in one round of loads up to N = 10;
         // fewer registers held across the gather for larger elements
         GroupSum3<NW, (N <= 10 ? 12 : 4)> gs;
@@ -887,7 +987,7 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 3 : 1, 8))) void k_minres_fu
             if (early3) c.ld(a.y3 + w * len, r3);
         }
         if (fresh) {
-            if (!a.first && a.step >= 2) {
+            if (!a.first && a.step >= 2) {  // never in step mode 2 (first = 1)
                 double P[3] = {0.0, 0.0, 0.0};  // not read once the instance has stopped
                 if (rec) gs.finish(a.prev_sums + ci.y, nch, ci.z - ci.y, red, P);
                 minres_fused_prologue(sc, a.step, P);
@@ -898,7 +998,7 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 3 : 1, 8))) void k_minres_fu
             sc.p3_pending = 0;
             if (c0 == ci.y && threadIdx.x == 0) *static_cast<MinresState*>(a.st_out + inst) = sc;
             // stored basis: no w/x update (the solution is formed at the end)
-            if (a.rec) pend = false;
+            if (BASIS || a.rec) pend = false;
         }
         // a stopped instance has no work left: skip the rest of its items
         // (its sums are never read again)
@@ -948,16 +1048,21 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 3 : 1, 8))) void k_minres_fu
 #pragma unroll
                 for (int i = 0; i < N; ++i) qs[i] = yc[i];
             }
-            T ga[N], gb[N], gc[N], z[N];
-            c.ld(a.geo, ga);
-            c.ld(a.geo + len, gb);
-            c.ld(a.geo + 2 * len, gc);
-            if constexpr (MF != 0) stiffness_mf<N, MF>(mf_dmat<N>(D, lane), mfs[wv], lane, qs, ga, gb, gc, z);
-            else stiffness<N>(D, lds[wv][0], lds[wv][1], el, l, act, qs, ga, gb, gc, z);
+            T z[N];
+            if constexpr (sizeof(T) == 4 && MF == 0) {
+                const QuadColumn<N> qc(a.qm, chunk * S::K + el, l, act);
+                stiffness<N>(D, lds[wv][0], lds[wv][1], el, l, act, qs,
+                             [&](int i, T& ga, T& gb, T& gc) { qc.at(a.qm, i, ga, gb, gc); }, z);
+            } else {
+                T ga[N], gb[N], gc[N];
+                geo_stiff<N>(a.geo, len, a.qm, c, chunk * S::K + el, l, act, ga, gb, gc);
+                if constexpr (MF != 0) stiffness_mf<N, MF>(mf_dmat<N>(D, lane), mfs[wv], lane, qs, ga, gb, gc, z);
+                else stiffness<N>(D, lds[wv][0], lds[wv][1], el, l, act, qs, ga, gb, gc, z);
+            }
             if constexpr (KIND == K_RES) {
                 T cf[N], gj[N];
                 c.ld(a.cf, cf);
-                c.ld(a.geo + 3 * len, gj);
+                geo_mass<N>(a.geo, len, a.qm, c, chunk * S::K + el, l, act, gj);
 #pragma unroll
                 for (int i = 0; i < N; ++i) {
                     const T t = a.alpha * (z[i] + cf[i] * qs[i]);
@@ -1011,11 +1116,7 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 3 : 1, 8))) void k_minres_fu
         __syncthreads();
         return -1;
     };
-    if constexpr (PERSIST) {
-        for_items_persistent(n_items, body);
-    } else if ((int)blockIdx.x < n_items) {
-        body(xcd_item(blockIdx.x, n_items));
-    }
+    if ((int)blockIdx.x < n_items) body(xcd_item(blockIdx.x, n_items));
 }
 
 #ifndef SEM_OP_FP32
@@ -1274,9 +1375,12 @@ static void minres_fused_NC(int kind, const MinresFusedArgs<T>& a, const Dm<N, T
     const dim3 g(a.nchunks / CPB);
     const dim3 b2(128 * CPB);
     if (kind != K_RES) throw std::runtime_error("minres: the one-launch step serves the Schrodinger stage operator");
-    if (a.mf == 0) SEM_LAUNCH_TIMED((k_minres_fused<N, T, K_RES, CPB, false>), g, b2, s, e0, e1, a, D);
-    else if (a.mf == 1) SEM_LAUNCH_TIMED((k_minres_fused<N, T, K_RES, CPB, false, 1>), g, b2, s, e0, e1, a, D);
-    else SEM_LAUNCH_TIMED((k_minres_fused<N, T, K_RES, CPB, false, 3>), g, b2, s, e0, e1, a, D);
+    // step mode 2 with the stored basis runs its own compiled form
+    const bool f3 = a.ext && a.rec != nullptr;
+    if (a.mf == 0 && f3) SEM_LAUNCH_TIMED((k_minres_fused<N, T, K_RES, CPB, 3>), g, b2, s, e0, e1, a, D);
+    else if (a.mf == 0) SEM_LAUNCH_TIMED((k_minres_fused<N, T, K_RES, CPB, 0>), g, b2, s, e0, e1, a, D);
+    else if (a.mf == 1) SEM_LAUNCH_TIMED((k_minres_fused<N, T, K_RES, CPB, 0, 1>), g, b2, s, e0, e1, a, D);
+    else SEM_LAUNCH_TIMED((k_minres_fused<N, T, K_RES, CPB, 0, 3>), g, b2, s, e0, e1, a, D);
 }
 
 template <int N, typename T>

@@ -121,6 +121,20 @@ void quad_metrics(const double* xs, const double* ys, const std::vector<double>&
         }
 }
 
+// The four metric functions of quad_metrics are linear in the node z:
+// 0.25 ((1 - z) a + (1 + z) b) = 0.25 (a + b) + 0.25 (b - a) z.
+void quad_map(const double* xs, const double* ys, float* out8)
+{
+    auto lin = [&](double a, double b, float* o) {
+        o[0] = (float)(0.25 * (a + b));
+        o[1] = (float)(0.25 * (b - a));
+    };
+    lin(xs[1] - xs[0], xs[2] - xs[3], out8 + 0);  // X_xi
+    lin(ys[1] - ys[0], ys[2] - ys[3], out8 + 2);  // Y_xi
+    lin(xs[3] - xs[0], xs[2] - xs[1], out8 + 4);  // X_eta
+    lin(ys[3] - ys[0], ys[2] - ys[1], out8 + 6);  // Y_eta
+}
+
 // Mesh topology with load_mesh's semantics (load_mesh.hpp:76-209): the owner
 // of an edge or corner is the first element, in element order, to touch it;
 // the second element's side is negative when it runs against the owner's.

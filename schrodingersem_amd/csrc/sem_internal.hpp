@@ -103,6 +103,22 @@ void from_device_layout(const Layout& L, const double* dev, double* ref);
 enum OpKind { K_LAPLACE = 0, K_HELMHOLTZ = 1, K_MASS = 2, K_RES = 3, K_F = 4, K_L = 5 };
 inline int op_ncomp(int kind) { return (kind == K_RES || kind == K_F) ? 2 : 1; }
 
+// Geometry of the FP32 operator applies: the metric terms of Quad.hpp:39-54
+// are built from four functions of the GLL node that are linear in it,
+//   X_xi(t) = ax + bx z_t, Y_xi(t) = ay + by z_t,
+//   X_eta(t) = cx + dx z_t, Y_eta(t) = cy + dy z_t,
+// so the FP32 kernels form gA, gB, gC, gJ from 8 values per element (32
+// bytes) instead of loading 4 N^2 stored values. FP64 applies load the
+// stored factors.
+struct QuadMap {
+    const float4* map;  // per device element: (ax, bx, ay, by), (cx, dx, cy, dy); zero for padding and ghosts
+    float z[kMaxN];     // GLL nodes
+    float w[kMaxN];     // GLL weights
+};
+
+// the map of corners (xs, ys) (counter-clockwise, Quad.hpp's corner order)
+void quad_map(const double* xs, const double* ys, float* out8);
+
 // Arguments of the element-operator kernels. `cf` is the premultiplied
 // coefficient field gJ*c (Helmholtz) or gJ*V (Schrodinger), device layout.
 template <typename T>
@@ -116,6 +132,7 @@ struct OpArgs {
     size_t len;
     int nchunks;
     int mf;        // FP32 Res only: stiffness contractions on MFMA (0 VALU, 1 bf16, 3 split bf16)
+    QuadMap qm;    // FP32: the geometry (geo unused)
 };
 
 // Device-resident Krylov scalars (FP64), one record per instance.
@@ -171,6 +188,7 @@ struct MinresOpArgs {
     T alpha;
     size_t len;
     int nchunks;
+    QuadMap qm;            // FP32: the geometry (geo unused)
     const uint8_t* obits;  // ownership bits per element
     double* partial;       // per-chunk alpha partial sums (out)
     InstTable it;
@@ -229,6 +247,7 @@ struct MinresFusedArgs {
     T alpha;
     size_t len;
     int nchunks;
+    QuadMap qm;  // FP32: the geometry (geo unused)
     const uint8_t* obits;
     const int* eside;
     const int4* ecq;
@@ -236,6 +255,8 @@ struct MinresFusedArgs {
     const int* c_pos;
     double* sums;              // 3 * nchunks per-chunk sums of this step (out)
     const double* prev_sums;   // the same of the previous step
+    int ext;                   // step mode 2: a closing kernel evaluates the recurrence after every step
+                               // and every launch starts from its state (first = 1)
     // Schrodinger stage (Res) only: diagonal scaling S of the system, one
     // value per node; the step applies S A S instead of A (nullptr: none)
     const float* dscale;
