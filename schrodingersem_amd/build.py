@@ -17,9 +17,13 @@ LIB = os.path.join(HERE, "libsem_hip.so")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
-# packed FP32 instructions are disabled for all device code
+# packed FP32 instructions are disabled for all device code. a*b+c is
+# contracted into an FMA only within one source expression (-ffp-contract=on):
+# every compiled form of a kernel (work shapes, step modes, batched or single
+# instances, the Krylov recurrence in whichever kernel evaluates it) then does
+# the same arithmetic and gives the same bits.
 COMMON = [
-    "--offload-arch=" + ARCH, "-O3", "-std=c++20", "-fPIC",
+    "--offload-arch=" + ARCH, "-O3", "-std=c++20", "-fPIC", "-ffp-contract=on",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-I" + os.path.join(ROOT, "include"), "-I" + CSRC,
     "-Wno-unused-result",

@@ -94,7 +94,8 @@ struct sem_mesh {
     Layout L;
     hipStream_t stream = nullptr;
     double* geo = nullptr;
-    float* qmap = nullptr;  // QuadMap::map (8 floats per device element)
+    float* geo32 = nullptr;  // FP32 factors where !map_geometry(N)
+    float* qmap = nullptr;   // QuadMap::map (8 floats per device element)
     QuadMap qm{};
     uint8_t* obits = nullptr;
     std::vector<uint8_t> h_obits;
@@ -201,6 +202,7 @@ struct sem_mesh {
         for (hipEvent_t e : ev) hipEventDestroy(e);
         for (auto& kv : pool) hipFree(kv.second.p);
         hipFree(geo);
+        hipFree(geo32);
         hipFree(qmap);
         hipFree(obits);
         hipFree(pairs);
@@ -275,7 +277,7 @@ struct sem_mesh {
 template <>
 const double* sem_mesh::geo_t<double>() const { return geo; }
 template <>
-const float* sem_mesh::geo_t<float>() const { return nullptr; }  // FP32 applies use qm
+const float* sem_mesh::geo_t<float>() const { return geo32; }  // nullptr where the FP32 kernels use qm
 template <>
 const double* sem_mesh::dmat<double>() const { return D64.data(); }
 template <>
@@ -310,8 +312,13 @@ static void setup_device(sem_mesh* m)
     }
     SEM_HIP(hipMalloc(&m->geo, g.size() * sizeof(double)));
     SEM_HIP(hipMemcpy(m->geo, g.data(), g.size() * sizeof(double), hipMemcpyHostToDevice));
-    // the FP32 applies' geometry: each element's bilinear map (zero for
-    // padding elements and ghosts, whose factors are zero)
+    if (!map_geometry(N)) {  // the FP32 kernels of this degree load stored factors
+        std::vector<float> g32(g.begin(), g.end());
+        SEM_HIP(hipMalloc(&m->geo32, g32.size() * sizeof(float)));
+        SEM_HIP(hipMemcpy(m->geo32, g32.data(), g32.size() * sizeof(float), hipMemcpyHostToDevice));
+    }
+    // the FP32 applies' geometry where map_geometry(N): each element's
+    // bilinear map (zero for padding elements and ghosts, whose factors are zero)
     {
         std::vector<float> qm((size_t)L.nchunks * L.K * 8, 0.0f);
         for (int e = 0; e < hm.n_el; ++e) {
@@ -762,7 +769,8 @@ static Results minres_fused1(sem_mesh* m, int kind, const T* cf, double alpha, c
         // 32-byte bilinear map in FP32
         const double EN2 = (double)m->L.E * m->L.N * m->L.N;
         const double cfv = (kind == K_RES || kind == K_HELMHOLTZ) ? 1 : 0;
-        const double geo = sizeof(T) == 8 ? (kind == K_RES ? 4.0 : 3.0) * sizeof(T) * EN2 : 32.0 * m->L.E;
+        const double geo = (sizeof(T) == 4 && map_geometry(m->L.N)) ? 32.0 * m->L.E
+                                                                    : (kind == K_RES ? 4.0 : 3.0) * sizeof(T) * EN2;
         m->kt_bytes[ti] = (cfv + (cap ? 5.0 : 10.0) * nc) * sizeof(T) * EN2 + geo + (dscale ? sizeof(float) * EN2 : 0.0);
     }
     KrylovScalars* buf[2] = {m->sc, m->sc_alt};
@@ -1003,7 +1011,8 @@ static Results minres_fused(sem_mesh* m, int kind, const T* cf, double alpha, co
         // per component
         const double EN2 = (double)m->L.E * m->L.N * m->L.N;
         const double op_vals = (kind == K_RES) ? 5 : (kind == K_HELMHOLTZ ? 4 : 2);  // vectors and coefficient
-        const double geo = sizeof(T) == 8 ? (kind == K_RES ? 4.0 : 3.0) * sizeof(T) * EN2 : 32.0 * m->L.E;
+        const double geo = (sizeof(T) == 4 && map_geometry(m->L.N)) ? 32.0 * m->L.E
+                                                                    : (kind == K_RES ? 4.0 : 3.0) * sizeof(T) * EN2;
         m->kt_bytes[ti] = (op_vals + 6.0 * nc) * sizeof(T) * EN2 + geo;
         // update kernel: yraw and r2 in, y out, per component
         m->kt_bytes[2 + ti] = 3.0 * nc * sizeof(T) * EN2;
@@ -1228,6 +1237,14 @@ static void autotune_steps(sem_mesh* m, int kind, int steps, std::vector<double>
     // shape: its persistent candidates repeat the one-shot timing of their
     // chunks per workgroup and are not picked
     const bool fused1 = fused_step(m, ti);
+    // the mixed-precision stage solves run the one-launch step on the scaled
+    // system (S A S): time it with a scale, so that the step loads what it
+    // loads there
+    float* ds = nullptr;
+    if (fused1 && kind == K_RES) {
+        ds = m->buf<float>("tune_ds", m->len());
+        launch_fill<float>(ds, 1.0f, m->len(), m->stream);
+    }
     for (int rep = 0; rep < 2; ++rep)  // first pass warms up and allocates
         for (size_t c = 0; c < cand.size(); ++c) {
             if (fused1 && cand[c].second > 0) {
@@ -1245,7 +1262,7 @@ static void autotune_steps(sem_mesh* m, int kind, int steps, std::vector<double>
                 m->kt_us[q] = 0;
             }
             m->iter_hint[7] = steps;
-            minres_fused<T>(m, kind, cf, 0.5, b, x, true, o);
+            minres_fused<T>(m, kind, cf, 0.5, b, x, true, o, ds);
             if (rep == 1)
                 for (int q = 0; q < 2; ++q)
                     us[2 * c + q] = m->kt_us[2 * q + ti] / std::max<long>(1, m->kt_launches[2 * q + ti]);
@@ -2811,7 +2828,7 @@ int sem_element_apply_fp32(sem_mesh* m, int kind, const double* coef, double alp
         OpArgs<float> a{};
         a.in = fin;
         a.out = fout;
-        a.geo = nullptr;
+        a.geo = m->geo32;
         a.qm = m->qm;
         a.cf = f.c32;
         a.alpha = (float)alpha;
@@ -3867,9 +3884,9 @@ int sem_time_element_op(sem_mesh* m, int kind, int precision, int reps, double* 
         const size_t n = (size_t)nc * m->len();
         const int N = m->L.N;
         const double EN2 = (double)m->L.E * N * N;
-        // algorithmic bytes per launch: in FP64 the reference data model (8 B
-        // values); in FP32, 4 B values without the geometric factors, which
-        // the kernels form from each element's 32-byte bilinear map
+        // algorithmic bytes per launch: the reference data model (8 B FP64 or
+        // 4 B FP32 values); where the FP32 kernels form the geometric factors
+        // from each element's 32-byte bilinear map, without the factors
         const double vb = precision ? 4.0 : 8.0;
         double fields, geo;
         switch (kind) {
@@ -3881,7 +3898,8 @@ int sem_time_element_op(sem_mesh* m, int kind, int precision, int reps, double* 
         default: fields = 7, geo = 4; break;           // u, V, J, A, B, C in; y out
         }
         if (bytes_per_launch)
-            *bytes_per_launch = precision ? (fields - geo) * vb * EN2 + 32.0 * m->L.E : fields * vb * EN2;
+            *bytes_per_launch = (precision && map_geometry(N)) ? (fields - geo) * vb * EN2 + 32.0 * m->L.E
+                                                               : fields * vb * EN2;
         hipEvent_t e0, e1;
         SEM_HIP(hipEventCreate(&e0));
         SEM_HIP(hipEventCreate(&e1));
@@ -3892,7 +3910,7 @@ int sem_time_element_op(sem_mesh* m, int kind, int precision, int reps, double* 
             float* cf = m->buf<float>("t_cf32", m->len());
             launch_fill<float>(in, 0.5f, n, m->stream);
             launch_fill<float>(cf, 0.25f, m->len(), m->stream);
-            OpArgs<float> a{in, out, nullptr, cf, 0.1f, 1.0f, m->len(), m->L.nchunks, kind == K_RES ? m->contraction : 0, m->qm};
+            OpArgs<float> a{in, out, m->geo32, cf, 0.1f, 1.0f, m->len(), m->L.nchunks, kind == K_RES ? m->contraction : 0, m->qm};
             launch_element_op_d<float>(N, kind, a, m->D32.data(), m->stream);
             SEM_HIP(hipEventRecord(e0, m->stream));
             for (int r = 0; r < reps; ++r) launch_element_op_d<float>(N, kind, a, m->D32.data(), m->stream);

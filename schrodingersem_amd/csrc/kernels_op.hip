@@ -390,12 +390,13 @@ struct QuadColumn {
     }
 };
 
-// gA, gB, gC of column l of element e: formed from the map in FP32, loaded in FP64
+// gA, gB, gC of column l of element e: formed from the map (FP32 where
+// map_geometry(N)), loaded otherwise
 template <int N, typename T, class C>
 __device__ __forceinline__ void geo_stiff(const T* geo, size_t len, const QuadMap& qm, const C& c, int e, int l,
                                           bool act, T (&ga)[N], T (&gb)[N], T (&gc)[N])
 {
-    if constexpr (sizeof(T) == 4) {
+    if constexpr (sizeof(T) == 4 && map_geometry(N)) {
         QuadColumn<N>(qm, e, l, act).stiff(qm, ga, gb, gc);
     } else {
         c.ld(geo, ga);
@@ -409,7 +410,7 @@ template <int N, typename T, class C>
 __device__ __forceinline__ void geo_mass(const T* geo, size_t len, const QuadMap& qm, const C& c, int e, int l,
                                          bool act, T (&gj)[N])
 {
-    if constexpr (sizeof(T) == 4) QuadColumn<N>(qm, e, l, act).mass(qm, gj);
+    if constexpr (sizeof(T) == 4 && map_geometry(N)) QuadColumn<N>(qm, e, l, act).mass(qm, gj);
     else c.ld(geo + 3 * len, gj);
 }
 
@@ -1049,7 +1050,7 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 3 : 1, 8))) void k_minres_fu
                 for (int i = 0; i < N; ++i) qs[i] = yc[i];
             }
             T z[N];
-            if constexpr (sizeof(T) == 4 && MF == 0) {
+            if constexpr (sizeof(T) == 4 && MF == 0 && map_geometry(N)) {
                 const QuadColumn<N> qc(a.qm, chunk * S::K + el, l, act);
                 stiffness<N>(D, lds[wv][0], lds[wv][1], el, l, act, qs,
                              [&](int i, T& ga, T& gb, T& gc) { qc.at(a.qm, i, ga, gb, gc); }, z);

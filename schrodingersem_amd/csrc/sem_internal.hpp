@@ -107,9 +107,12 @@ inline int op_ncomp(int kind) { return (kind == K_RES || kind == K_F) ? 2 : 1; }
 // are built from four functions of the GLL node that are linear in it,
 //   X_xi(t) = ax + bx z_t, Y_xi(t) = ay + by z_t,
 //   X_eta(t) = cx + dx z_t, Y_eta(t) = cy + dy z_t,
-// so the FP32 kernels form gA, gB, gC, gJ from 8 values per element (32
-// bytes) instead of loading 4 N^2 stored values. FP64 applies load the
-// stored factors.
+// so the FP32 kernels can form gA, gB, gC, gJ from 8 values per element (32
+// bytes) instead of loading 4 N^2 stored values. That trades loads for about
+// 20 FP32 operations per node, which pays where the kernels are bound by
+// memory and occupancy and not by their arithmetic: the degrees for which
+// map_geometry(N) holds (chosen by measurement). The others, and every FP64
+// apply, load the stored factors.
 struct QuadMap {
     const float4* map;  // per device element: (ax, bx, ay, by), (cx, dx, cy, dy); zero for padding and ghosts
     float z[kMaxN];     // GLL nodes
@@ -118,6 +121,17 @@ struct QuadMap {
 
 // the map of corners (xs, ys) (counter-clockwise, Quad.hpp's corner order)
 void quad_map(const double* xs, const double* ys, float* out8);
+
+// FP32 kernels of N nodes per direction form their geometry from the map
+// when bit N of the mask is set. Measured on MI355X, fused Lanczos step of the
+// scaled stage system at about 4.2 M nodes, stored factors vs map (us):
+// N = 3: 80.1 / 70.5, 4: 60.3 / 53.4, 5: 62.2 / 60.1, 6: 62.4 / 63.7,
+// 7: 65.4 / 61.7, 8: 51.3 / 50.2, 9: 64.0 / 71.4, 10: 83.4 / 87.3, and the
+// stored factors stay ahead up to N = 13. So the map for N = 2..5, 7, 8.
+#ifndef SEM_MAPGEO_MASK
+#define SEM_MAPGEO_MASK 0x1BCu
+#endif
+constexpr bool map_geometry(int N) { return ((SEM_MAPGEO_MASK >> N) & 1u) != 0; }
 
 // Arguments of the element-operator kernels. `cf` is the premultiplied
 // coefficient field gJ*c (Helmholtz) or gJ*V (Schrodinger), device layout.
@@ -132,7 +146,7 @@ struct OpArgs {
     size_t len;
     int nchunks;
     int mf;        // FP32 Res only: stiffness contractions on MFMA (0 VALU, 1 bf16, 3 split bf16)
-    QuadMap qm;    // FP32: the geometry (geo unused)
+    QuadMap qm;    // FP32 with map_geometry(N): the geometry
 };
 
 // Device-resident Krylov scalars (FP64), one record per instance.
@@ -188,7 +202,7 @@ struct MinresOpArgs {
     T alpha;
     size_t len;
     int nchunks;
-    QuadMap qm;            // FP32: the geometry (geo unused)
+    QuadMap qm;            // FP32 with map_geometry(N): the geometry
     const uint8_t* obits;  // ownership bits per element
     double* partial;       // per-chunk alpha partial sums (out)
     InstTable it;
@@ -247,7 +261,7 @@ struct MinresFusedArgs {
     T alpha;
     size_t len;
     int nchunks;
-    QuadMap qm;  // FP32: the geometry (geo unused)
+    QuadMap qm;  // FP32 with map_geometry(N): the geometry
     const uint8_t* obits;
     const int* eside;
     const int4* ecq;
