@@ -937,8 +937,11 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 3 : 1, 8))) void k_minres_fu
     // component exchange (Res): with VALU contractions it lives in the wave's
     // own transpose buffers (2 S::LDS >= 64 N values), free until the
     // stiffness apply that follows the exchange's second barrier; the LDS a
-    // workgroup holds decides how many fit on a CU
-    constexpr bool XALIAS = MF == 0;
+    // workgroup holds decides how many fit on a CU. Form 3 keeps it in its own
+    // buffer instead and reads both components back from there after the
+    // stiffness apply, so they are not held in registers across it (its
+    // occupancy is set by registers, not LDS).
+    constexpr bool XALIAS = MF == 0 && FORM != 3;
     __shared__ T xch_own[(NC == 2 && !XALIAS) ? NW : 1][(NC == 2 && !XALIAS) ? 64 * N : 1];
     static_assert(!XALIAS || 2 * S::LDS >= 64 * N, "exchange does not fit the transpose buffers");
     __shared__ double wpart[3][NW];
@@ -1036,9 +1039,11 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 3 : 1, 8))) void k_minres_fu
 #pragma unroll
                 for (int i = 0; i < N; ++i) sv[i] = act ? (T)a.dscale[base + (size_t)i * S::KN] : T(0);
             }
+            T* mine = nullptr;   // this wave's y (scaled), then the other wave's
+            T* other = nullptr;
             if constexpr (NC == 2) {
-                T* mine = XALIAS ? &lds[wv][0][0] : xch_own[wv];
-                T* other = XALIAS ? &lds[sub * NC + (1 - w)][0][0] : xch_own[sub * NC + (1 - w)];
+                mine = XALIAS ? &lds[wv][0][0] : xch_own[wv];
+                other = XALIAS ? &lds[sub * NC + (1 - w)][0][0] : xch_own[sub * NC + (1 - w)];
 #pragma unroll
                 for (int i = 0; i < N; ++i) mine[i * 64 + lane] = scl ? yc[i] * sv[i] : yc[i];
                 __syncthreads();
@@ -1061,24 +1066,35 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 3 : 1, 8))) void k_minres_fu
                 else stiffness<N>(D, lds[wv][0], lds[wv][1], el, l, act, qs, ga, gb, gc, z);
             }
             if constexpr (KIND == K_RES) {
+                // u = A (S y) on this component, z = S u; y.z = (S y).u
                 T cf[N], gj[N];
                 c.ld(a.cf, cf);
                 geo_mass<N>(a.geo, len, a.qm, c, chunk * S::K + el, l, act, gj);
 #pragma unroll
                 for (int i = 0; i < N; ++i) {
-                    const T t = a.alpha * (z[i] + cf[i] * qs[i]);
-                    const T yo = scl ? yc[i] * sv[i] : yc[i];
-                    z[i] = (w == 0) ? gj[i] * yo + t : t - gj[i] * yo;
-                    if (scl) z[i] = z[i] * sv[i];
+                    T qv, yo;
+                    if constexpr (!XALIAS) {
+                        qv = other[i * 64 + lane];
+                        yo = mine[i * 64 + lane];
+                    } else {
+                        qv = qs[i];
+                        yo = scl ? yc[i] * sv[i] : yc[i];
+                    }
+                    const T t = a.alpha * (z[i] + cf[i] * qv);
+                    const T u = (w == 0) ? gj[i] * yo + t : t - gj[i] * yo;
+                    p1 += (double)yo * (double)u;
+                    z[i] = scl ? u * sv[i] : u;
                 }
-            } else if constexpr (KIND == K_HELMHOLTZ) {
-                T cf[N];
-                c.ld(a.cf, cf);
+            } else {
+                if constexpr (KIND == K_HELMHOLTZ) {
+                    T cf[N];
+                    c.ld(a.cf, cf);
 #pragma unroll
-                for (int i = 0; i < N; ++i) z[i] = cf[i] * qs[i] + z[i];
+                    for (int i = 0; i < N; ++i) z[i] = cf[i] * qs[i] + z[i];
+                }
+#pragma unroll
+                for (int i = 0; i < N; ++i) p1 += (double)yc[i] * (double)z[i];
             }
-#pragma unroll
-            for (int i = 0; i < N; ++i) p1 += (double)yc[i] * (double)z[i];
             c.st(a.z_out + w * len, z);
         }
         if (pend) {
