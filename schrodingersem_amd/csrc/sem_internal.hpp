@@ -125,11 +125,12 @@ void quad_map(const double* xs, const double* ys, float* out8);
 // FP32 kernels of N nodes per direction form their geometry from the map
 // when bit N of the mask is set. Measured on MI355X, fused Lanczos step of the
 // scaled stage system at about 4.2 M nodes, stored factors vs map (us):
-// N = 3: 80.1 / 70.5, 4: 60.3 / 53.4, 5: 62.2 / 60.1, 6: 62.4 / 63.7,
-// 7: 65.4 / 61.7, 8: 51.3 / 50.2, 9: 64.0 / 71.4, 10: 83.4 / 87.3, and the
-// stored factors stay ahead up to N = 13. So the map for N = 2..5, 7, 8.
+// N = 3: 78.7 / 71.1, 4: 60.8 / 52.9, 5: 62.9 / 55.5, 6: 64.2 / 58.5,
+// 7: 61.9 / 61.6, 8: 51.3 / 48.0, 9: 59.5 / 61.5, 10: 76.4 / 88.0,
+// 11: 93.0 / 104.4, 12: 112.5 / 108.8, 13: 129.5 / 142.2. So the map up to
+// N = 8; above it the kernels are bound by arithmetic and registers.
 #ifndef SEM_MAPGEO_MASK
-#define SEM_MAPGEO_MASK 0x1BCu
+#define SEM_MAPGEO_MASK 0x1FCu
 #endif
 constexpr bool map_geometry(int N) { return ((SEM_MAPGEO_MASK >> N) & 1u) != 0; }
 
