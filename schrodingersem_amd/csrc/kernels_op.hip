@@ -979,9 +979,12 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 3 : 1, 8))) void k_minres_fu
         // the recurrence's sums are loaded first, so that its wait does not
         // include the gather
         const bool rec = !EXT && fresh && !a.first && a.step >= 2 && !sc.done;
-        // all of a 128 x 128 mesh's sums in one round of loads up to N = 10;
-        // fewer registers held across the gather for larger elements
-        GroupSum3<NW, (N <= 10 ? 12 : 4)> gs;
+        // four loads per lane and array: the in-kernel reduction serves step
+        // mode 1, meshes (instances) of under 1024 chunks, so one or two
+        // rounds; more loads per lane held registers across the gather (155
+        // instead of 112 VGPRs at N = 8) for no gain. The order of the
+        // additions does not depend on the loads per round.
+        GroupSum3<NW, 4> gs;
         if (rec) gs.issue(a.prev_sums + ci.y, nch, ci.z - ci.y);
         ColumnGather<N, T> g;
         T r2[N], r3[N];  // y_{i-2}, y_{i-3} of this component

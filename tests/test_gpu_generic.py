@@ -139,7 +139,18 @@ def test_preconditioned_mesh_solvers(sem, orc):
     o = orc.Mesh.from_arrays(6, xy, q)
     X, Y = m.coords()
     F = np.sin(3 * X) * np.cos(2 * Y) + X
-    uo, ok, ito, _ = o.poisson(np.zeros(m.size), F, 10 * m.dof(), 1e-13)
+    # the oracle's iteration counts come from one thread: its OpenMP
+    # reductions sum in a schedule-dependent order, and on this graded mesh
+    # the indefinite MINRES count moved between 1182 and 1251 from run to run
+    # at 16 threads (1253 at one thread, every time)
+    threads = orc.get_threads()
+    orc.set_threads(1)
+    try:
+        uo, ok, ito, _ = o.poisson(np.zeros(m.size), F, 10 * m.dof(), 1e-13)
+        c = np.full(m.size, -40.0 - np.sqrt(2.0))
+        uh, okh, itoh, _ = o.helmholtz(np.zeros(m.size), c, F, 20 * m.dof(), 1e-12)
+    finally:
+        orc.set_threads(threads)
     assert ok
     its = {}
     for pc in (sem.PRECOND_IDENTITY, sem.PRECOND_JACOBI):
@@ -154,8 +165,7 @@ def test_preconditioned_mesh_solvers(sem, orc):
     assert abs(its[sem.PRECOND_IDENTITY] - ito) <= max(2, ito // 100)
     assert its[sem.PRECOND_JACOBI] < its[sem.PRECOND_IDENTITY], its
     # indefinite Helmholtz (tests/helmholtz.cpp's c) with |diag| Jacobi
-    c = np.full(m.size, -40.0 - np.sqrt(2.0))
-    uo, ok, ito, _ = o.helmholtz(np.zeros(m.size), c, F, 20 * m.dof(), 1e-12)
+    uo, ok, ito = uh, okh, itoh
     assert ok
     hits = {}
     for pc in (sem.PRECOND_IDENTITY, sem.PRECOND_JACOBI):
